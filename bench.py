@@ -1,0 +1,175 @@
+"""Throughput of the explicit finite-volume step on one MI355X.
+
+Workload: the periodic Navier-Stokes case on the unit square, 1024x1024
+quads split into 2,097,152 triangles (seeded +-10% node jitter), uniform
+Mach-0.5 flow at 30 degrees plus a Gaussian density/pressure pulse and 1e-3
+velocity noise; mu = 1e-4, Pr = 0.72, gamma = 1.4; least-squares gradients,
+exact Godunov flux, SSP-RK3, CFL 0.4 with the spectral time step fixed at
+init.  A step = one full time step (three RK stages) over all cells; the
+metric is cell-updates/s with the state resident in HBM.
+
+Prints one JSON line.  --gpus above 1 exits: this build targets one GPU.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(REPO, "cfd-2d_amd"), os.path.join(REPO, "oracle")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "FP64 cell-updates/s (2.1M-tri NS) + % HBM peak; batched/serial and adjoint/fwd ratios"
+
+# algorithmic bytes per launch (state arrays touched once per pass, neighbour
+# reuse perfect): gradient 156 B/cell with LSQ (164 with Green-Gauss), edge
+# flux 128 B/edge + 96 B/cell, update 212 B/cell
+def alg_bytes(kernel, nc, ne, lsq=True):
+    if kernel == "gradient":
+        return (156 if lsq else 164) * nc
+    if kernel == "edge_flux":
+        return 128 * ne + 96 * nc
+    if kernel == "update":
+        return 212 * nc
+    return 0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(case_fn, steps, threads=1):
+    import oracle
+    c = case_fn()
+    t0 = time.perf_counter()
+    s = oracle.OracleSolver(c, threads=threads)
+    setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    s.run(steps)
+    dt = time.perf_counter() - t0
+    return dict(value=s.nc * steps / dt, unit="cell-updates/s", cores=threads, kind="port",
+                sample="%d step(s) of the same %d-cell workload on %d host thread(s) (C restatement, gcc -O2)"
+                % (steps, s.nc, threads), seconds=round(dt, 3), setup_seconds=round(setup, 3))
+
+
+def reference_cpu(steps=100):
+    """The reference's own FVM_TVD (compiled from its sources into
+    oracle/_ref) on the 8,000-cell shock tube, where it can run."""
+    exe = os.path.join(REPO, "oracle", "_ref", "ref_harness")
+    if not os.path.exists(exe):
+        return None
+    from cfd2d import meshgen, unv
+    with tempfile.TemporaryDirectory() as d:
+        m = meshgen.rect_mesh(200, 20, 1.0, 0.2, seed=1)
+        unv.write_unv(m, os.path.join(d, "mesh.unv"))
+        regions = [dict(name="cells_left", Vx=0.0, Vy=0.0, T=1.0, P=1.0),
+                   dict(name="cells_right", Vx=0.0, Vy=0.0, T=1.0, P=0.5)]
+        bnds = [dict(name="edge_bottom", type="BOUND_WALL_SLIP"), dict(name="edge_top", type="BOUND_WALL_SLIP"),
+                dict(name="edge_left", type="BOUND_OUTLET"), dict(name="edge_right", type="BOUND_OUTLET")]
+        with open(os.path.join(d, "task.xml"), "w") as f:
+            f.write(unv.task_xml("mesh.unv", regions=regions, boundaries=bnds, step_max=steps))
+        try:
+            out = subprocess.run([exe, "time", "task.xml", str(steps)], cwd=d, capture_output=True, text=True,
+                                 timeout=120, check=True).stdout
+        except (subprocess.SubprocessError, OSError) as e:
+            log("reference timing skipped: %s" % e)
+            return None
+    line = [l for l in out.splitlines() if l.startswith("{")][-1]
+    r = json.loads(line)
+    return dict(value=r["cell_updates_per_s"], unit="cell-updates/s", cores=1, kind="reference",
+                sample="%d steps of the reference FVM_TVD (GG, exact Godunov, RK2) on the 8,000-cell tube, g++ -O2"
+                % steps)
+
+
+def load_traffic(workload):
+    """HBM bytes per launch of each kernel from the committed rocprofv3 PMC summary."""
+    path = os.path.join(REPO, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t.get(workload)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=1024, help="quads per side of the periodic square")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="replay each step as a recorded HIP graph")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from cfd2d import case as cases
+    from cfd2d import native
+
+    if native.device_count() < 1:
+        sys.exit("bench.py: no HIP device")
+    t0 = time.perf_counter()
+    case_fn = lambda: cases.periodic_pulse(args.n, seed=2)
+    c = case_fn()
+    s = native.GpuSolver(c)
+    log("setup %.1f s: %d cells, %d edges" % (time.perf_counter() - t0, s.nc, s.ne))
+    s.run(args.warmup, graph=args.graph)
+    s.timing(True)
+    s.sync()
+    t0 = time.perf_counter()
+    if args.graph:
+        s.step_graph(args.steps)
+    else:
+        s.step(args.steps)
+    s.sync()
+    dt = time.perf_counter() - t0
+    tim = s.timing_read()
+    s.timing(False)
+    hits, nonfinite = s.status()
+    s.close()
+    value = s.nc * args.steps / dt
+    per = {}
+    for k, (ms, n) in tim.items():
+        if n:
+            avg = ms / n
+            b = alg_bytes(k, s.nc, s.ne)
+            per[k] = dict(avg_ms=round(avg, 5), launches=n, alg_bytes=b,
+                          gbs=round(b / (avg * 1e-3) / 1e9, 1) if b else None)
+    dom = max(per, key=lambda k: per[k]["avg_ms"] * per[k]["launches"])
+    d = per[dom]
+    traffic = load_traffic("periodic_ns_%d" % args.n)
+    roof = dict(bound="hbm", kernel=dom, achieved=d["gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(d["gbs"] / HBM_PEAK_GBS, 4) if d["gbs"] else None,
+                traffic=(traffic or {}).get(dom), alg_bytes_per_launch=d["alg_bytes"], avg_ms=d["avg_ms"])
+    step_bytes = 1968 * s.nc  # per cell-update: 3 stages x (464 + 128 e), e = 1.5
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded jittered periodic triangulation, Gaussian pulse + seeded velocity noise)",
+        "config": {"workload": "periodic Navier-Stokes, %d triangles (%dx%d quads x 2), LSQ gradients, exact "
+                               "Godunov flux, SSP-RK3, CFL 0.4 spectral dt, mu=1e-4, Pr=0.72" % (s.nc, args.n, args.n),
+                   "cells": s.nc, "edges": s.ne, "graph": bool(args.graph)},
+        "roofline": roof,
+        "step_hbm_frac": round(step_bytes * args.steps / dt / 1e9 / HBM_PEAK_GBS, 4),
+        "kernels": per,
+        "newton_cap_hits": hits, "nonfinite": nonfinite,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(case_fn, args.cpu_steps)
+        ref = reference_cpu()
+        if ref:
+            out["cpu_reference"] = ref
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

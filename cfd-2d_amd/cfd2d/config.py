@@ -132,6 +132,8 @@ class SolverConfig:
             st.Cp[i] = m.Cp
             st.K[i] = m.K
             st.ML[i] = m.ML
+        if not hasattr(st, "bc_type"):
+            return st  # the library takes boundary objects as separate arrays
         bcs = [b for b in self.boundaries if b.type != "BOUND_PERIODIC"]
         if len(bcs) > MAXBC:
             raise ValueError("at most %d boundary conditions" % MAXBC)

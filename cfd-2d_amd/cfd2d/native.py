@@ -1,0 +1,338 @@
+"""ctypes binding of libcfd2d.so (include/cfd2d.h).
+
+The library is built in-tree by cfd-2d_amd/csrc/Makefile.  There is no
+fallback: if the library or a HIP device is missing, the calls fail loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcfd2d.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+
+K_GRADIENT, K_EDGE_FLUX, K_UPDATE, K_REMEDIATE, K_TAU, K_PRIM = range(6)
+KERNEL_NAMES = ["gradient", "edge_flux", "update", "remediate", "tau", "prim"]
+
+
+class Params(C.Structure):
+    _fields_ = [("flux", C.c_int), ("gradient", C.c_int), ("limiter", C.c_int), ("rk_stages", C.c_int),
+                ("viscous", C.c_int), ("steady", C.c_int), ("cfl_mode", C.c_int), ("max_newton", C.c_int),
+                ("cfl", C.c_double), ("tau", C.c_double), ("gam_flux", C.c_double),
+                ("ro_min", C.c_double), ("ro_max", C.c_double), ("p_min", C.c_double),
+                ("p_max", C.c_double), ("u_max", C.c_double), ("prandtl", C.c_double),
+                ("nmat", C.c_int), ("M", C.c_double * 8), ("Cp", C.c_double * 8), ("K", C.c_double * 8),
+                ("ML", C.c_double * 8)]
+
+
+class Cfd2dError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("cfd2d error %d: %s" % (code, msg))
+        self.code = code
+
+
+# every exported symbol of include/cfd2d.h with its argument types
+SIGNATURES = {
+    "cfd2d_error_string": (C.c_char_p, [C.c_int]),
+    "cfd2d_last_error": (C.c_char_p, []),
+    "cfd2d_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "cfd2d_mesh_create": (C.c_int, [C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, _ip, C.POINTER(C.c_void_p)]),
+    "cfd2d_mesh_read": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    "cfd2d_mesh_set_groups": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_char_p)]),
+    "cfd2d_mesh_set_cell_groups": (C.c_int, [C.c_void_p, _ip]),
+    "cfd2d_mesh_periodic": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double]),
+    "cfd2d_mesh_concat": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "cfd2d_mesh_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "cfd2d_mesh_nodes": (C.c_int, [C.c_void_p, _dp]),
+    "cfd2d_mesh_group_count": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    "cfd2d_mesh_group_name": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_char_p, C.c_int]),
+    "cfd2d_mesh_get_cell_groups": (C.c_int, [C.c_void_p, _ip]),
+    "cfd2d_mesh_case_offsets": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_void_p, C.c_void_p]),
+    "cfd2d_mesh_export": (C.c_int, [C.c_void_p, _dp, _ip, _ip, _dp, _ip, _ip, _dp]),
+    "cfd2d_mesh_destroy": (None, [C.c_void_p]),
+    "cfd2d_ctx_create": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int, _ip, _dp, _ip, _ip,
+                                   C.POINTER(C.c_void_p)]),
+    "cfd2d_ctx_destroy": (None, [C.c_void_p]),
+    "cfd2d_set_state": (C.c_int, [C.c_void_p, _dp]),
+    "cfd2d_get_state": (C.c_int, [C.c_void_p, _dp]),
+    "cfd2d_set_state_device": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cfd2d_get_state_device": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cfd2d_set_flags": (C.c_int, [C.c_void_p, _up]),
+    "cfd2d_get_flags": (C.c_int, [C.c_void_p, _up]),
+    "cfd2d_calc_tau": (C.c_int, [C.c_void_p]),
+    "cfd2d_get_tau": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_void_p]),
+    "cfd2d_set_tau": (C.c_int, [C.c_void_p, C.c_double]),
+    "cfd2d_gradients": (C.c_int, [C.c_void_p, _dp]),
+    "cfd2d_edge_fluxes": (C.c_int, [C.c_void_p, _dp]),
+    "cfd2d_residual": (C.c_int, [C.c_void_p, _dp]),
+    "cfd2d_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "cfd2d_step_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "cfd2d_sync": (C.c_int, [C.c_void_p]),
+    "cfd2d_status": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_int)]),
+    "cfd2d_wall_force": (C.c_int, [C.c_void_p, _ip, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "cfd2d_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "cfd2d_timing_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
+    "cfd2d_ctx_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "cfd2d_ctx_stream": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libcfd2d.so is not built (run __graft_entry__.build() or make -C cfd-2d_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().cfd2d_last_error().decode(errors="replace")
+        raise Cfd2dError(rc, msg)
+
+
+def device_count():
+    n = C.c_int(0)
+    rc = lib().cfd2d_device_count(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+class Mesh:
+    """A host mesh (cfd2d_mesh) built from a TriMesh or read from files."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+        nn, nc, ne = C.c_int(), C.c_int(), C.c_int()
+        check(lib().cfd2d_mesh_counts(self._h, C.byref(nn), C.byref(nc), C.byref(ne)))
+        self.nn, self.nc, self.ne = nn.value, nc.value, ne.value
+
+    @classmethod
+    def from_trimesh(cls, m, periodic=()):
+        L = lib()
+        h = C.c_void_p()
+        xy = np.ascontiguousarray(m.xy, np.float64).ravel()
+        tri = np.ascontiguousarray(m.tri, np.int32).ravel()
+        be = np.ascontiguousarray(m.bedges, np.int32).ravel()
+        bg = np.ascontiguousarray(m.bedge_group, np.int32)
+        if be.size == 0:
+            be = np.zeros(2, np.int32)
+            bg = np.zeros(1, np.int32)
+        check(L.cfd2d_mesh_create(m.n_nodes, xy, m.n_cells, tri, m.n_bedges, be, bg, C.byref(h)))
+        names = (C.c_char_p * len(m.edge_group_names))(*[s.encode() for s in m.edge_group_names])
+        check(L.cfd2d_mesh_set_groups(h, 0, len(m.edge_group_names), names))
+        cnames = (C.c_char_p * len(m.cell_group_names))(*[s.encode() for s in m.cell_group_names])
+        check(L.cfd2d_mesh_set_groups(h, 1, len(m.cell_group_names), cnames))
+        check(L.cfd2d_mesh_set_cell_groups(h, np.ascontiguousarray(m.cell_group, np.int32)))
+        for (ga, gb, sx, sy) in periodic:
+            check(L.cfd2d_mesh_periodic(h, ga, gb, sx, sy))
+        return cls(h)
+
+    @classmethod
+    def read(cls, files_type, name):
+        h = C.c_void_p()
+        check(lib().cfd2d_mesh_read(files_type.encode(), name.encode(), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def concat(cls, meshes):
+        arr = (C.c_void_p * len(meshes))(*[m._h.value for m in meshes])
+        h = C.c_void_p()
+        check(lib().cfd2d_mesh_concat(len(meshes), arr, C.byref(h)))
+        return cls(h)
+
+    def group_names(self, kind=0):
+        n = C.c_int()
+        check(lib().cfd2d_mesh_group_count(self._h, kind, C.byref(n)))
+        out = []
+        buf = C.create_string_buffer(256)
+        for i in range(n.value):
+            check(lib().cfd2d_mesh_group_name(self._h, kind, i, buf, 256))
+            out.append(buf.value.decode())
+        return out
+
+    def cell_groups(self):
+        g = np.zeros(self.nc, np.int32)
+        check(lib().cfd2d_mesh_get_cell_groups(self._h, g))
+        return g
+
+    def nodes(self):
+        xy = np.zeros(2 * self.nn)
+        check(lib().cfd2d_mesh_nodes(self._h, xy))
+        return xy.reshape(self.nn, 2)
+
+    def case_offsets(self):
+        n = C.c_int()
+        check(lib().cfd2d_mesh_case_offsets(self._h, C.byref(n), None, None))
+        co = np.zeros(n.value + 1, np.int32)
+        eo = np.zeros(n.value + 1, np.int32)
+        check(lib().cfd2d_mesh_case_offsets(self._h, C.byref(n), co.ctypes.data_as(C.c_void_p),
+                                            eo.ctypes.data_as(C.c_void_p)))
+        return co, eo
+
+    def export(self):
+        nc, ne = self.nc, self.ne
+        out = dict(cell_geom=np.zeros((nc, 4)), cell_nodes=np.zeros((nc, 3), np.int32),
+                   cell_edges=np.zeros((nc, 3), np.int32), edge_geom=np.zeros((ne, 9)),
+                   edge_int=np.zeros((ne, 5), np.int32), edge_group=np.zeros(ne, np.int32),
+                   edge_shift=np.zeros((ne, 2)))
+        check(lib().cfd2d_mesh_export(self._h, out["cell_geom"].ravel(), out["cell_nodes"].ravel(),
+                                      out["cell_edges"].ravel(), out["edge_geom"].ravel(),
+                                      out["edge_int"].ravel(), out["edge_group"], out["edge_shift"].ravel()))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value and _lib is not None:
+            _lib.cfd2d_mesh_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Context:
+    """A device context (cfd2d_ctx): state, passes and time steps on the GPU."""
+
+    def __init__(self, mesh: Mesh, config, group_bc, cell_mat=None):
+        L = lib()
+        self.mesh = mesh
+        p = Params()
+        config.fill(p)
+        bcs = [b for b in config.boundaries if b.type != "BOUND_PERIODIC"]
+        bt = np.array([b.code for b in bcs] or [0], np.int32)
+        bp = np.array([list(b.params()) for b in bcs] or [[0, 0, 0, 0]], np.float64).ravel()
+        gbc = np.ascontiguousarray(group_bc if len(group_bc) else [-1], np.int32)
+        cm = np.ascontiguousarray(cell_mat if cell_mat is not None else np.zeros(mesh.nc), np.int32)
+        h = C.c_void_p()
+        check(L.cfd2d_ctx_create(mesh._h, C.byref(p), len(bcs), bt, bp, gbc, cm, C.byref(h)))
+        self._h = h
+        self.nc, self.ne = mesh.nc, mesh.ne
+
+    def set_state(self, U):
+        check(lib().cfd2d_set_state(self._h, np.ascontiguousarray(U, np.float64).ravel()))
+
+    def set_state_device(self, ptr):
+        check(lib().cfd2d_set_state_device(self._h, C.c_void_p(ptr)))
+
+    def get_state_device(self, ptr):
+        check(lib().cfd2d_get_state_device(self._h, C.c_void_p(ptr)))
+
+    def state(self):
+        U = np.zeros(4 * self.nc)
+        check(lib().cfd2d_get_state(self._h, U))
+        return U.reshape(4, self.nc)
+
+    def flags(self):
+        f = np.zeros(self.nc, np.uint32)
+        check(lib().cfd2d_get_flags(self._h, f))
+        return f
+
+    def set_flags(self, f):
+        check(lib().cfd2d_set_flags(self._h, np.ascontiguousarray(f, np.uint32)))
+
+    def calc_tau(self):
+        check(lib().cfd2d_calc_tau(self._h))
+
+    def tau(self):
+        t = C.c_double()
+        ct = np.zeros(self.nc)
+        check(lib().cfd2d_get_tau(self._h, C.byref(t), ct.ctypes.data_as(C.c_void_p)))
+        return t.value, ct
+
+    def set_tau(self, tau):
+        check(lib().cfd2d_set_tau(self._h, tau))
+
+    def gradients(self):
+        g = np.zeros(8 * self.nc)
+        check(lib().cfd2d_gradients(self._h, g))
+        return g.reshape(8, self.nc)
+
+    def edge_fluxes(self):
+        f = np.zeros(4 * self.ne)
+        check(lib().cfd2d_edge_fluxes(self._h, f))
+        return f.reshape(self.ne, 4)
+
+    def residual(self):
+        r = np.zeros(4 * self.nc)
+        check(lib().cfd2d_residual(self._h, r))
+        return r.reshape(4, self.nc)
+
+    def step(self, n=1):
+        check(lib().cfd2d_step(self._h, int(n)))
+
+    def step_graph(self, n=1):
+        check(lib().cfd2d_step_graph(self._h, int(n)))
+
+    def sync(self):
+        check(lib().cfd2d_sync(self._h))
+
+    def status(self):
+        hits, nonfinite = C.c_longlong(), C.c_int()
+        check(lib().cfd2d_status(self._h, C.byref(hits), C.byref(nonfinite)))
+        return hits.value, nonfinite.value
+
+    def wall_force(self, edge_mask):
+        fx, fy = C.c_double(), C.c_double()
+        check(lib().cfd2d_wall_force(self._h, np.ascontiguousarray(edge_mask, np.int32), C.byref(fx), C.byref(fy)))
+        return fx.value, fy.value
+
+    def timing(self, enable=True):
+        check(lib().cfd2d_timing(self._h, 1 if enable else 0))
+
+    def timing_read(self):
+        out = {}
+        for k, name in enumerate(KERNEL_NAMES):
+            ms, n = C.c_double(), C.c_longlong()
+            check(lib().cfd2d_timing_read(self._h, k, C.byref(ms), C.byref(n)))
+            out[name] = (ms.value, n.value)
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value and _lib is not None:
+            _lib.cfd2d_ctx_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class GpuSolver:
+    """A cfd2d.case.Case on the GPU: mesh build, boundary mapping, initial
+    state and time step as FVM_TVD::init does them (fvm_tvd.cpp:7-213)."""
+
+    def __init__(self, case, calc_tau=True):
+        self.case = case
+        self.mesh = Mesh.from_trimesh(case.mesh, case.periodic_pairs())
+        self.ctx = Context(self.mesh, case.config, case.group_bc(), case.cell_material())
+        self.nc, self.ne = self.mesh.nc, self.mesh.ne
+        self.ctx.set_state(case.initial_state())
+        if calc_tau:
+            self.ctx.calc_tau()
+
+    def __getattr__(self, name):
+        return getattr(self.ctx, name)
+
+    def run(self, n, graph=False):
+        if graph:
+            self.ctx.step_graph(n)
+        else:
+            self.ctx.step(n)
+        self.ctx.sync()
+
+    def close(self):
+        self.ctx.close()
+        self.mesh.close()

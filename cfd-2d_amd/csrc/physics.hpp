@@ -1,0 +1,331 @@
+// Device physics of the explicit finite-volume step (gfx950, FP64).
+//
+// Each function restates one piece of the reference and says which; the
+// extensions (LAX is the implicit method's, HLLC, Roe, viscous flux,
+// pressure outlet) have no counterpart in FVM_TVD and are defined here.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace cfd2d {
+namespace dev {
+
+constexpr double GAS_R = 8.314472;  // src/global.cpp:6
+
+struct MatV {
+	double M, Cv, gam, mu, kc;  // molar mass, Cv, gamma, viscosity, conductivity
+};
+
+struct Prim {
+	double r, p, u, v;
+};
+
+// convertConsToPar + Material::URS(0) (src/methods/fvm_tvd.cpp:803-813,
+// src/global.cpp:15-18): u = ru/r, v = rv/r, E = re/r, e = E - |u|^2/2,
+// p = r e (gamma - 1)
+__device__ __forceinline__ Prim cons_to_prim(double ro, double ru, double rv, double re, const MatV& m)
+{
+	Prim q;
+	q.r = ro;
+	q.u = ru / ro;
+	q.v = rv / ro;
+	double E = re / ro;
+	double e = E - 0.5 * (q.u * q.u + q.v * q.v);
+	q.p = q.r * e * (m.gam - 1);
+	return q;
+}
+
+// Material::URS(1) temperature, e = p / (r (gamma - 1)), T = e / Cv
+__device__ __forceinline__ double temperature(const Prim& q, const MatV& m)
+{
+	double e = q.p / (q.r * (m.gam - 1));
+	return e / m.Cv;
+}
+
+enum { BND_INLET = 1, BND_OUTLET = 2, BND_WALL = 3, BND_PRESSURE_OUTLET = 4 };
+
+// FVM_TVD::boundaryCond + CFDBoundary::run (src/methods/fvm_tvd.cpp:694-728,
+// src/bnd_cond.cpp:75-110): inlet sets u, v, T, p; outlet copies; walls
+// mirror the normal velocity; then URS(2) gives rho = p M / (T gR) with the
+// cell-centre T (Tc) of the inside state.  Only (r, p, u, v) of the ghost
+// reach the flux.  Pressure outlet (extension): rho, u, v copied, p fixed
+// while |u.n| < c of the cell centre (czc).
+__device__ __forceinline__ Prim ghost_state(int type, const double* par, double nx, double ny, const Prim& pL,
+                                            double Tc, double czc, const MatV& m)
+{
+	Prim g = pL;
+	double T = Tc;
+	if (type == BND_INLET) {
+		g.u = par[0];
+		g.v = par[1];
+		T = par[2];
+		g.p = par[3];
+	} else if (type == BND_PRESSURE_OUTLET) {
+		double un = pL.u * nx + pL.v * ny;
+		if (fabs(un) < czc) g.p = par[0];
+		return g;
+	} else if (type != BND_OUTLET) {
+		double Un = pL.u * nx + pL.v * ny;
+		double Vx = nx * Un * 2.0;
+		double Vy = ny * Un * 2.0;
+		g.u = pL.u - Vx;
+		g.v = pL.v - Vy;
+	}
+	g.r = g.p * m.M / (T * GAS_R);
+	return g;
+}
+
+// rim_orig (src/global.cpp:232-405): exact Riemann solver, Newton on p*
+// from the acoustic guess with the P >= 1e-5 clamp, stopping when
+// |DU - F1 - F2| <= 1e-5 is seen before the update; powers as
+// exp(log(x) k).  max_iter > 0 caps the loop and reports it in *hit.
+// Outputs the state sampled at x/t = 0: density RI, internal energy EI,
+// pressure PI, normal velocity UI, tangential velocity VI.
+__device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& UI, double& VI, double RB,
+                                    double PB, double UB, double VB, double RE, double PE, double UE, double VE,
+                                    double GAM, int max_iter, int& hit)
+{
+	const double AGAM = (GAM - 1.0);
+	const double DGAM = (2.0 / AGAM);
+	const double GGAM = (sqrt(GAM * AGAM));
+	const double HGAM = (AGAM / 2.0);
+	const double FGAM = (3.0 * GAM - 1.0);
+	const double OGAM = (AGAM / (2.0 * GAM));
+	const double QGAM = (GAM + 1.0);
+	const double PGAM = (QGAM / (2.0 * GAM));
+	const double RGAM = (4.0 * GAM);
+	const double SGAM = (GAM * AGAM);
+	const double TGAM = (QGAM / 2.0);
+	const double eps = 1.0e-5;
+
+	double US = 0.0, UF = 0.0;
+	double RF, RS, EF, ES, SBL, SFL, SSL, SEL;
+	double CB = sqrt(GAM * PB / RB);
+	double CE = sqrt(GAM * PE / RE);
+	double EB = CB * CB / SGAM;
+	double EE = CE * CE / SGAM;
+	double RCB = RB * CB;
+	double RCE = RE * CE;
+	double DU = UB - UE;
+	hit = 0;
+	if (DU < -2.0 * (CB + CE) / AGAM) {
+		RF = 0.0; RS = 0.0; EF = 0.0; ES = 0.0;
+		SBL = UB - CB;
+		SFL = UB + 2.0 * CB / AGAM;
+		SSL = UE - 2.0 * CE / AGAM;
+		SEL = UE + CE;
+	} else {
+		double P = (PB * RCE + PE * RCB + DU * RCB * RCE) / (RCB + RCE);
+		int it = 0;
+		for (;;) {
+			if (P < eps) P = eps;
+			double F1, FS1, F2, FS2;
+			double PPB = P / PB;
+			if (PB > P) {
+				double ZFB = CB * exp(log(PPB) * OGAM);
+				F1 = DGAM * (ZFB - CB);
+				FS1 = ZFB / (GAM * P);
+			} else {
+				double PKB = PGAM * PPB + OGAM;
+				double ZNB = RCB * sqrt(PKB);
+				F1 = (P - PB) / ZNB;
+				FS1 = (QGAM * PPB + FGAM) / (RGAM * ZNB * PKB);
+			}
+			double PPE = P / PE;
+			if (PE > P) {
+				double ZFE = CE * exp(log(PPE) * OGAM);
+				F2 = DGAM * (ZFE - CE);
+				FS2 = ZFE / (GAM * P);
+			} else {
+				double PKE = PGAM * PPE + OGAM;
+				double ZNE = RCE * sqrt(PKE);
+				F2 = (P - PE) / ZNE;
+				FS2 = (QGAM * PPE + FGAM) / (RGAM * ZNE * PKE);
+			}
+			double res = DU - F1 - F2;
+			P = P + res / (FS1 + FS2);
+			++it;
+			if (!(fabs(res) > eps)) break;
+			if (max_iter > 0 && it >= max_iter) {
+				hit = 1;
+				break;
+			}
+		}
+		double PPB = P / PB;
+		double PPE = P / PE;
+		double ZFB = CB * exp(log(PPB) * OGAM);
+		double ZFE = CE * exp(log(PPE) * OGAM);
+		if (PB > P) {
+			EF = ZFB * ZFB / SGAM;
+			UF = UB + DGAM * (CB - ZFB);
+			RF = P / (AGAM * EF);
+			SBL = UB - CB;
+			SFL = UF - ZFB;
+		} else {
+			double D = UB - sqrt((TGAM * P + HGAM * PB) / RB);
+			double UBD = UB - D;
+			double RUBD = RB * UBD;
+			RF = RUBD * RUBD / (PB - P + RUBD * UBD);
+			UF = D + RUBD / RF;
+			EF = P / (AGAM * RF);
+			SBL = D;
+			SFL = D;
+		}
+		if (PE > P) {
+			ES = ZFE * ZFE / SGAM;
+			US = UE - DGAM * (CE - ZFE);
+			RS = P / (AGAM * ES);
+			SSL = US + ZFE;
+			SEL = UE + CE;
+		} else {
+			double D = UE + sqrt((TGAM * P + HGAM * PE) / RE);
+			double UED = UE - D;
+			double RUED = RE * UED;
+			RS = RUED * RUED / (PE - P + RUED * UED);
+			US = D + RUED / RS;
+			ES = P / (AGAM * RS);
+			SEL = D;
+			SSL = D;
+		}
+	}
+	if (SEL <= 0.0) {
+		RI = RE; EI = EE; UI = UE; VI = VE;
+	} else if (SBL >= 0.0) {
+		RI = RB; EI = EB; UI = UB; VI = VB;
+	} else if ((SSL >= 0.0) && (SFL <= 0.0)) {
+		if (US >= 0.0) {
+			RI = RF; EI = EF; UI = UF; VI = VB;
+		} else {
+			RI = RS; EI = ES; UI = US; VI = VE;
+		}
+	} else if (SFL > 0.0) {
+		UI = (UB + DGAM * GGAM * sqrt(EB)) / (1 + DGAM);
+		VI = VB;
+		EI = (UI * UI) / SGAM;
+		RI = RB * exp(log(EI / EB) * (1 / AGAM));
+	} else {
+		UI = (UE - DGAM * GGAM * sqrt(EE)) / (1 + DGAM);
+		VI = VE;
+		EI = (UI * UI) / SGAM;
+		RI = RE * exp(log(EI / EE) * (1 / AGAM));
+	}
+	PI = AGAM * EI * RI;
+}
+
+// FVM_TVD::calcFlux, Godunov branch (src/methods/fvm_tvd.cpp:602-622)
+__device__ __forceinline__ void flux_godunov(double* f, const Prim& L, const Prim& R, double nx, double ny,
+                                             double GAM, int max_iter, int& hit)
+{
+	double unl = L.u * nx + L.v * ny;
+	double unr = R.u * nx + R.v * ny;
+	double utl = L.u * ny - L.v * nx;
+	double utr = R.u * ny - R.v * nx;
+	double RI, EI, PI, UN, UT;
+	rim(RI, EI, PI, UN, UT, L.r, L.p, unl, utl, R.r, R.p, unr, utr, GAM, max_iter, hit);
+	double UI = UN * nx + UT * ny;
+	double VI = UN * ny - UT * nx;
+	f[0] = RI * UN;
+	f[1] = f[0] * UI + PI * nx;
+	f[2] = f[0] * VI + PI * ny;
+	f[3] = (RI * (EI + 0.5 * (UI * UI + VI * VI)) + PI) * UN;
+}
+
+// FLUX=LAX as FVM_TVD_IMPLICIT::calcFlux defines it (src/methods/fvm_tvd_implicit.cpp:939-960)
+__device__ __forceinline__ void flux_lax(double* f, const Prim& L, const Prim& R, double nx, double ny, double GAM)
+{
+	double unl = L.u * nx + L.v * ny;
+	double unr = R.u * nx + R.v * ny;
+	double al = fabs(unl) + sqrt(GAM * L.p / L.r);
+	double ar = fabs(unr) + sqrt(GAM * R.p / R.r);
+	double alpha = al > ar ? al : ar;
+	double rol = L.r, rul = L.r * L.u, rvl = L.r * L.v;
+	double rel = L.p / (GAM - 1.0) + 0.5 * L.r * (L.u * L.u + L.v * L.v);
+	double ror = R.r, rur = R.r * R.u, rvr = R.r * R.v;
+	double rer = R.p / (GAM - 1.0) + 0.5 * R.r * (R.u * R.u + R.v * R.v);
+	double frl = rol * unl;
+	double frr = ror * unr;
+	f[0] = 0.5 * (frr + frl - alpha * (ror - rol));
+	f[1] = 0.5 * (frr * R.u + frl * L.u + (R.p + L.p) * nx - alpha * (rur - rul));
+	f[2] = 0.5 * (frr * R.v + frl * L.v + (R.p + L.p) * ny - alpha * (rvr - rvl));
+	f[3] = 0.5 * ((rer + R.p) * unr + (rel + L.p) * unl - alpha * (rer - rel));
+}
+
+// Extension: HLLC (Toro, Riemann Solvers, 3rd ed., 10.4-10.6), Davis wave speeds
+__device__ __forceinline__ void flux_hllc(double* f, const Prim& L, const Prim& R, double nx, double ny, double GAM)
+{
+	double unl = L.u * nx + L.v * ny, unr = R.u * nx + R.v * ny;
+	double cl = sqrt(GAM * L.p / L.r), cr = sqrt(GAM * R.p / R.r);
+	double El = L.p / (GAM - 1.0) + 0.5 * L.r * (L.u * L.u + L.v * L.v);
+	double Er = R.p / (GAM - 1.0) + 0.5 * R.r * (R.u * R.u + R.v * R.v);
+	double sl = fmin(unl - cl, unr - cr);
+	double sr = fmax(unl + cl, unr + cr);
+	if (sl >= 0.0) {
+		f[0] = L.r * unl;
+		f[1] = L.r * L.u * unl + L.p * nx;
+		f[2] = L.r * L.v * unl + L.p * ny;
+		f[3] = (El + L.p) * unl;
+		return;
+	}
+	if (sr <= 0.0) {
+		f[0] = R.r * unr;
+		f[1] = R.r * R.u * unr + R.p * nx;
+		f[2] = R.r * R.v * unr + R.p * ny;
+		f[3] = (Er + R.p) * unr;
+		return;
+	}
+	double ml = L.r * (sl - unl), mr = R.r * (sr - unr);
+	double ss = (R.p - L.p + L.r * unl * (sl - unl) - R.r * unr * (sr - unr)) / (ml - mr);
+	const Prim& K = ss >= 0.0 ? L : R;
+	double sk = ss >= 0.0 ? sl : sr;
+	double unk = ss >= 0.0 ? unl : unr;
+	double Ek = ss >= 0.0 ? El : Er;
+	double k = (ss >= 0.0 ? ml : mr) / (sk - ss);
+	double us0 = k, us1 = k * (K.u + (ss - unk) * nx), us2 = k * (K.v + (ss - unk) * ny);
+	double us3 = k * (Ek / K.r + (ss - unk) * (ss + K.p / (K.r * (sk - unk))));
+	double fk0 = K.r * unk, fk1 = K.r * K.u * unk + K.p * nx, fk2 = K.r * K.v * unk + K.p * ny,
+	       fk3 = (Ek + K.p) * unk;
+	f[0] = fk0 + sk * (us0 - K.r);
+	f[1] = fk1 + sk * (us1 - K.r * K.u);
+	f[2] = fk2 + sk * (us2 - K.r * K.v);
+	f[3] = fk3 + sk * (us3 - Ek);
+}
+
+// Extension: Roe flux on Roe averages (as roe_orig forms them, src/global.cpp:407-441)
+// with Harten's entropy fix, delta = 0.1 c
+__device__ __forceinline__ void flux_roe(double* f, const Prim& L, const Prim& R, double nx, double ny, double GAM)
+{
+	double unl = L.u * nx + L.v * ny, unr = R.u * nx + R.v * ny;
+	double El = L.p / (GAM - 1.0) + 0.5 * L.r * (L.u * L.u + L.v * L.v);
+	double Er = R.p / (GAM - 1.0) + 0.5 * R.r * (R.u * R.u + R.v * R.v);
+	double Hl = (El + L.p) / L.r, Hr = (Er + R.p) / R.r;
+	double sl = sqrt(L.r), sr = sqrt(R.r), w = 1.0 / (sl + sr);
+	double u = (sl * L.u + sr * R.u) * w, v = (sl * L.v + sr * R.v) * w;
+	double H = (sl * Hl + sr * Hr) * w;
+	double q2 = u * u + v * v;
+	double c = sqrt((GAM - 1.0) * (H - 0.5 * q2));
+	double un = u * nx + v * ny, ut = -u * ny + v * nx;
+	double dr = R.r - L.r, dp = R.p - L.p;
+	double dun = unr - unl;
+	double dut = (-R.u * ny + R.v * nx) - (-L.u * ny + L.v * nx);
+	double rr = sl * sr;
+	double a1 = (dp - rr * c * dun) / (2.0 * c * c);
+	double a2 = dr - dp / (c * c);
+	double a3 = rr * dut;
+	double a4 = (dp + rr * c * dun) / (2.0 * c * c);
+	double l1 = fabs(un - c), l2 = fabs(un), l4 = fabs(un + c);
+	double dlt = 0.1 * c;
+	if (l1 < dlt) l1 = (l1 * l1 + dlt * dlt) / (2.0 * dlt);
+	if (l4 < dlt) l4 = (l4 * l4 + dlt * dlt) / (2.0 * dlt);
+	double tx = -ny, ty = nx;
+	double d0 = l1 * a1 + l2 * a2 + l4 * a4;
+	double d1 = l1 * a1 * (u - c * nx) + l2 * a2 * u + l2 * a3 * tx + l4 * a4 * (u + c * nx);
+	double d2 = l1 * a1 * (v - c * ny) + l2 * a2 * v + l2 * a3 * ty + l4 * a4 * (v + c * ny);
+	double d3 = l1 * a1 * (H - un * c) + l2 * a2 * 0.5 * q2 + l2 * a3 * ut + l4 * a4 * (H + un * c);
+	f[0] = 0.5 * (L.r * unl + R.r * unr - d0);
+	f[1] = 0.5 * ((L.r * L.u * unl + L.p * nx) + (R.r * R.u * unr + R.p * nx) - d1);
+	f[2] = 0.5 * ((L.r * L.v * unl + L.p * ny) + (R.r * R.v * unr + R.p * ny) - d2);
+	f[3] = 0.5 * ((El + L.p) * unl + (Er + R.p) * unr - d3);
+}
+
+}  // namespace dev
+}  // namespace cfd2d

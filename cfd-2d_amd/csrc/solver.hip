@@ -1,0 +1,1154 @@
+// Explicit finite-volume step on MI355X (gfx950): kernels, device context
+// and the context half of the C ABI.
+//
+// Data layout (structure of arrays, FP64, cells in the reference's order):
+//   cells  U[4][nc] conserved, U0[4][nc] step start, P[4][nc] primitives
+//          (r, p, u, v) of U, G[8][nc] gradients, dtS[nc] = tau_i / S_i,
+//          S[nc], faces cface[3][nc] = edge*2 + side in ascending edge order,
+//          cnb[3][nc] neighbour across each face (-1 on a boundary),
+//          lsqw[6][nc] least-squares weights, flags[nc]
+//   edges  ec1/ec2 (ec2 = -(bc+1) on a boundary), enx, eny, el,
+//          edl[8][ne] Gauss-point offsets (c1: gp1 x,y gp2 x,y; c2: same),
+//          F[4][ne] = (flux at gp1 + flux at gp2) * l/2, written once
+// Each RK stage is three passes: gradient (cell gather), edge flux (one
+// thread per edge, result written once to F), update (cell gather of its
+// three faces in ascending edge order, then the RK combination).  No
+// floating-point atomics: results do not depend on scheduling.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/cfd2d.h"
+#include "common.hpp"
+#include "physics.hpp"
+
+using namespace cfd2d;
+using namespace cfd2d::dev;
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int NXCD = 8;
+
+struct Bufs {
+	int nc, ne, nblocks, nmat;
+	double *U, *U0, *P, *G, *dtS, *S, *lsqw, *loc, *R;
+	int *cface, *cnb, *cmat;
+	unsigned* flags;
+	int* blockflag;
+	int *ec1, *ec2;
+	double *enx, *eny, *el, *edl, *F;
+	int* bctype;
+	double* bcpar;
+	double* mtab;  // 5 per material: M, Cv, gam, mu, kc
+	unsigned long long* err;  // [0] Newton cap hits, [1] non-finite cells
+};
+
+struct Phys {
+	MatV m0;
+	double gam_flux, cfl;
+	double ro_min, ro_max, p_min, p_max, u_max;
+	int max_newton, cfl_mode, steady;
+};
+
+// Block index remapped so that consecutive index ranges stay on one XCD
+// (workgroups are dispatched round-robin over the 8 XCDs): neighbouring
+// cells/edges then share that XCD's L2.
+__device__ __forceinline__ int xcd_block()
+{
+	int b = blockIdx.x, n = gridDim.x;
+	int full = n / NXCD, rem = n % NXCD;
+	int x = b % NXCD, loc = b / NXCD;
+	int base = x < rem ? x * (full + 1) : rem * (full + 1) + (x - rem) * full;
+	return base + loc;
+}
+
+template <bool MM>
+__device__ __forceinline__ MatV mat_of(const Bufs& b, const Phys& ph, int i)
+{
+	if (!MM) return ph.m0;
+	const double* t = &b.mtab[5 * b.cmat[i]];
+	return MatV{t[0], t[1], t[2], t[3], t[4]};
+}
+
+__device__ __forceinline__ Prim load_prim(const Bufs& b, int i)
+{
+	size_t n = b.nc;
+	return Prim{b.P[i], b.P[n + i], b.P[2 * n + i], b.P[3 * n + i]};
+}
+
+__device__ __forceinline__ void store_prim(const Bufs& b, int i, const Prim& q)
+{
+	size_t n = b.nc;
+	b.P[i] = q.r;
+	b.P[n + i] = q.p;
+	b.P[2 * n + i] = q.u;
+	b.P[3 * n + i] = q.v;
+}
+
+// ghost of cell i's (cell-centre or reconstructed) state across boundary edge e
+template <bool MM>
+__device__ __forceinline__ Prim ghost_of(const Bufs& b, const Phys& ph, int e, int c1, const Prim& inside,
+                                         const Prim& centre)
+{
+	int bc = -b.ec2[e] - 1;
+	MatV m = mat_of<MM>(b, ph, c1);
+	double Tc = temperature(centre, m);
+	double czc = sqrt(m.gam * centre.p / centre.r);
+	return ghost_state(b.bctype[bc], &b.bcpar[4 * bc], b.enx[e], b.eny[e], inside, Tc, czc, m);
+}
+
+// ---------------------------------------------------------------- kernels
+
+template <bool MM>
+__global__ __launch_bounds__(BLOCK) void k_prim(Bufs b, Phys ph)
+{
+	int i = xcd_block() * BLOCK + threadIdx.x;
+	if (i >= b.nc) return;
+	size_t n = b.nc;
+	MatV m = mat_of<MM>(b, ph, i);
+	store_prim(b, i, cons_to_prim(b.U[i], b.U[n + i], b.U[2 * n + i], b.U[3 * n + i], m));
+}
+
+// Barth-Jespersen scaling with minmod(1, r) at the cell's six Gauss points
+// (extension; the oracle's limit_minmod)
+__device__ __forceinline__ void minmod_limit(const Bufs& b, int i, const int* face, const Prim& pi,
+                                             const Prim* nb, double* g)
+{
+	double phi[4] = {pi.r, pi.p, pi.u, pi.v};
+	double lo[4], hi[4];
+	for (int q = 0; q < 4; q++) lo[q] = hi[q] = phi[q];
+	for (int k = 0; k < 3; k++) {
+		double v[4] = {nb[k].r, nb[k].p, nb[k].u, nb[k].v};
+		for (int q = 0; q < 4; q++) {
+			if (v[q] < lo[q]) lo[q] = v[q];
+			if (v[q] > hi[q]) hi[q] = v[q];
+		}
+	}
+	double d[6][2];
+	size_t ne = b.ne;
+	for (int k = 0; k < 3; k++) {
+		int e = face[k] >> 1, side = face[k] & 1;
+		for (int gp = 0; gp < 2; gp++) {
+			d[2 * k + gp][0] = b.edl[(size_t)(4 * side + 2 * gp) * ne + e];
+			d[2 * k + gp][1] = b.edl[(size_t)(4 * side + 2 * gp + 1) * ne + e];
+		}
+	}
+	for (int q = 0; q < 4; q++) {
+		double gx = g[2 * q], gy = g[2 * q + 1];
+		double psi = 1.0;
+		for (int k = 0; k < 6; k++) {
+			double D = gx * d[k][0] + gy * d[k][1];
+			double r = 1.0;
+			if (D > 0.0) r = (hi[q] - phi[q]) / D;
+			else if (D < 0.0) r = (lo[q] - phi[q]) / D;
+			if (r < psi) psi = r;
+		}
+		if (psi < 0.0) psi = 0.0;
+		g[2 * q] = psi * gx;
+		g[2 * q + 1] = psi * gy;
+	}
+}
+
+// Green-Gauss gradients, FVM_TVD::calcGrad (src/methods/fvm_tvd.cpp:242-301)
+// as a gather: the faces of cell i in ascending edge order, each adding
+// (phi_c1 + phi_c2)/2 * n * l with + on c1's side and - on c2's, then / S.
+// LSQ (extension): grad = sum_k w_k (phi_k - phi_i).
+template <bool MM, bool LSQ, bool LIM>
+__global__ __launch_bounds__(BLOCK) void k_gradient(Bufs b, Phys ph)
+{
+	int i = xcd_block() * BLOCK + threadIdx.x;
+	if (i >= b.nc) return;
+	size_t n = b.nc;
+	Prim pi = load_prim(b, i);
+	int face[3];
+	Prim nb[3];
+	for (int k = 0; k < 3; k++) {
+		face[k] = b.cface[k * n + i];
+		int j = b.cnb[k * n + i];
+		if (j >= 0) nb[k] = load_prim(b, j);
+		else nb[k] = ghost_of<MM>(b, ph, face[k] >> 1, i, pi, pi);
+	}
+	double g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	if (LSQ) {
+		for (int k = 0; k < 3; k++) {
+			double wx = b.lsqw[(size_t)(2 * k) * n + i], wy = b.lsqw[(size_t)(2 * k + 1) * n + i];
+			double dphi[4] = {nb[k].r - pi.r, nb[k].p - pi.p, nb[k].u - pi.u, nb[k].v - pi.v};
+			for (int q = 0; q < 4; q++) {
+				g[2 * q] += wx * dphi[q];
+				g[2 * q + 1] += wy * dphi[q];
+			}
+		}
+	} else {
+		for (int k = 0; k < 3; k++) {
+			int e = face[k] >> 1, side = face[k] & 1;
+			double nx = b.enx[e], ny = b.eny[e], l = b.el[e];
+			double s[4] = {pi.r + nb[k].r, pi.p + nb[k].p, pi.u + nb[k].u, pi.v + nb[k].v};
+			for (int q = 0; q < 4; q++) {
+				double tx = s[q] / 2 * nx * l;
+				double ty = s[q] / 2 * ny * l;
+				if (side == 0) {
+					g[2 * q] += tx;
+					g[2 * q + 1] += ty;
+				} else {
+					g[2 * q] -= tx;
+					g[2 * q + 1] -= ty;
+				}
+			}
+		}
+		double si = b.S[i];
+		for (int q = 0; q < 8; q++) g[q] = g[q] / si;
+	}
+	if (LIM) minmod_limit(b, i, face, pi, nb, g);
+	for (int q = 0; q < 8; q++) b.G[q * n + i] = g[q];
+}
+
+__device__ __forceinline__ void recon(Prim& q, const Bufs& b, int c, double dx, double dy)
+{
+	size_t n = b.nc;
+	q.r += b.G[c] * dx + b.G[n + c] * dy;
+	q.p += b.G[2 * n + c] * dx + b.G[3 * n + c] * dy;
+	q.u += b.G[4 * n + c] * dx + b.G[5 * n + c] * dy;
+	q.v += b.G[6 * n + c] * dx + b.G[7 * n + c] * dy;
+}
+
+__device__ __forceinline__ void grad_T(const Bufs& b, int c, const Prim& q, const MatV& m, double& tx, double& ty)
+{
+	size_t n = b.nc;
+	double Rg = GAS_R / m.M;
+	double pr = q.p / q.r;
+	tx = (b.G[2 * n + c] - pr * b.G[c]) / (q.r * Rg);
+	ty = (b.G[3 * n + c] - pr * b.G[n + c]) / (q.r * Rg);
+}
+
+// Edge flux pass, FVM_TVD::run edge loop + reconstruct + calcFlux
+// (src/methods/fvm_tvd.cpp:329-352, 646-691, 602-622): at both Gauss points
+// reconstruct (r, p, u, v) linearly from each side's centroid (ghost of the
+// reconstructed left state on a boundary), evaluate the numerical flux,
+// subtract the viscous flux (extension), and write (f1 + f2) * l/2 once.
+template <bool MM, int FLUX, bool VISC>
+__global__ __launch_bounds__(BLOCK) void k_edge_flux(Bufs b, Phys ph)
+{
+	int e = xcd_block() * BLOCK + threadIdx.x;
+	if (e >= b.ne) return;
+	size_t ne = b.ne;
+	int c1 = b.ec1[e], c2 = b.ec2[e];
+	double nx = b.enx[e], ny = b.eny[e];
+	Prim p1 = load_prim(b, c1);
+	Prim p2 = c2 >= 0 ? load_prim(b, c2) : p1;
+	double acc[4] = {0.0, 0.0, 0.0, 0.0};
+	long long hits = 0;
+	for (int gp = 0; gp < 2; gp++) {
+		Prim L = p1, R;
+		recon(L, b, c1, b.edl[(size_t)(2 * gp) * ne + e], b.edl[(size_t)(2 * gp + 1) * ne + e]);
+		if (c2 >= 0) {
+			R = p2;
+			recon(R, b, c2, b.edl[(size_t)(4 + 2 * gp) * ne + e], b.edl[(size_t)(5 + 2 * gp) * ne + e]);
+		} else {
+			R = ghost_of<MM>(b, ph, e, c1, L, p1);
+		}
+		double f[4];
+		int hit = 0;
+		if (FLUX == CFD2D_FLUX_LAX) flux_lax(f, L, R, nx, ny, ph.gam_flux);
+		else if (FLUX == CFD2D_FLUX_HLLC) flux_hllc(f, L, R, nx, ny, ph.gam_flux);
+		else if (FLUX == CFD2D_FLUX_ROE) flux_roe(f, L, R, nx, ny, ph.gam_flux);
+		else flux_godunov(f, L, R, nx, ny, ph.gam_flux, ph.max_newton, hit);
+		hits += hit;
+		if (VISC) {
+			MatV m = mat_of<MM>(b, ph, c1);
+			size_t n = b.nc;
+			double ux = b.G[4 * n + c1], uy = b.G[5 * n + c1], vx = b.G[6 * n + c1], vy = b.G[7 * n + c1];
+			double tx, ty;
+			grad_T(b, c1, p1, m, tx, ty);
+			if (c2 >= 0) {
+				MatV m2 = mat_of<MM>(b, ph, c2);
+				double tx2, ty2;
+				grad_T(b, c2, p2, m2, tx2, ty2);
+				ux = 0.5 * (ux + b.G[4 * n + c2]);
+				uy = 0.5 * (uy + b.G[5 * n + c2]);
+				vx = 0.5 * (vx + b.G[6 * n + c2]);
+				vy = 0.5 * (vy + b.G[7 * n + c2]);
+				tx = 0.5 * (tx + tx2);
+				ty = 0.5 * (ty + ty2);
+			}
+			double div = ux + vy;
+			double txx = m.mu * (2.0 * ux - (2.0 / 3.0) * div);
+			double tyy = m.mu * (2.0 * vy - (2.0 / 3.0) * div);
+			double txy = m.mu * (uy + vx);
+			double uf = 0.5 * (L.u + R.u), vf = 0.5 * (L.v + R.v);
+			double sx = txx * nx + txy * ny, sy = txy * nx + tyy * ny;
+			f[1] -= sx;
+			f[2] -= sy;
+			f[3] -= uf * sx + vf * sy + m.kc * (tx * nx + ty * ny);
+		}
+		for (int q = 0; q < 4; q++) acc[q] += f[q];
+	}
+	if (hits) atomicAdd(&b.err[0], (unsigned long long)hits);
+	double hl = b.el[e] * 0.5;
+	for (int q = 0; q < 4; q++) b.F[q * ne + e] = acc[q] * hl;
+}
+
+// residual of cell i: -F on c1's side, +F on c2's, ascending edge order
+// (the scatter of fvm_tvd.cpp:353-363 seen from the cell)
+__device__ __forceinline__ void gather_residual(const Bufs& b, int i, double* r)
+{
+	size_t n = b.nc, ne = b.ne;
+	r[0] = r[1] = r[2] = r[3] = 0.0;
+	for (int k = 0; k < 3; k++) {
+		int f = b.cface[k * n + i];
+		int e = f >> 1;
+		if (f & 1)
+			for (int q = 0; q < 4; q++) r[q] += b.F[q * ne + e];
+		else
+			for (int q = 0; q < 4; q++) r[q] -= b.F[q * ne + e];
+	}
+}
+
+__global__ __launch_bounds__(BLOCK) void k_residual(Bufs b)
+{
+	int i = xcd_block() * BLOCK + threadIdx.x;
+	if (i >= b.nc) return;
+	double r[4];
+	gather_residual(b, i, r);
+	for (int q = 0; q < 4; q++) b.R[(size_t)q * b.nc + i] = r[q];
+}
+
+enum { UPD_FIRST = 0, UPD_RK2_LAST = 2, UPD_RK3_MID = 3, UPD_RK3_LAST = 4 };
+
+// Update pass (fvm_tvd.cpp:366-374, 419-447): LIM-flagged cells stay frozen;
+// the first stage also saves U0 (the U_old copy of 318-321); the last stage
+// forms the RK combination, refreshes the primitives and sets the bound
+// flags (clamped values are not written back, 439-446).
+template <bool MM, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_update(Bufs b, Phys ph)
+{
+	int lb = xcd_block();
+	int i = lb * BLOCK + threadIdx.x;
+	int flagged = 0;
+	if (i < b.nc) {
+		size_t n = b.nc;
+		unsigned fl = b.flags[i];
+		bool lim = (fl & 0x2u) != 0;
+		double u[4], u0[4];
+		for (int q = 0; q < 4; q++) u[q] = b.U[q * n + i];
+		if (MODE == UPD_FIRST)
+			for (int q = 0; q < 4; q++) b.U0[q * n + i] = u[q];
+		if (!lim) {
+			double r[4];
+			gather_residual(b, i, r);
+			double cfl = b.dtS[i];
+			if (MODE != UPD_FIRST)
+				for (int q = 0; q < 4; q++) u0[q] = b.U0[q * n + i];
+			for (int q = 0; q < 4; q++) {
+				double t = u[q] + cfl * r[q];
+				if (MODE == UPD_FIRST) u[q] = t;
+				else if (MODE == UPD_RK2_LAST) u[q] = 0.5 * (u0[q] + t);
+				else if (MODE == UPD_RK3_MID) u[q] = 0.75 * u0[q] + 0.25 * t;
+				else u[q] = (1.0 / 3.0) * u0[q] + (2.0 / 3.0) * t;
+			}
+			bool finite = true;
+			for (int q = 0; q < 4; q++) {
+				b.U[q * n + i] = u[q];
+				finite = finite && isfinite(u[q]);
+			}
+			if (!finite) atomicAdd(&b.err[1], 1ull);
+			MatV m = mat_of<MM>(b, ph, i);
+			Prim p = cons_to_prim(u[0], u[1], u[2], u[3], m);
+			store_prim(b, i, p);
+			if (MODE == UPD_RK2_LAST || MODE == UPD_RK3_LAST) {
+				bool out = p.r < ph.ro_min || p.r > ph.ro_max || p.p < ph.p_min || p.p > ph.p_max ||
+				           fabs(p.u) > ph.u_max || fabs(p.v) > ph.u_max;
+				if (out) {
+					fl |= 0x2u;
+					b.flags[i] = fl;
+				}
+			}
+		}
+		flagged = (fl & 0x2u) != 0;
+	}
+	if (MODE == UPD_RK2_LAST || MODE == UPD_RK3_LAST) {
+		int any = __syncthreads_or(flagged);
+		if (threadIdx.x == 0) b.blockflag[lb] = any;
+	}
+}
+
+// FVM_TVD::remediateLimCells (src/methods/fvm_tvd.cpp:464-499): one
+// workgroup finds the 256-cell blocks that hold LIM-flagged cells, then one
+// lane walks those cells in ascending order (the reference's sequential
+// sweep: a later cell sees values remediated earlier), replacing each by the
+// area-weighted mean of the c2 cells of its edges in edge-list order (which
+// may be the cell itself) and advancing its 32-step counter.
+template <bool MM>
+__global__ __launch_bounds__(BLOCK) void k_remediate(Bufs b, Phys ph)
+{
+	size_t n = b.nc;
+	for (int base = 0; base < b.nblocks; base += BLOCK) {
+		int blk = base + (int)threadIdx.x;
+		int mine = blk < b.nblocks ? b.blockflag[blk] : 0;
+		if (!__syncthreads_or(mine)) continue;
+		if (threadIdx.x == 0) {
+			int lim = min(base + BLOCK, b.nblocks);
+			for (int bb = base; bb < lim; bb++) {
+				if (!b.blockflag[bb]) continue;
+				int end = min((bb + 1) * BLOCK, b.nc);
+				for (int i = bb * BLOCK; i < end; i++) {
+					unsigned fl = b.flags[i];
+					if (!(fl & 0x2u)) continue;
+					double sr[4] = {0.0, 0.0, 0.0, 0.0}, S = 0.0;
+					for (int k = 0; k < 3; k++) {
+						int e = b.cface[k * n + i] >> 1;
+						int j = b.ec2[e];
+						if (j >= 0) {
+							double s = b.S[j];
+							S += s;
+							for (int q = 0; q < 4; q++) sr[q] += b.U[q * n + j] * s;
+						}
+					}
+					double u[4];
+					for (int q = 0; q < 4; q++) {
+						u[q] = sr[q] / S;
+						b.U[q * n + i] = u[q];
+					}
+					fl += 0x010000u;
+					if (fl & 0x200000u) fl &= 0x001110u;
+					b.flags[i] = fl;
+					MatV m = mat_of<MM>(b, ph, i);
+					store_prim(b, i, cons_to_prim(u[0], u[1], u[2], u[3], m));
+				}
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// FVM_TVD::calcTimeStep (src/methods/fvm_tvd.cpp:216-240): local step
+// CFL*S/max(|u|+c, |v|+c) with the cell area S (spectral variant:
+// 2*CFL*S / sum_f lambda_f l_f); steady runs keep it per cell.
+template <bool MM>
+__global__ __launch_bounds__(BLOCK) void k_tau(Bufs b, Phys ph)
+{
+	int i = xcd_block() * BLOCK + threadIdx.x;
+	if (i >= b.nc) return;
+	size_t n = b.nc;
+	Prim p = load_prim(b, i);
+	MatV m = mat_of<MM>(b, ph, i);
+	double cz = sqrt(m.gam * p.p / p.r);
+	double S = b.S[i];
+	double loc;
+	if (ph.cfl_mode == CFD2D_CFL_SPECTRAL) {
+		double li = sqrt(p.u * p.u + p.v * p.v) + cz;
+		double sum = 0.0;
+		for (int k = 0; k < 3; k++) {
+			int f = b.cface[k * n + i];
+			int e = f >> 1;
+			int j = b.cnb[k * n + i];
+			Prim q;
+			double czj;
+			if (j >= 0) {
+				q = load_prim(b, j);
+				MatV mj = mat_of<MM>(b, ph, j);
+				czj = sqrt(mj.gam * q.p / q.r);
+			} else {
+				q = ghost_of<MM>(b, ph, e, i, p, p);
+				czj = sqrt(m.gam * q.p / q.r);
+			}
+			double lj = sqrt(q.u * q.u + q.v * q.v) + czj;
+			sum += 0.5 * (li + lj) * b.el[e];
+		}
+		loc = 2.0 * ph.cfl * S / sum;
+	} else {
+		double a = fabs(p.u) + cz, c = fabs(p.v) + cz;
+		loc = ph.cfl * S / (a > c ? a : c);
+	}
+	b.loc[i] = loc;
+	if (ph.steady) b.dtS[i] = loc / S;
+}
+
+}  // namespace
+
+// ======================================================================== context
+
+struct cfd2d_ctx {
+	int nc = 0, ne = 0, ncase = 1, nbc = 0;
+	bool mm = false;
+	cfd2d_params p{};
+	Phys ph{};
+	Bufs b{};
+	hipStream_t st = nullptr;
+	std::vector<double> hS;
+	std::vector<int> case_off;
+	std::vector<double> case_tau;
+	// timing
+	bool timing = false;
+	std::vector<hipEvent_t> ev0, ev1;
+	std::vector<int> ev_kernel;
+	int ev_used = 0;
+	double tot_ms[CFD2D_K_COUNT] = {0};
+	long long launches[CFD2D_K_COUNT] = {0};
+	// graph of one step
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t graph_exec = nullptr;
+	bool capturing = false;
+	std::vector<void*> allocs;
+};
+
+namespace {
+
+constexpr int EV_POOL = 8192;
+
+int hip_fail(hipError_t e, const char* what)
+{
+	return fail(CFD2D_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(x)                                      \
+	do {                                               \
+		hipError_t _e = (x);                           \
+		if (_e != hipSuccess) return hip_fail(_e, #x); \
+	} while (0)
+
+template <typename T>
+int dalloc(cfd2d_ctx* c, T** p, size_t count)
+{
+	if (count == 0) count = 1;
+	void* q = nullptr;
+	hipError_t e = hipMalloc(&q, count * sizeof(T));
+	if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+	c->allocs.push_back(q);
+	*p = (T*)q;
+	return 0;
+}
+
+template <typename T>
+int upload(cfd2d_ctx* c, T** p, const std::vector<T>& v)
+{
+	int rc = dalloc(c, p, v.size());
+	if (rc) return rc;
+	if (!v.empty()) HIPCHK(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+	return 0;
+}
+
+int grid_of(int n) { return (n + BLOCK - 1) / BLOCK; }
+
+void rec_start(cfd2d_ctx* c, int k)
+{
+	if (!c->timing || c->capturing || c->ev_used >= EV_POOL) return;
+	if ((int)c->ev0.size() <= c->ev_used) {
+		hipEvent_t a, z;
+		if (hipEventCreate(&a) != hipSuccess) return;
+		if (hipEventCreate(&z) != hipSuccess) {
+			(void)hipEventDestroy(a);
+			return;
+		}
+		c->ev0.push_back(a);
+		c->ev1.push_back(z);
+		c->ev_kernel.push_back(k);
+	}
+	c->ev_kernel[c->ev_used] = k;
+	(void)hipEventRecord(c->ev0[c->ev_used], c->st);
+}
+
+void rec_stop(cfd2d_ctx* c)
+{
+	if (!c->timing || c->capturing || c->ev_used >= EV_POOL || c->ev_used >= (int)c->ev0.size()) return;
+	(void)hipEventRecord(c->ev1[c->ev_used], c->st);
+	c->ev_used++;
+}
+
+void harvest(cfd2d_ctx* c)
+{
+	for (int k = 0; k < c->ev_used; k++) {
+		float ms = 0.0f;
+		if (hipEventElapsedTime(&ms, c->ev0[k], c->ev1[k]) == hipSuccess) {
+			c->tot_ms[c->ev_kernel[k]] += ms;
+			c->launches[c->ev_kernel[k]]++;
+		}
+	}
+	c->ev_used = 0;
+}
+
+#define LAUNCH(c, kid, kern, n, ...)                                                        \
+	do {                                                                                    \
+		rec_start(c, kid);                                                                  \
+		hipLaunchKernelGGL(kern, dim3(grid_of(n)), dim3(BLOCK), 0, (c)->st, __VA_ARGS__);   \
+		rec_stop(c);                                                                        \
+	} while (0)
+
+template <bool MM>
+void launch_prim(cfd2d_ctx* c)
+{
+	LAUNCH(c, CFD2D_K_PRIM, (k_prim<MM>), c->nc, c->b, c->ph);
+}
+
+template <bool MM>
+void launch_gradient(cfd2d_ctx* c)
+{
+	bool lsq = c->p.gradient == CFD2D_GRAD_LSQ, lim = c->p.limiter == CFD2D_LIMITER_MINMOD;
+	if (lsq && lim) LAUNCH(c, CFD2D_K_GRADIENT, (k_gradient<MM, true, true>), c->nc, c->b, c->ph);
+	else if (lsq) LAUNCH(c, CFD2D_K_GRADIENT, (k_gradient<MM, true, false>), c->nc, c->b, c->ph);
+	else if (lim) LAUNCH(c, CFD2D_K_GRADIENT, (k_gradient<MM, false, true>), c->nc, c->b, c->ph);
+	else LAUNCH(c, CFD2D_K_GRADIENT, (k_gradient<MM, false, false>), c->nc, c->b, c->ph);
+}
+
+template <bool MM, bool VISC>
+void launch_edge_v(cfd2d_ctx* c)
+{
+	switch (c->p.flux) {
+	case CFD2D_FLUX_LAX: LAUNCH(c, CFD2D_K_EDGE_FLUX, (k_edge_flux<MM, CFD2D_FLUX_LAX, VISC>), c->ne, c->b, c->ph); break;
+	case CFD2D_FLUX_HLLC: LAUNCH(c, CFD2D_K_EDGE_FLUX, (k_edge_flux<MM, CFD2D_FLUX_HLLC, VISC>), c->ne, c->b, c->ph); break;
+	case CFD2D_FLUX_ROE: LAUNCH(c, CFD2D_K_EDGE_FLUX, (k_edge_flux<MM, CFD2D_FLUX_ROE, VISC>), c->ne, c->b, c->ph); break;
+	default: LAUNCH(c, CFD2D_K_EDGE_FLUX, (k_edge_flux<MM, CFD2D_FLUX_GODUNOV, VISC>), c->ne, c->b, c->ph); break;
+	}
+}
+
+template <bool MM>
+void launch_edge(cfd2d_ctx* c)
+{
+	if (c->p.viscous) launch_edge_v<MM, true>(c);
+	else launch_edge_v<MM, false>(c);
+}
+
+template <bool MM>
+void launch_update(cfd2d_ctx* c, int mode)
+{
+	switch (mode) {
+	case UPD_FIRST: LAUNCH(c, CFD2D_K_UPDATE, (k_update<MM, UPD_FIRST>), c->nc, c->b, c->ph); break;
+	case UPD_RK2_LAST: LAUNCH(c, CFD2D_K_UPDATE, (k_update<MM, UPD_RK2_LAST>), c->nc, c->b, c->ph); break;
+	case UPD_RK3_MID: LAUNCH(c, CFD2D_K_UPDATE, (k_update<MM, UPD_RK3_MID>), c->nc, c->b, c->ph); break;
+	default: LAUNCH(c, CFD2D_K_UPDATE, (k_update<MM, UPD_RK3_LAST>), c->nc, c->b, c->ph); break;
+	}
+}
+
+template <bool MM>
+void enqueue_step(cfd2d_ctx* c)
+{
+	if (c->p.steady) LAUNCH(c, CFD2D_K_TAU, (k_tau<MM>), c->nc, c->b, c->ph);
+	if (c->p.rk_stages == 3) {
+		const int modes[3] = {UPD_FIRST, UPD_RK3_MID, UPD_RK3_LAST};
+		for (int s = 0; s < 3; s++) {
+			launch_gradient<MM>(c);
+			launch_edge<MM>(c);
+			launch_update<MM>(c, modes[s]);
+		}
+	} else {
+		const int modes[2] = {UPD_FIRST, UPD_RK2_LAST};
+		for (int s = 0; s < 2; s++) {
+			launch_gradient<MM>(c);
+			launch_edge<MM>(c);
+			launch_update<MM>(c, modes[s]);
+		}
+	}
+	rec_start(c, CFD2D_K_REMEDIATE);
+	hipLaunchKernelGGL((k_remediate<MM>), dim3(1), dim3(BLOCK), 0, c->st, c->b, c->ph);
+	rec_stop(c);
+}
+
+void step_once(cfd2d_ctx* c)
+{
+	if (c->mm) enqueue_step<true>(c);
+	else enqueue_step<false>(c);
+}
+
+void do_prim(cfd2d_ctx* c)
+{
+	if (c->mm) launch_prim<true>(c);
+	else launch_prim<false>(c);
+}
+
+void do_gradient(cfd2d_ctx* c)
+{
+	if (c->mm) launch_gradient<true>(c);
+	else launch_gradient<false>(c);
+}
+
+void do_edge(cfd2d_ctx* c)
+{
+	if (c->mm) launch_edge<true>(c);
+	else launch_edge<false>(c);
+}
+
+int check_launch(const char* what)
+{
+	hipError_t e = hipGetLastError();
+	if (e != hipSuccess) return hip_fail(e, what);
+	return 0;
+}
+
+// least-squares weights w_k = A^{-1} d_k, A = sum d d^T (the oracle's lsq_weights)
+void lsq_weights(const Mesh& m, std::vector<double>& w)
+{
+	int nc = m.nc;
+	w.assign(6 * (size_t)nc, 0.0);
+	for (int i = 0; i < nc; i++) {
+		double d[3][2];
+		for (int k = 0; k < 3; k++) {
+			int e = m.cedges[3 * (size_t)i + k];
+			int c1 = m.c1[e], c2 = m.c2[e];
+			if (c2 < 0) {
+				double mx = m.gp[6 * (size_t)e], my = m.gp[6 * (size_t)e + 1];
+				double dn = (mx - m.cx[i]) * m.nx[e] + (my - m.cy[i]) * m.ny[e];
+				d[k][0] = 2.0 * dn * m.nx[e];
+				d[k][1] = 2.0 * dn * m.ny[e];
+			} else if (i == c1) {
+				d[k][0] = (m.cx[c2] + m.shx[e]) - m.cx[c1];
+				d[k][1] = (m.cy[c2] + m.shy[e]) - m.cy[c1];
+			} else {
+				d[k][0] = m.cx[c1] - (m.cx[c2] + m.shx[e]);
+				d[k][1] = m.cy[c1] - (m.cy[c2] + m.shy[e]);
+			}
+		}
+		double a11 = 0, a12 = 0, a22 = 0;
+		for (int k = 0; k < 3; k++) {
+			a11 += d[k][0] * d[k][0];
+			a12 += d[k][0] * d[k][1];
+			a22 += d[k][1] * d[k][1];
+		}
+		double det = a11 * a22 - a12 * a12;
+		for (int k = 0; k < 3; k++) {
+			w[(size_t)(2 * k) * nc + i] = (a22 * d[k][0] - a12 * d[k][1]) / det;
+			w[(size_t)(2 * k + 1) * nc + i] = (a11 * d[k][1] - a12 * d[k][0]) / det;
+		}
+	}
+}
+
+void destroy_ctx(cfd2d_ctx* c)
+{
+	if (!c) return;
+	if (c->st) (void)hipStreamSynchronize(c->st);
+	if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+	if (c->graph) (void)hipGraphDestroy(c->graph);
+	for (auto e : c->ev0) (void)hipEventDestroy(e);
+	for (auto e : c->ev1) (void)hipEventDestroy(e);
+	for (void* p : c->allocs) (void)hipFree(p);
+	if (c->st) (void)hipStreamDestroy(c->st);
+	delete c;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cfd2d_device_count(int* n)
+{
+	if (!n) return fail(CFD2D_ERR_ARG, "null");
+	hipError_t e = hipGetDeviceCount(n);
+	if (e != hipSuccess) {
+		*n = 0;
+		return hip_fail(e, "hipGetDeviceCount");
+	}
+	return CFD2D_OK;
+}
+
+int cfd2d_ctx_create(const cfd2d_mesh* mesh, const cfd2d_params* p, int n_bc, const int* bc_type,
+                     const double* bc_par, const int* group_bc, const int* cell_mat, cfd2d_ctx** out)
+{
+	if (!mesh || !p || !out || n_bc < 0 || (n_bc > 0 && (!bc_type || !bc_par)))
+		return fail(CFD2D_ERR_ARG, "cfd2d_ctx_create: bad arguments");
+	const Mesh& m = mesh->m;
+	if (p->nmat < 1 || p->nmat > CFD2D_MAXMAT) return fail(CFD2D_ERR_CONFIG, "materials count must be 1..8");
+	if (p->rk_stages != 2 && p->rk_stages != 3) return fail(CFD2D_ERR_CONFIG, "rk_stages must be 2 or 3");
+	if (p->flux < 0 || p->flux > 3 || p->gradient < 0 || p->gradient > 1 || p->limiter < 0 || p->limiter > 1)
+		return fail(CFD2D_ERR_CONFIG, "unknown flux/gradient/limiter");
+	for (int k = 0; k < n_bc; k++)
+		if (bc_type[k] < CFD2D_BND_INLET || bc_type[k] > CFD2D_BND_PRESSURE_OUTLET)
+			return fail(CFD2D_ERR_CONFIG, "unknown boundary type");
+	int ngroups = (int)m.edge_groups.size();
+	// every boundary edge needs a boundary object (fvm_tvd.cpp:132-174)
+	std::vector<int> ec2(m.ne);
+	for (int e = 0; e < m.ne; e++) {
+		if (m.c2[e] >= 0) {
+			ec2[e] = m.c2[e];
+			continue;
+		}
+		int g = m.egroup[e];
+		int bc = (group_bc && g >= 0 && (ngroups == 0 || g < ngroups)) ? group_bc[g] : -1;
+		if (bc < 0 || bc >= n_bc) {
+			char buf[96];
+			snprintf(buf, sizeof(buf), "ERROR (boundary condition): unknown edge type of edge %d...", e);
+			return fail(CFD2D_ERR_BOUNDARY, buf);
+		}
+		ec2[e] = -(bc + 1);
+	}
+	std::vector<int> cmat(m.nc, 0);
+	bool mm = false;
+	if (cell_mat) {
+		for (int i = 0; i < m.nc; i++) {
+			if (cell_mat[i] < 0 || cell_mat[i] >= p->nmat) return fail(CFD2D_ERR_CONFIG, "cell material out of range");
+			cmat[i] = cell_mat[i];
+			if (cmat[i] != 0) mm = true;
+		}
+	}
+	auto* c = new (std::nothrow) cfd2d_ctx;
+	if (!c) return fail(CFD2D_ERR_ARG, "out of host memory");
+	c->nc = m.nc;
+	c->ne = m.ne;
+	c->nbc = n_bc;
+	c->p = *p;
+	c->mm = mm;
+	c->ncase = (int)m.case_cell_offset.size() - 1;
+	c->case_off = m.case_cell_offset;
+	c->case_tau.assign(c->ncase, p->tau);
+	c->hS = m.S;
+	std::vector<double> mtab(5 * (size_t)p->nmat);
+	for (int k = 0; k < p->nmat; k++) {
+		double Cv = p->Cp[k] - GAS_R / p->M[k];
+		double gam = p->Cp[k] / Cv;
+		double kc = p->K[k] > 0.0 ? p->K[k] : p->ML[k] * p->Cp[k] / (p->prandtl > 0.0 ? p->prandtl : 0.72);
+		mtab[5 * k] = p->M[k];
+		mtab[5 * k + 1] = Cv;
+		mtab[5 * k + 2] = gam;
+		mtab[5 * k + 3] = p->ML[k];
+		mtab[5 * k + 4] = kc;
+	}
+	c->ph.m0 = MatV{mtab[0], mtab[1], mtab[2], mtab[3], mtab[4]};
+	c->ph.gam_flux = p->gam_flux;
+	c->ph.cfl = p->cfl;
+	c->ph.ro_min = p->ro_min;
+	c->ph.ro_max = p->ro_max;
+	c->ph.p_min = p->p_min;
+	c->ph.p_max = p->p_max;
+	c->ph.u_max = p->u_max;
+	c->ph.max_newton = p->max_newton;
+	c->ph.cfl_mode = p->cfl_mode;
+	c->ph.steady = p->steady;
+
+	int nc = m.nc, ne = m.ne;
+	std::vector<int> cface(3 * (size_t)nc), cnb(3 * (size_t)nc);
+	for (int i = 0; i < nc; i++)
+		for (int k = 0; k < 3; k++) {
+			int e = m.cedges[3 * (size_t)i + k];
+			int side = (m.c1[e] == i) ? 0 : 1;
+			cface[(size_t)k * nc + i] = 2 * e + side;
+			cnb[(size_t)k * nc + i] = side == 0 ? m.c2[e] : m.c1[e];
+		}
+	std::vector<double> edl(8 * (size_t)ne, 0.0);
+	for (int e = 0; e < ne; e++) {
+		int c1 = m.c1[e], c2 = m.c2[e];
+		for (int g = 0; g < 2; g++) {
+			double px = m.gp[6 * (size_t)e + 2 + 2 * g], py = m.gp[6 * (size_t)e + 3 + 2 * g];
+			edl[(size_t)(2 * g) * ne + e] = px - m.cx[c1];
+			edl[(size_t)(2 * g + 1) * ne + e] = py - m.cy[c1];
+			if (c2 >= 0) {
+				edl[(size_t)(4 + 2 * g) * ne + e] = px - (m.cx[c2] + m.shx[e]);
+				edl[(size_t)(5 + 2 * g) * ne + e] = py - (m.cy[c2] + m.shy[e]);
+			}
+		}
+	}
+	std::vector<double> bcp(4 * (size_t)(n_bc > 0 ? n_bc : 1), 0.0);
+	std::vector<int> bct(n_bc > 0 ? n_bc : 1, 0);
+	for (int k = 0; k < n_bc; k++) {
+		bct[k] = bc_type[k];
+		for (int q = 0; q < 4; q++) bcp[4 * k + q] = bc_par[4 * k + q];
+	}
+	Bufs& b = c->b;
+	b.nc = nc;
+	b.ne = ne;
+	b.nblocks = grid_of(nc);
+	b.nmat = p->nmat;
+	int rc = 0;
+	size_t n4 = 4 * (size_t)nc;
+	if ((rc = dalloc(c, &b.U, n4)) || (rc = dalloc(c, &b.U0, n4)) || (rc = dalloc(c, &b.P, n4)) ||
+	    (rc = dalloc(c, &b.G, 2 * n4)) || (rc = dalloc(c, &b.dtS, nc)) || (rc = upload(c, &b.S, m.S)) ||
+	    (rc = dalloc(c, &b.loc, nc)) || (rc = dalloc(c, &b.R, n4)) || (rc = upload(c, &b.cface, cface)) ||
+	    (rc = upload(c, &b.cnb, cnb)) || (rc = upload(c, &b.cmat, cmat)) || (rc = dalloc(c, &b.flags, nc)) ||
+	    (rc = dalloc(c, &b.blockflag, b.nblocks)) || (rc = upload(c, &b.ec1, m.c1)) ||
+	    (rc = upload(c, &b.ec2, ec2)) || (rc = upload(c, &b.enx, m.nx)) || (rc = upload(c, &b.eny, m.ny)) ||
+	    (rc = upload(c, &b.el, m.l)) || (rc = upload(c, &b.edl, edl)) || (rc = dalloc(c, &b.F, 4 * (size_t)ne)) ||
+	    (rc = upload(c, &b.bctype, bct)) || (rc = upload(c, &b.bcpar, bcp)) || (rc = upload(c, &b.mtab, mtab)) ||
+	    (rc = dalloc(c, &b.err, 2))) {
+		destroy_ctx(c);
+		return rc;
+	}
+	if (p->gradient == CFD2D_GRAD_LSQ) {
+		std::vector<double> w;
+		lsq_weights(m, w);
+		if ((rc = upload(c, &b.lsqw, w))) {
+			destroy_ctx(c);
+			return rc;
+		}
+	} else {
+		b.lsqw = nullptr;
+	}
+	hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+	if (e != hipSuccess) {
+		destroy_ctx(c);
+		return hip_fail(e, "hipStreamCreate");
+	}
+	std::vector<unsigned> zf(nc, 0u);
+	std::vector<int> zb(b.nblocks, 0);
+	unsigned long long zerr[2] = {0, 0};
+	if (hipMemcpy(b.flags, zf.data(), nc * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(b.blockflag, zb.data(), b.nblocks * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(b.err, zerr, sizeof(zerr), hipMemcpyHostToDevice) != hipSuccess) {
+		destroy_ctx(c);
+		return fail(CFD2D_ERR_HIP, "initial upload failed");
+	}
+	*out = c;
+	return CFD2D_OK;
+}
+
+void cfd2d_ctx_destroy(cfd2d_ctx* c) { destroy_ctx(c); }
+
+int cfd2d_set_state(cfd2d_ctx* c, const double* u)
+{
+	if (!c || !u) return fail(CFD2D_ERR_ARG, "null");
+	HIPCHK(hipMemcpyAsync(c->b.U, u, 4 * (size_t)c->nc * sizeof(double), hipMemcpyHostToDevice, c->st));
+	do_prim(c);
+	HIPCHK(hipStreamSynchronize(c->st));
+	return check_launch("k_prim");
+}
+
+int cfd2d_set_state_device(cfd2d_ctx* c, const double* du)
+{
+	if (!c || !du) return fail(CFD2D_ERR_ARG, "null");
+	HIPCHK(hipMemcpyAsync(c->b.U, du, 4 * (size_t)c->nc * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+	do_prim(c);
+	HIPCHK(hipStreamSynchronize(c->st));
+	return check_launch("k_prim");
+}
+
+int cfd2d_get_state(cfd2d_ctx* c, double* u)
+{
+	if (!c || !u) return fail(CFD2D_ERR_ARG, "null");
+	HIPCHK(hipMemcpyAsync(u, c->b.U, 4 * (size_t)c->nc * sizeof(double), hipMemcpyDeviceToHost, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	return CFD2D_OK;
+}
+
+int cfd2d_get_state_device(cfd2d_ctx* c, double* du)
+{
+	if (!c || !du) return fail(CFD2D_ERR_ARG, "null");
+	HIPCHK(hipMemcpyAsync(du, c->b.U, 4 * (size_t)c->nc * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	return CFD2D_OK;
+}
+
+int cfd2d_set_flags(cfd2d_ctx* c, const unsigned* f)
+{
+	if (!c || !f) return fail(CFD2D_ERR_ARG, "null");
+	HIPCHK(hipMemcpyAsync(c->b.flags, f, (size_t)c->nc * sizeof(unsigned), hipMemcpyHostToDevice, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	return CFD2D_OK;
+}
+
+int cfd2d_get_flags(cfd2d_ctx* c, unsigned* f)
+{
+	if (!c || !f) return fail(CFD2D_ERR_ARG, "null");
+	HIPCHK(hipMemcpyAsync(f, c->b.flags, (size_t)c->nc * sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	return CFD2D_OK;
+}
+
+// Unsteady: TAU_case = min(TAU, min over the case's cells of the local step),
+// every cell of the case gets TAU_case (fvm_tvd.cpp:226-237).  Steady: local.
+int cfd2d_calc_tau(cfd2d_ctx* c)
+{
+	if (!c) return fail(CFD2D_ERR_ARG, "null");
+	if (c->mm) LAUNCH(c, CFD2D_K_TAU, (k_tau<true>), c->nc, c->b, c->ph);
+	else LAUNCH(c, CFD2D_K_TAU, (k_tau<false>), c->nc, c->b, c->ph);
+	int rc = check_launch("k_tau");
+	if (rc) return rc;
+	if (c->p.steady) {
+		HIPCHK(hipStreamSynchronize(c->st));
+		return CFD2D_OK;
+	}
+	std::vector<double> loc(c->nc);
+	HIPCHK(hipMemcpyAsync(loc.data(), c->b.loc, c->nc * sizeof(double), hipMemcpyDeviceToHost, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	std::vector<double> dts(c->nc);
+	for (int k = 0; k < c->ncase; k++) {
+		double T = c->case_tau[k];
+		for (int i = c->case_off[k]; i < c->case_off[k + 1]; i++)
+			if (T > loc[i]) T = loc[i];
+		c->case_tau[k] = T;
+		for (int i = c->case_off[k]; i < c->case_off[k + 1]; i++) dts[i] = T / c->hS[i];
+	}
+	HIPCHK(hipMemcpyAsync(c->b.dtS, dts.data(), c->nc * sizeof(double), hipMemcpyHostToDevice, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	return CFD2D_OK;
+}
+
+int cfd2d_get_tau(cfd2d_ctx* c, double* tau0, double* ctau)
+{
+	if (!c) return fail(CFD2D_ERR_ARG, "null");
+	if (tau0) *tau0 = c->case_tau[0];
+	if (ctau) {
+		std::vector<double> dts(c->nc);
+		HIPCHK(hipMemcpyAsync(dts.data(), c->b.dtS, c->nc * sizeof(double), hipMemcpyDeviceToHost, c->st));
+		HIPCHK(hipStreamSynchronize(c->st));
+		if (c->p.steady) {
+			std::vector<double> loc(c->nc);
+			HIPCHK(hipMemcpy(loc.data(), c->b.loc, c->nc * sizeof(double), hipMemcpyDeviceToHost));
+			for (int i = 0; i < c->nc; i++) ctau[i] = loc[i];
+		} else {
+			for (int k = 0; k < c->ncase; k++)
+				for (int i = c->case_off[k]; i < c->case_off[k + 1]; i++) ctau[i] = c->case_tau[k];
+		}
+	}
+	return CFD2D_OK;
+}
+
+int cfd2d_set_tau(cfd2d_ctx* c, double tau)
+{
+	if (!c || !(tau > 0.0)) return fail(CFD2D_ERR_ARG, "bad tau");
+	std::vector<double> dts(c->nc);
+	for (int k = 0; k < c->ncase; k++) c->case_tau[k] = tau;
+	for (int i = 0; i < c->nc; i++) dts[i] = tau / c->hS[i];
+	HIPCHK(hipMemcpyAsync(c->b.dtS, dts.data(), c->nc * sizeof(double), hipMemcpyHostToDevice, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	return CFD2D_OK;
+}
+
+int cfd2d_gradients(cfd2d_ctx* c, double* g)
+{
+	if (!c || !g) return fail(CFD2D_ERR_ARG, "null");
+	do_gradient(c);
+	int rc = check_launch("k_gradient");
+	if (rc) return rc;
+	HIPCHK(hipMemcpyAsync(g, c->b.G, 8 * (size_t)c->nc * sizeof(double), hipMemcpyDeviceToHost, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	return CFD2D_OK;
+}
+
+int cfd2d_edge_fluxes(cfd2d_ctx* c, double* f)
+{
+	if (!c || !f) return fail(CFD2D_ERR_ARG, "null");
+	do_gradient(c);
+	do_edge(c);
+	int rc = check_launch("k_edge_flux");
+	if (rc) return rc;
+	std::vector<double> F(4 * (size_t)c->ne);
+	HIPCHK(hipMemcpyAsync(F.data(), c->b.F, F.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	for (int e = 0; e < c->ne; e++)
+		for (int q = 0; q < 4; q++) f[4 * (size_t)e + q] = F[(size_t)q * c->ne + e];
+	return CFD2D_OK;
+}
+
+int cfd2d_residual(cfd2d_ctx* c, double* r)
+{
+	if (!c || !r) return fail(CFD2D_ERR_ARG, "null");
+	do_gradient(c);
+	do_edge(c);
+	hipLaunchKernelGGL(k_residual, dim3(grid_of(c->nc)), dim3(BLOCK), 0, c->st, c->b);
+	int rc = check_launch("k_residual");
+	if (rc) return rc;
+	HIPCHK(hipMemcpyAsync(r, c->b.R, 4 * (size_t)c->nc * sizeof(double), hipMemcpyDeviceToHost, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	return CFD2D_OK;
+}
+
+int cfd2d_step(cfd2d_ctx* c, int nsteps)
+{
+	if (!c || nsteps < 0) return fail(CFD2D_ERR_ARG, "bad arguments");
+	for (int s = 0; s < nsteps; s++) step_once(c);
+	return check_launch("cfd2d_step");
+}
+
+int cfd2d_step_graph(cfd2d_ctx* c, int nsteps)
+{
+	if (!c || nsteps < 0) return fail(CFD2D_ERR_ARG, "bad arguments");
+	if (!c->graph_exec) {
+		c->capturing = true;
+		HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+		step_once(c);
+		hipError_t e = hipStreamEndCapture(c->st, &c->graph);
+		c->capturing = false;
+		if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+		HIPCHK(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+	}
+	for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(c->graph_exec, c->st));
+	return CFD2D_OK;
+}
+
+int cfd2d_sync(cfd2d_ctx* c)
+{
+	if (!c) return fail(CFD2D_ERR_ARG, "null");
+	HIPCHK(hipStreamSynchronize(c->st));
+	harvest(c);
+	unsigned long long err[2];
+	HIPCHK(hipMemcpy(err, c->b.err, sizeof(err), hipMemcpyDeviceToHost));
+	if (err[0]) return fail(CFD2D_ERR_NEWTON, "Riemann Newton iteration hit its cap " + std::to_string(err[0]) + " times");
+	if (err[1]) return fail(CFD2D_ERR_NONFINITE, "non-finite state in " + std::to_string(err[1]) + " cell updates");
+	return CFD2D_OK;
+}
+
+int cfd2d_status(cfd2d_ctx* c, long long* hits, int* nonfinite)
+{
+	if (!c) return fail(CFD2D_ERR_ARG, "null");
+	HIPCHK(hipStreamSynchronize(c->st));
+	unsigned long long err[2];
+	HIPCHK(hipMemcpy(err, c->b.err, sizeof(err), hipMemcpyDeviceToHost));
+	if (hits) *hits = (long long)err[0];
+	if (nonfinite) *nonfinite = (int)err[1];
+	return CFD2D_OK;
+}
+
+int cfd2d_wall_force(cfd2d_ctx* c, const int* mask, double* fx, double* fy)
+{
+	if (!c || !mask) return fail(CFD2D_ERR_ARG, "null");
+	std::vector<double> P(4 * (size_t)c->nc), el(c->ne), nx(c->ne), ny(c->ne);
+	std::vector<int> c1(c->ne);
+	HIPCHK(hipStreamSynchronize(c->st));
+	HIPCHK(hipMemcpy(P.data(), c->b.P, P.size() * sizeof(double), hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(el.data(), c->b.el, el.size() * sizeof(double), hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(nx.data(), c->b.enx, nx.size() * sizeof(double), hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(ny.data(), c->b.eny, ny.size() * sizeof(double), hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(c1.data(), c->b.ec1, c1.size() * sizeof(int), hipMemcpyDeviceToHost));
+	double X = 0.0, Y = 0.0;
+	for (int e = 0; e < c->ne; e++) {
+		if (!mask[e]) continue;
+		double p = P[(size_t)c->nc + c1[e]];
+		X += p * el[e] * nx[e];
+		Y += p * el[e] * ny[e];
+	}
+	if (fx) *fx = X;
+	if (fy) *fy = Y;
+	return CFD2D_OK;
+}
+
+int cfd2d_timing(cfd2d_ctx* c, int enable)
+{
+	if (!c) return fail(CFD2D_ERR_ARG, "null");
+	HIPCHK(hipStreamSynchronize(c->st));
+	harvest(c);
+	c->timing = enable != 0;
+	for (int k = 0; k < CFD2D_K_COUNT; k++) {
+		c->tot_ms[k] = 0.0;
+		c->launches[k] = 0;
+	}
+	return CFD2D_OK;
+}
+
+int cfd2d_timing_read(cfd2d_ctx* c, int kernel, double* total_ms, long long* launches)
+{
+	if (!c || kernel < 0 || kernel >= CFD2D_K_COUNT) return fail(CFD2D_ERR_ARG, "bad kernel id");
+	HIPCHK(hipStreamSynchronize(c->st));
+	harvest(c);
+	if (total_ms) *total_ms = c->tot_ms[kernel];
+	if (launches) *launches = c->launches[kernel];
+	return CFD2D_OK;
+}
+
+int cfd2d_ctx_info(cfd2d_ctx* c, int* nc, int* ne, int* ncase)
+{
+	if (!c) return fail(CFD2D_ERR_ARG, "null");
+	if (nc) *nc = c->nc;
+	if (ne) *ne = c->ne;
+	if (ncase) *ncase = c->ncase;
+	return CFD2D_OK;
+}
+
+int cfd2d_ctx_stream(cfd2d_ctx* c, void** s)
+{
+	if (!c || !s) return fail(CFD2D_ERR_ARG, "null");
+	*s = (void*)c->st;
+	return CFD2D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,150 @@
+/*
+ * cfd2d -- MI355X (gfx950) explicit finite-volume step, C ABI.
+ *
+ * The drop-in boundary for the reference's explicit method FVM_TVD
+ * (src/methods/fvm_tvd.h:7-115, src/methods/fvm_tvd.cpp): its Method
+ * interface init/run/done (src/methods/method.h:6-11) is kept by the C++
+ * host class in cfd-2d_amd/csrc/method.cpp, which calls the entry points
+ * below; other hosts (ctypes, C/C++ tests) call them directly.
+ *
+ * Conventions: every function returns CFD2D_OK (0) or an error code;
+ * cfd2d_last_error() gives the message of the last failure on the calling
+ * thread.  No exception crosses the ABI.  Host buffers belong to the caller;
+ * state arrays are structure-of-arrays double[4][n_cells] in the order
+ * (rho, rho*u, rho*v, rho*E).  Each context owns one HIP stream and device
+ * buffers; a context is not thread-safe, distinct contexts are.
+ */
+#ifndef CFD2D_H
+#define CFD2D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+	CFD2D_OK = 0,
+	CFD2D_ERR_ARG = 1,          /* bad argument / size */
+	CFD2D_ERR_HIP = 2,          /* HIP runtime failure */
+	CFD2D_ERR_NEWTON = 3,       /* Riemann Newton iteration hit its cap */
+	CFD2D_ERR_NONFINITE = 4,    /* non-finite state after an update */
+	CFD2D_ERR_CONFIG = 5,       /* task / configuration error */
+	CFD2D_ERR_BOUNDARY = 102,   /* edge without a boundary condition (Exception::TYPE_BOUND_UNKNOWN) */
+	CFD2D_ERR_MESH_NAME = 201,  /* unknown mesh type (TYPE_MESH_WRONG_NAME) */
+	CFD2D_ERR_UNV_ELEMENT = 221,/* TYPE_MESH_UNV_UNKNOWN_ELEMENT */
+	CFD2D_ERR_UNV_BND_EDGE = 222,/* TYPE_MESH_UNV_NOT_DEFINED_BND_EDGE */
+	CFD2D_ERR_MESH = 230,       /* malformed mesh input */
+	CFD2D_ERR_FILE = 301,       /* FILE_OPENING_ERROR */
+	CFD2D_ERR_PERIODIC = 302
+};
+
+enum { CFD2D_FLUX_GODUNOV = 0, CFD2D_FLUX_LAX = 1, CFD2D_FLUX_HLLC = 2, CFD2D_FLUX_ROE = 3 };
+enum { CFD2D_GRAD_GREEN_GAUSS = 0, CFD2D_GRAD_LSQ = 1 };
+enum { CFD2D_LIMITER_NONE = 0, CFD2D_LIMITER_MINMOD = 1 };
+enum { CFD2D_CFL_REFERENCE = 0, CFD2D_CFL_SPECTRAL = 1 };
+/* boundary types: CFDBndInlet / CFDBndOutlet / CFDBndWallSlip (src/bnd_cond.h:35-53);
+ * BOUND_WALL_NO_SLIP maps to the slip wall as the reference's factory does
+ * (src/bnd_cond.cpp:50-51); PRESSURE_OUTLET is an extension */
+enum { CFD2D_BND_INLET = 1, CFD2D_BND_OUTLET = 2, CFD2D_BND_WALL = 3, CFD2D_BND_PRESSURE_OUTLET = 4 };
+enum { CFD2D_MAXMAT = 8 };
+
+/* kernels reported by cfd2d_timing_read */
+enum {
+	CFD2D_K_GRADIENT = 0, CFD2D_K_EDGE_FLUX = 1, CFD2D_K_UPDATE = 2, CFD2D_K_REMEDIATE = 3,
+	CFD2D_K_TAU = 4, CFD2D_K_PRIM = 5, CFD2D_K_COUNT = 6
+};
+
+/*
+ * What FVM_TVD::init reads from <control>, <limits> and <materials>
+ * (src/methods/fvm_tvd.cpp:27-68), plus extension switches whose zero
+ * values are the reference behaviour.
+ */
+typedef struct cfd2d_params {
+	int flux;          /* CFD2D_FLUX_*; the reference hard-wires GODUNOV (fvm_tvd.cpp:602-622) */
+	int gradient;      /* CFD2D_GRAD_*; reference: Green-Gauss (fvm_tvd.cpp:242-301) */
+	int limiter;       /* CFD2D_LIMITER_*; reference: none */
+	int rk_stages;     /* 2 = the reference's Heun RK2 (fvm_tvd.cpp:323-438); 3 = SSP-RK3 */
+	int viscous;       /* 1 = Navier-Stokes viscous flux (extension) */
+	int steady;        /* <STEADY>: 1 = local time step recomputed every step */
+	int cfl_mode;      /* CFD2D_CFL_* */
+	int max_newton;    /* cap on rim_orig's Newton loop (0 = none, as the reference) */
+	double cfl;        /* <CFL> */
+	double tau;        /* <TAU>: upper bound of the unsteady time step */
+	double gam_flux;   /* gamma used by the flux: 1.4 in the reference (fvm_tvd.cpp:345) */
+	double ro_min, ro_max, p_min, p_max, u_max;   /* <limits> (fvm_tvd.cpp:43-48) */
+	double prandtl;    /* conductivity = ML*Cp/Pr when K == 0 (extension) */
+	int nmat;          /* <materials count>, at most CFD2D_MAXMAT */
+	double M[CFD2D_MAXMAT], Cp[CFD2D_MAXMAT], K[CFD2D_MAXMAT], ML[CFD2D_MAXMAT];
+} cfd2d_params;
+
+typedef struct cfd2d_mesh cfd2d_mesh;
+typedef struct cfd2d_ctx cfd2d_ctx;
+
+const char* cfd2d_error_string(int code);
+const char* cfd2d_last_error(void);
+int cfd2d_device_count(int* n);
+
+/* ---- mesh (host) ----
+ * Replaces MeshReader::create/read (src/mesh/MeshReader.cpp:7-23) and the
+ * Grid construction of MeshReaderSalomeUnv::read
+ * (src/mesh/MeshReaderSalomeUnv.cpp:25-264). */
+int  cfd2d_mesh_create(int n_nodes, const double* xy, int n_cells, const int* tri, int n_bedges,
+                       const int* bedge_nodes, const int* bedge_group, cfd2d_mesh** out);
+int  cfd2d_mesh_read(const char* files_type, const char* name, cfd2d_mesh** out);  /* "salome_unv" | "berkeley_triangle" */
+int  cfd2d_mesh_set_groups(cfd2d_mesh* m, int kind /*0 edge, 1 cell*/, int n, const char* const* names);
+int  cfd2d_mesh_set_cell_groups(cfd2d_mesh* m, const int* cell_group);
+int  cfd2d_mesh_periodic(cfd2d_mesh* m, int group_a, int group_b, double sx, double sy);
+int  cfd2d_mesh_concat(int n, const cfd2d_mesh* const* parts, cfd2d_mesh** out);
+int  cfd2d_mesh_counts(const cfd2d_mesh* m, int* n_nodes, int* n_cells, int* n_edges);
+int  cfd2d_mesh_nodes(const cfd2d_mesh* m, double* xy);   /* double[2][n_nodes] interleaved x,y */
+int  cfd2d_mesh_group_count(const cfd2d_mesh* m, int kind, int* n);
+int  cfd2d_mesh_group_name(const cfd2d_mesh* m, int kind, int i, char* buf, int len);
+int  cfd2d_mesh_get_cell_groups(const cfd2d_mesh* m, int* cell_group);
+int  cfd2d_mesh_case_offsets(const cfd2d_mesh* m, int* n_cases, int* cell_offsets, int* edge_offsets);
+/* geometry in the reference's layout: cell (S, cx, cy, 0); edge (nx, ny, l, 3 points);
+ * edge ints (n1, n2, c1, c2, type); any pointer may be NULL */
+int  cfd2d_mesh_export(const cfd2d_mesh* m, double* cell_geom, int* cell_nodes, int* cell_edges,
+                       double* edge_geom, int* edge_int, int* edge_group, double* edge_shift);
+void cfd2d_mesh_destroy(cfd2d_mesh* m);
+
+/* ---- solver context (device) ----
+ * Replaces the state arrays and passes of FVM_TVD (fvm_tvd.cpp:177-197,
+ * 216-499).  bc_type/bc_par describe n_bc boundary objects (CFDBoundary,
+ * src/bnd_cond.h:12-53: par = Vx, Vy, T, P for inlets, P for pressure
+ * outlets); group_bc maps each edge group to a boundary object (-1: none);
+ * cell_mat gives each cell's material (NULL: all 0).  Every boundary edge
+ * must map to an object (else CFD2D_ERR_BOUNDARY, as fvm_tvd.cpp:148-152). */
+int  cfd2d_ctx_create(const cfd2d_mesh* mesh, const cfd2d_params* p, int n_bc, const int* bc_type,
+                      const double* bc_par, const int* group_bc, const int* cell_mat, cfd2d_ctx** out);
+void cfd2d_ctx_destroy(cfd2d_ctx* c);
+int  cfd2d_set_state(cfd2d_ctx* c, const double* u);          /* host double[4][nc] */
+int  cfd2d_get_state(cfd2d_ctx* c, double* u);
+int  cfd2d_set_state_device(cfd2d_ctx* c, const double* d_u); /* device double[4][nc] */
+int  cfd2d_get_state_device(cfd2d_ctx* c, double* d_u);
+int  cfd2d_set_flags(cfd2d_ctx* c, const unsigned* flags);   /* Cell::flag (src/mesh/grid.h:35) */
+int  cfd2d_get_flags(cfd2d_ctx* c, unsigned* flags);
+/* FVM_TVD::calcTimeStep (fvm_tvd.cpp:216-240) on the current state; per case for concatenated meshes */
+int  cfd2d_calc_tau(cfd2d_ctx* c);
+int  cfd2d_get_tau(cfd2d_ctx* c, double* tau_case0, double* ctau /* nc, may be NULL */);
+int  cfd2d_set_tau(cfd2d_ctx* c, double tau);
+/* stage diagnostics on the current state (parity checks) */
+int  cfd2d_gradients(cfd2d_ctx* c, double* g);      /* double[8][nc]: rx ry px py ux uy vx vy */
+int  cfd2d_edge_fluxes(cfd2d_ctx* c, double* f);    /* double[ne][4]: (sum of Gauss-point fluxes) * l/2 */
+int  cfd2d_residual(cfd2d_ctx* c, double* r);       /* double[4][nc]: the *_int arrays after one stage */
+/* nsteps time steps of FVM_TVD::run (fvm_tvd.cpp:310-449), enqueued on the context stream */
+int  cfd2d_step(cfd2d_ctx* c, int nsteps);
+/* the same steps, recorded once as a HIP graph and replayed (use_graph != 0) */
+int  cfd2d_step_graph(cfd2d_ctx* c, int nsteps);
+int  cfd2d_sync(cfd2d_ctx* c);                      /* waits; returns a device error if one was raised */
+int  cfd2d_status(cfd2d_ctx* c, long long* newton_cap_hits, int* nonfinite);
+/* pressure force over the edges with mask != 0 (calcLiftForce, fvm_tvd_implicit.cpp:1088-1110) */
+int  cfd2d_wall_force(cfd2d_ctx* c, const int* edge_mask, double* fx, double* fy);
+/* per-kernel HIP-event timing on the context stream */
+int  cfd2d_timing(cfd2d_ctx* c, int enable);
+int  cfd2d_timing_read(cfd2d_ctx* c, int kernel, double* total_ms, long long* launches);
+int  cfd2d_ctx_info(cfd2d_ctx* c, int* n_cells, int* n_edges, int* n_cases);
+int  cfd2d_ctx_stream(cfd2d_ctx* c, void** hip_stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

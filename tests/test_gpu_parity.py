@@ -1,0 +1,209 @@
+"""HIP path vs the reference (golden vectors) and vs the oracle, through the C ABI.
+
+Tolerances (FP64 on both sides; the device contracts a*b+c into FMA and its
+exp/log differ from glibc's in the last bit, so agreement is to round-off,
+not bitwise):
+  time step, gradients, edge fluxes      1e-12 relative to the component's max
+  stage-1 residual                       |dR| / max(|R|, sum_f |F_f|_inf l/2) <= 1e-12
+  state after 1..100 steps               1e-11 relative L-inf per component
+  state after 1000 steps                 1e-9  relative L-inf per component
+"""
+import numpy as np
+import pytest
+
+import golden_cases as gc
+import oracle
+from cfd2d import case as cases
+from cfd2d import native
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b, axis=None):
+    den = np.max(np.abs(b), axis=axis)
+    den = np.where(den == 0, 1.0, den)
+    return np.max(np.abs(a - b), axis=axis) / den
+
+
+@pytest.fixture(scope="module", params=gc.CASES)
+def golden(request):
+    name = request.param
+    return name, gc.case(name), gc.load(name)
+
+
+def test_device_present():
+    assert native.device_count() >= 1
+
+
+def test_time_step(golden):
+    name, case, g = golden
+    s = native.GpuSolver(case)
+    tau, ct = s.tau()
+    assert abs(tau - g["tau"][0]) <= 1e-13 * g["tau"][0]
+    assert np.max(np.abs(ct - g["tau"][1:]) / g["tau"][1:]) <= 1e-13
+    s.close()
+
+
+def test_gradients(golden):
+    name, case, g = golden
+    s = native.GpuSolver(case)
+    G = s.gradients()
+    assert np.all(_rel(G, g["grad0"], axis=1) <= 1e-12), _rel(G, g["grad0"], axis=1)
+    s.close()
+
+
+def test_edge_fluxes(golden):
+    name, case, g = golden
+    s = native.GpuSolver(case)
+    F = s.edge_fluxes()
+    assert np.all(_rel(F, g["edgeflux0"], axis=0) <= 1e-12), _rel(F, g["edgeflux0"], axis=0)
+    s.close()
+
+
+def residual_scale(cell_edges, edgeflux):
+    """Per cell: sum over its faces of the face-flux vector's max-norm |F_f|_inf l_f/2.
+
+    The round-off of every residual component is set by the largest flux
+    components (pressure terms), so the scale of a component whose fluxes
+    vanish in exact arithmetic (mass flux of a fluid at rest) is that of
+    the whole flux vector, not of its own noise."""
+    fmax = np.abs(edgeflux).max(axis=1)
+    return fmax[cell_edges].sum(axis=1)
+
+
+def test_residual_1e12(golden):
+    """The north-star residual gate: per-cell residual within 1e-12 relative,
+    |R_gpu - R_ref| / max(|R_ref|, sum_f |F_f|_inf l_f/2) <= 1e-12."""
+    name, case, g = golden
+    s = native.GpuSolver(case)
+    R = s.residual()
+    ref = g["residual0"]
+    scale = residual_scale(s.mesh.export()["cell_edges"], g["edgeflux0"])
+    den = np.maximum(np.abs(ref), scale[None, :])
+    err = np.max(np.abs(R - ref) / den)
+    assert err <= 1e-12, err
+    s.close()
+
+
+def test_steps(golden):
+    name, case, g = golden
+    s = native.GpuSolver(case)
+    steps = sorted(int(k.split("step")[1]) for k in g.files if k.startswith("u_step"))
+    done = 0
+    for st in steps:
+        s.run(st - done)
+        done = st
+        U = s.state()
+        tol = 1e-9 if st >= 1000 else 1e-11
+        err = _rel(U, g["u_step%d" % st], axis=1)
+        assert np.all(err <= tol), (name, st, err)
+        np.testing.assert_array_equal(s.flags(), g["flags_step%d" % st])
+    assert s.status() == (0, 0)
+    s.close()
+
+
+def test_deterministic_and_graph_replay():
+    """Two eager runs and one hipGraph run agree bit-for-bit."""
+    case = gc.case("inlet")
+    outs = []
+    for graph in (False, False, True):
+        s = native.GpuSolver(case)
+        s.run(7, graph=graph)
+        s.run(5, graph=graph)
+        outs.append(s.state())
+        s.close()
+    assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[0], outs[2])
+
+
+EXT = [  # (full Sod state?, options): unlimited variants on the mild tube, limited ones on Sod
+    (False, dict(flux="LAX")), (False, dict(flux="HLLC")), (False, dict(flux="ROE")),
+    (False, dict(gradient="LSQ")), (False, dict(rk_stages=3)), (False, dict(cfl_mode="SPECTRAL", cfl=0.4)),
+    (False, dict(gradient="LSQ", rk_stages=3, flux="HLLC")), (False, dict(steady=True, cfl=0.4)),
+    (True, dict(limiter="MINMOD")), (True, dict(limiter="MINMOD", flux="ROE")),
+    (True, dict(limiter="MINMOD", gradient="LSQ", rk_stages=3, flux="HLLC", cfl_mode="SPECTRAL", cfl=0.4)),
+]
+
+
+@pytest.mark.parametrize("sod,opts", EXT, ids=lambda o: str(o))
+def test_extensions_match_oracle(sod, opts):
+    case = cases.shock_tube(40, 6, seed=4, mild=not sod, max_newton=200, **opts)
+    s = native.GpuSolver(case)
+    o = oracle.OracleSolver(case)
+    assert abs(s.tau()[0] - o.tau()[0]) <= 1e-13 * o.tau()[0]
+    R = s.residual()
+    Ro = o.residual()
+    assert np.all(_rel(R, Ro, axis=1) <= 1e-11), _rel(R, Ro, axis=1)
+    s.run(20)
+    o.run(20)
+    assert np.all(_rel(s.state(), o.state(), axis=1) <= 1e-10), _rel(s.state(), o.state(), axis=1)
+    assert np.array_equal(s.flags(), o.flags())
+    assert s.status() == (0, 0) and o.newton_cap_hits() == 0
+    s.close()
+
+
+def test_viscous_periodic_matches_oracle():
+    case = cases.periodic_pulse(24, seed=5)
+    s = native.GpuSolver(case)
+    o = oracle.OracleSolver(case)
+    assert abs(s.tau()[0] - o.tau()[0]) <= 1e-13 * o.tau()[0]
+    assert np.all(_rel(s.residual(), o.residual(), axis=1) <= 1e-11)
+    s.run(10)
+    o.run(10)
+    assert np.all(_rel(s.state(), o.state(), axis=1) <= 1e-11)
+    s.close()
+
+
+def test_periodic_conservation():
+    """Periodic torus: sum S*U is conserved to round-off."""
+    case = cases.periodic_pulse(32, seed=6)
+    s = native.GpuSolver(case)
+    S = s.mesh.export()["cell_geom"][:, 0]
+    tot0 = (s.state() * S).sum(axis=1)
+    s.run(100)
+    tot1 = (s.state() * S).sum(axis=1)
+    assert np.all(np.abs(tot1 - tot0) <= 1e-13 * np.abs(tot0).max())
+    s.close()
+
+
+def test_sod_full_with_minmod_runs_and_bounds():
+    """The full Sod state (which hangs the reference's unlimited path) runs
+    with the minmod limiter; density stays within the initial bounds."""
+    case = cases.shock_tube(200, 20, seed=1, mild=False, limiter="MINMOD")
+    s = native.GpuSolver(case)
+    s.run(200)
+    U = s.state()
+    assert U[0].min() >= 0.125 - 1e-12 and U[0].max() <= 1.0 + 1e-12
+    assert s.status() == (0, 0)
+    s.close()
+
+
+def test_newton_cap_reports_error():
+    """Unlimited Sod makes the reconstructed pressure negative: the bounded
+    Newton loop stops at its cap and the context reports CFD2D_ERR_NEWTON."""
+    case = cases.shock_tube(10, 2, seed=5, mild=False, max_newton=50)
+    s = native.GpuSolver(case)
+    s.step(1)
+    with pytest.raises(native.Cfd2dError) as e:
+        s.sync()
+    assert e.value.code == 3
+    s.close()
+
+
+def test_batched_equals_individual():
+    """Concatenated cases advance exactly as when run one by one."""
+    cs = [cases.bump_channel(ny, height=h, mach=m, seed=k) for k, (ny, h, m) in
+          enumerate([(6, 0.05, 0.5), (8, 0.08, 1.5), (5, 0.03, 2.5)])]
+    singles = []
+    for c in cs:
+        s = native.GpuSolver(c)
+        s.run(15)
+        singles.append(s.state())
+        s.close()
+    from cfd2d import batch
+    b = batch.BatchSolver(cs)
+    b.run(15)
+    states = b.states()
+    for a, ref in zip(states, singles):
+        assert np.array_equal(a, ref)
+    b.close()
