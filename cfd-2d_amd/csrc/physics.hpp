@@ -81,21 +81,39 @@ __device__ __forceinline__ Prim ghost_state(int type, const double* par, double 
 // exp(log(x) k).  max_iter > 0 caps the loop and reports it in *hit.
 // Outputs the state sampled at x/t = 0: density RI, internal energy EI,
 // pressure PI, normal velocity UI, tangential velocity VI.
+// The gamma-dependent constants of rim_orig (src/global.cpp:236-250),
+// formed once on the host with the same expressions instead of per call.
+struct RimConst {
+	double GAM, AGAM, DGAM, GGAM, HGAM, FGAM, OGAM, QGAM, PGAM, RGAM, SGAM, TGAM, DGAM1, IAGAM, DGGAM;
+};
+
+__host__ __device__ inline RimConst rim_const(double GAM)
+{
+	RimConst k;
+	k.GAM = GAM;
+	k.AGAM = (GAM - 1.0);
+	k.DGAM = (2.0 / k.AGAM);
+	k.GGAM = (sqrt(GAM * k.AGAM));
+	k.HGAM = (k.AGAM / 2.0);
+	k.FGAM = (3.0 * GAM - 1.0);
+	k.OGAM = (k.AGAM / (2.0 * GAM));
+	k.QGAM = (GAM + 1.0);
+	k.PGAM = (k.QGAM / (2.0 * GAM));
+	k.RGAM = (4.0 * GAM);
+	k.SGAM = (GAM * k.AGAM);
+	k.TGAM = (k.QGAM / 2.0);
+	k.DGAM1 = (1 + k.DGAM);
+	k.IAGAM = (1 / k.AGAM);
+	k.DGGAM = k.DGAM * k.GGAM;
+	return k;
+}
+
 __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& UI, double& VI, double RB,
                                     double PB, double UB, double VB, double RE, double PE, double UE, double VE,
-                                    double GAM, int max_iter, int& hit)
+                                    const RimConst& K, int max_iter, int& hit)
 {
-	const double AGAM = (GAM - 1.0);
-	const double DGAM = (2.0 / AGAM);
-	const double GGAM = (sqrt(GAM * AGAM));
-	const double HGAM = (AGAM / 2.0);
-	const double FGAM = (3.0 * GAM - 1.0);
-	const double OGAM = (AGAM / (2.0 * GAM));
-	const double QGAM = (GAM + 1.0);
-	const double PGAM = (QGAM / (2.0 * GAM));
-	const double RGAM = (4.0 * GAM);
-	const double SGAM = (GAM * AGAM);
-	const double TGAM = (QGAM / 2.0);
+	const double GAM = K.GAM, AGAM = K.AGAM, DGAM = K.DGAM, HGAM = K.HGAM, FGAM = K.FGAM, OGAM = K.OGAM;
+	const double QGAM = K.QGAM, PGAM = K.PGAM, RGAM = K.RGAM, SGAM = K.SGAM, TGAM = K.TGAM;
 	const double eps = 1.0e-5;
 
 	double US = 0.0, UF = 0.0;
@@ -151,11 +169,10 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 				break;
 			}
 		}
-		double PPB = P / PB;
-		double PPE = P / PE;
-		double ZFB = CB * exp(log(PPB) * OGAM);
-		double ZFE = CE * exp(log(PPE) * OGAM);
+		// the reference forms both rarefaction heads before branching; each is
+		// only read on its rarefaction side, so it is formed there
 		if (PB > P) {
+			double ZFB = CB * exp(log(P / PB) * OGAM);
 			EF = ZFB * ZFB / SGAM;
 			UF = UB + DGAM * (CB - ZFB);
 			RF = P / (AGAM * EF);
@@ -172,6 +189,7 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 			SFL = D;
 		}
 		if (PE > P) {
+			double ZFE = CE * exp(log(P / PE) * OGAM);
 			ES = ZFE * ZFE / SGAM;
 			US = UE - DGAM * (CE - ZFE);
 			RS = P / (AGAM * ES);
@@ -199,29 +217,29 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 			RI = RS; EI = ES; UI = US; VI = VE;
 		}
 	} else if (SFL > 0.0) {
-		UI = (UB + DGAM * GGAM * sqrt(EB)) / (1 + DGAM);
+		UI = (UB + K.DGGAM * sqrt(EB)) / K.DGAM1;
 		VI = VB;
 		EI = (UI * UI) / SGAM;
-		RI = RB * exp(log(EI / EB) * (1 / AGAM));
+		RI = RB * exp(log(EI / EB) * K.IAGAM);
 	} else {
-		UI = (UE - DGAM * GGAM * sqrt(EE)) / (1 + DGAM);
+		UI = (UE - K.DGGAM * sqrt(EE)) / K.DGAM1;
 		VI = VE;
 		EI = (UI * UI) / SGAM;
-		RI = RE * exp(log(EI / EE) * (1 / AGAM));
+		RI = RE * exp(log(EI / EE) * K.IAGAM);
 	}
 	PI = AGAM * EI * RI;
 }
 
 // FVM_TVD::calcFlux, Godunov branch (src/methods/fvm_tvd.cpp:602-622)
 __device__ __forceinline__ void flux_godunov(double* f, const Prim& L, const Prim& R, double nx, double ny,
-                                             double GAM, int max_iter, int& hit)
+                                             const RimConst& K, int max_iter, int& hit)
 {
 	double unl = L.u * nx + L.v * ny;
 	double unr = R.u * nx + R.v * ny;
 	double utl = L.u * ny - L.v * nx;
 	double utr = R.u * ny - R.v * nx;
 	double RI, EI, PI, UN, UT;
-	rim(RI, EI, PI, UN, UT, L.r, L.p, unl, utl, R.r, R.p, unr, utr, GAM, max_iter, hit);
+	rim(RI, EI, PI, UN, UT, L.r, L.p, unl, utl, R.r, R.p, unr, utr, K, max_iter, hit);
 	double UI = UN * nx + UT * ny;
 	double VI = UN * ny - UT * nx;
 	f[0] = RI * UN;
