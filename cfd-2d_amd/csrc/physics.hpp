@@ -85,6 +85,7 @@ __device__ __forceinline__ Prim ghost_state(int type, const double* par, double 
 // formed once on the host with the same expressions instead of per call.
 struct RimConst {
 	double GAM, AGAM, DGAM, GGAM, HGAM, FGAM, OGAM, QGAM, PGAM, RGAM, SGAM, TGAM, DGAM1, IAGAM, DGGAM;
+	int gamma14;  // OGAM = 1/7 and 1/AGAM = 5/2 to rounding (GAM = 1.4, the reference's value)
 };
 
 __host__ __device__ inline RimConst rim_const(double GAM)
@@ -105,7 +106,38 @@ __host__ __device__ inline RimConst rim_const(double GAM)
 	k.DGAM1 = (1 + k.DGAM);
 	k.IAGAM = (1 / k.AGAM);
 	k.DGGAM = k.DGAM * k.GGAM;
+	k.gamma14 = fabs(7.0 * k.OGAM - 1.0) < 1e-14 && fabs(k.IAGAM - 2.5) < 1e-14;
 	return k;
+}
+
+// x^OGAM, the rarefaction power of rim_orig (computed there as
+// exp(log(x) * OGAM)).  For GAM = 1.4, OGAM is 1/7: a hardware single-
+// precision estimate of x^(1/7) refined by two Newton steps on y^7 = x,
+// y <- y + (x / y^6 - y) / 7, which is quadratically convergent from the
+// ~1e-7 estimate to within an ulp or two (about 40 instructions instead of
+// the ~160 of the double-precision log and exp).  Other gammas, and x
+// outside [1e-30, 1e30], take exp(log(x) * OGAM).
+__device__ __forceinline__ double pow_ogam(double x, const RimConst& K)
+{
+	if (K.gamma14 && x > 1e-30 && x < 1e30) {
+		float xf = (float)x;
+		double y = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf(xf) * (1.0f / 7.0f));
+		for (int it = 0; it < 2; it++) {
+			double y2 = y * y;
+			double y6 = y2 * y2 * y2;
+			y = y + (x / y6 - y) * (1.0 / 7.0);
+		}
+		return y;
+	}
+	return exp(log(x) * K.OGAM);
+}
+
+// x^(1/AGAM) of the fan density (rim_orig: exp(log(x) * (1/AGAM))); x^(5/2)
+// = x^2 sqrt(x) for GAM = 1.4
+__device__ __forceinline__ double pow_iagam(double x, const RimConst& K)
+{
+	if (K.gamma14 && x >= 0.0) return x * x * sqrt(x);
+	return exp(log(x) * K.IAGAM);
 }
 
 __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& UI, double& VI, double RB,
@@ -140,7 +172,7 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 			double F1, FS1, F2, FS2;
 			double PPB = P / PB;
 			if (PB > P) {
-				double ZFB = CB * exp(log(PPB) * OGAM);
+				double ZFB = CB * pow_ogam(PPB, K);
 				F1 = DGAM * (ZFB - CB);
 				FS1 = ZFB / (GAM * P);
 			} else {
@@ -151,7 +183,7 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 			}
 			double PPE = P / PE;
 			if (PE > P) {
-				double ZFE = CE * exp(log(PPE) * OGAM);
+				double ZFE = CE * pow_ogam(PPE, K);
 				F2 = DGAM * (ZFE - CE);
 				FS2 = ZFE / (GAM * P);
 			} else {
@@ -172,7 +204,7 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 		// the reference forms both rarefaction heads before branching; each is
 		// only read on its rarefaction side, so it is formed there
 		if (PB > P) {
-			double ZFB = CB * exp(log(P / PB) * OGAM);
+			double ZFB = CB * pow_ogam(P / PB, K);
 			EF = ZFB * ZFB / SGAM;
 			UF = UB + DGAM * (CB - ZFB);
 			RF = P / (AGAM * EF);
@@ -189,7 +221,7 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 			SFL = D;
 		}
 		if (PE > P) {
-			double ZFE = CE * exp(log(P / PE) * OGAM);
+			double ZFE = CE * pow_ogam(P / PE, K);
 			ES = ZFE * ZFE / SGAM;
 			US = UE - DGAM * (CE - ZFE);
 			RS = P / (AGAM * ES);
@@ -220,12 +252,12 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 		UI = (UB + K.DGGAM * sqrt(EB)) / K.DGAM1;
 		VI = VB;
 		EI = (UI * UI) / SGAM;
-		RI = RB * exp(log(EI / EB) * K.IAGAM);
+		RI = RB * pow_iagam(EI / EB, K);
 	} else {
 		UI = (UE - K.DGGAM * sqrt(EE)) / K.DGAM1;
 		VI = VE;
 		EI = (UI * UI) / SGAM;
-		RI = RE * exp(log(EI / EE) * K.IAGAM);
+		RI = RE * pow_iagam(EI / EE, K);
 	}
 	PI = AGAM * EI * RI;
 }
