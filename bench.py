@@ -86,6 +86,48 @@ def reference_cpu(steps=100):
                 % steps)
 
 
+def batched_vs_serial(n_cases, steps, seed=7):
+    """Config-3 style batch: n_cases bump channels (2k-60k cells, log-uniform,
+    inflow Mach 0.2-3.0, angle -5..5 deg; Euler, exact Godunov, minmod, RK2)
+    advanced together in one index space, against the same cases advanced one
+    at a time (eager launches, and replayed HIP graphs)."""
+    from cfd2d import batch
+    from cfd2d import case as cases
+    from cfd2d import meshgen
+    specs = meshgen.bump_batch(n_cases, seed=seed)
+    cs = [cases.bump_channel(s["ny"], height=s["height"], mach=s["mach"], angle_deg=s["angle_deg"],
+                             seed=s["seed"]) for s in specs]
+    cells = sum(c.mesh.n_cells for c in cs)
+    b = batch.BatchSolver(cs)
+    b.run(2)
+    t0 = time.perf_counter()
+    b.run(steps)
+    t_batch = time.perf_counter() - t0
+    b.close()
+    from cfd2d import native
+    solvers = [native.GpuSolver(c) for c in cs]
+    for s in solvers:
+        s.run(2)
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.step(steps)
+        s.sync()
+    t_serial = time.perf_counter() - t0
+    for s in solvers:
+        s.run(2, graph=True)
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.step_graph(steps)
+        s.sync()
+    t_graph = time.perf_counter() - t0
+    for s in solvers:
+        s.close()
+    rate = lambda t: cells * steps / t
+    return dict(cases=n_cases, cells=cells, steps=steps, batched=rate(t_batch), serial_eager=rate(t_serial),
+                serial_graph=rate(t_graph), ratio_vs_eager=t_serial / t_batch, ratio_vs_graph=t_graph / t_batch,
+                unit="cell-updates/s")
+
+
 def load_traffic(workload):
     """HBM bytes per launch of each kernel from the committed rocprofv3 PMC summary."""
     path = os.path.join(REPO, "profiles", "traffic.json")
@@ -106,6 +148,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay each step as a recorded HIP graph")
+    ap.add_argument("--batch-cases", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=50)
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -163,6 +208,9 @@ def main():
         "kernels": per,
         "newton_cap_hits": hits, "nonfinite": nonfinite,
     }
+    if not args.no_batch:
+        out["batched"] = batched_vs_serial(args.batch_cases, args.batch_steps)
+        log("batched: %s" % out["batched"])
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case_fn, args.cpu_steps)
         ref = reference_cpu()
