@@ -76,6 +76,8 @@ SIGNATURES = {
     "cfd2d_sync": (C.c_int, [C.c_void_p]),
     "cfd2d_status": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_int)]),
     "cfd2d_wall_force": (C.c_int, [C.c_void_p, _ip, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "cfd2d_adjoint": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip, C.POINTER(C.c_double), C.c_void_p,
+                                C.c_void_p]),
     "cfd2d_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "cfd2d_timing_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "cfd2d_ctx_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -221,6 +223,7 @@ class Context:
         check(L.cfd2d_ctx_create(mesh._h, C.byref(p), len(bcs), bt, bp, gbc, cm, C.byref(h)))
         self._h = h
         self.nc, self.ne = mesh.nc, mesh.ne
+        self.n_bc = len(bcs)
 
     def set_state(self, U):
         check(lib().cfd2d_set_state(self._h, np.ascontiguousarray(U, np.float64).ravel()))
@@ -289,6 +292,17 @@ class Context:
         fx, fy = C.c_double(), C.c_double()
         check(lib().cfd2d_wall_force(self._h, np.ascontiguousarray(edge_mask, np.int32), C.byref(fx), C.byref(fy)))
         return fx.value, fy.value
+
+    def adjoint(self, nsteps, edge_mask, ckpt_every=20):
+        """Runs nsteps steps and the reverse sweep for J = sum over masked
+        edges of p(c1) * l * n_y; returns (J, dJ/dU0 as (4, nc), dJ/d(boundary
+        parameters) as (n_bc, 4)).  The context ends at the final state."""
+        J = C.c_double()
+        gU = np.zeros(4 * self.nc)
+        gb = np.zeros(4 * max(self.n_bc, 1))
+        check(lib().cfd2d_adjoint(self._h, int(nsteps), int(ckpt_every), np.ascontiguousarray(edge_mask, np.int32),
+                                  C.byref(J), gU.ctypes.data_as(C.c_void_p), gb.ctypes.data_as(C.c_void_p)))
+        return J.value, gU.reshape(4, self.nc), gb.reshape(-1, 4)[:self.n_bc]
 
     def timing(self, enable=True):
         check(lib().cfd2d_timing(self._h, 1 if enable else 0))

@@ -1,0 +1,622 @@
+// Reverse mode (discrete adjoint) of the explicit step.  Included at the end
+// of solver.hip, whose kernels, buffers and context it reuses.
+//
+// Objective J = sum over the masked edges of p(c1) * l * n_y after nsteps
+// steps (the pressure force of FVM_TVD_IMPLICIT::calcLiftForce,
+// src/methods/fvm_tvd_implicit.cpp:1088-1110, y component).  The reverse
+// sweep gives dJ/dU0 (every cell) and dJ/d(boundary parameters).  Scope:
+// LAX or HLLC flux, Green-Gauss or LSQ gradients, Heun RK2 or SSP-RK3, all
+// boundary types, periodic pairing; no limiter, no viscous terms; the time
+// step is held at the value in effect (it is not differentiated).
+//
+// Per RK stage with input state U_s, U_{s+1} = a U0 + b (U_s + tau/S R(U_s)):
+//   given W = dJ/dU_{s+1}:  acc0 += a W,  dJ/dU_s = b W + (dR/dU_s)^T (b tau/S W)
+// The vector-Jacobian product of R runs as: primitives and gradients of U_s
+// (forward kernels), k_adj_edge (per edge: reverse of reconstruction, flux
+// and ghost into a per-edge buffer, both sides), k_adj_cell (each cell
+// gathers its faces' entries in ascending edge order), k_adj_grad (reverse of
+// the gradient operator: own and neighbours' gradient adjoints, ghost terms,
+// then the primitive->conserved Jacobian and the RK bookkeeping).  Boundary
+// parameter adjoints are summed per boundary object by one workgroup in edge
+// order.  No floating-point atomics: the result does not depend on
+// scheduling.
+//
+// Checkpoints: the forward pass keeps the state every ckpt steps; the reverse
+// sweep recomputes one segment at a time, recording the input state of every
+// RK stage, and walks it backwards.
+
+namespace {
+
+constexpr int EA = 24;  // per edge: c1 side (P 4, G 8), c2 side (P 4, G 8)
+
+struct AdjBufs {
+	double* Wn;    // dJ/dU_{s+1}
+	double* Ws;    // dJ/dU_s
+	double* Acc;   // sum of a * W over the stages of the current step
+	double* Pb;    // [4][nc]
+	double* Gb;    // [8][nc]
+	double* Ea;    // [24][ne]
+	double* Ebc;   // [4][ne] boundary-parameter adjoints from the edge pass
+	double* Ebc2;  // [4][ne] ... from the gradient pass (ghosts of cell centres)
+	double* bcbar; // [nbc][4]
+	int* cslot;    // [3][nc] slot of this cell in its neighbour's face list
+	int* bedges;   // boundary edges in ascending order
+	int nbedges;
+};
+
+// d(ghost)/d(inside L, centre C, parameters) applied to gb (r, p, u, v order):
+// the reverse of ghost_state
+__device__ __forceinline__ void ghost_vjp(int type, const double* par, double nx, double ny, const Prim& L,
+                                          const Prim& C, const MatV& m, const double* gb, double* Lb, double* Cb,
+                                          double* parb)
+{
+	if (type == BND_INLET) {
+		double T = par[2], P = par[3];
+		parb[0] += gb[2];
+		parb[1] += gb[3];
+		parb[3] += gb[1] + gb[0] * m.M / (T * GAS_R);
+		parb[2] += -gb[0] * P * m.M / (T * T * GAS_R);
+		return;
+	}
+	if (type == BND_PRESSURE_OUTLET) {
+		double un = L.u * nx + L.v * ny;
+		double czc = sqrt(m.gam * C.p / C.r);
+		Lb[0] += gb[0];
+		Lb[2] += gb[2];
+		Lb[3] += gb[3];
+		if (fabs(un) < czc) parb[0] += gb[1];
+		else Lb[1] += gb[1];
+		return;
+	}
+	// outlet and walls: r = pL M / (Tc gR), Tc = pc / (rc (gamma-1) Cv)
+	double Tc = temperature(C, m);
+	double k = m.M / GAS_R;
+	Lb[1] += gb[0] * k / Tc + gb[1];
+	double Tb = -gb[0] * L.p * k / (Tc * Tc);
+	double den = C.r * (m.gam - 1) * m.Cv;
+	Cb[1] += Tb / den;
+	Cb[0] += -Tb * C.p / (C.r * den);
+	if (type == BND_OUTLET) {
+		Lb[2] += gb[2];
+		Lb[3] += gb[3];
+	} else {
+		double Unb = -2.0 * (nx * gb[2] + ny * gb[3]);
+		Lb[2] += gb[2] + Unb * nx;
+		Lb[3] += gb[3] + Unb * ny;
+	}
+}
+
+// reverse of flux_lax: accumulates into Lb, Rb (r, p, u, v) the product of
+// fb with the flux Jacobian
+__device__ __forceinline__ void lax_vjp(const Prim& L, const Prim& R, double nx, double ny, double G, const double* fb,
+                                        double* Lb, double* Rb)
+{
+	double unl = L.u * nx + L.v * ny, unr = R.u * nx + R.v * ny;
+	double cl = sqrt(G * L.p / L.r), cr = sqrt(G * R.p / R.r);
+	double al = fabs(unl) + cl, ar = fabs(unr) + cr;
+	bool left = al > ar;
+	double alpha = left ? al : ar;
+	double rol = L.r, rul = L.r * L.u, rvl = L.r * L.v, rel = L.p / (G - 1.0) + 0.5 * L.r * (L.u * L.u + L.v * L.v);
+	double ror = R.r, rur = R.r * R.u, rvr = R.r * R.v, rer = R.p / (G - 1.0) + 0.5 * R.r * (R.u * R.u + R.v * R.v);
+	double frl = rol * unl, frr = ror * unr;
+	double h0 = 0.5 * fb[0], h1 = 0.5 * fb[1], h2 = 0.5 * fb[2], h3 = 0.5 * fb[3];
+	double alb = -(h0 * (ror - rol) + h1 * (rur - rul) + h2 * (rvr - rvl) + h3 * (rer - rel));
+	double frrb = h0 + h1 * R.u + h2 * R.v;
+	double frlb = h0 + h1 * L.u + h2 * L.v;
+	double Rub = h1 * frr, Lub = h1 * frl, Rvb = h2 * frr, Lvb = h2 * frl;
+	double Rpb = h1 * nx + h2 * ny + h3 * unr, Lpb = h1 * nx + h2 * ny + h3 * unl;
+	double unrb = h3 * (rer + R.p), unlb = h3 * (rel + L.p);
+	double rorb = -h0 * alpha, rolb = h0 * alpha;
+	double rurb = -h1 * alpha, rulb = h1 * alpha;
+	double rvrb = -h2 * alpha, rvlb = h2 * alpha;
+	double rerb = h3 * unr - h3 * alpha, relb = h3 * unl + h3 * alpha;
+	rorb += frrb * unr;
+	unrb += frrb * ror;
+	rolb += frlb * unl;
+	unlb += frlb * rol;
+	double clb = 0.0, crb = 0.0;
+	if (left) {
+		unlb += alb * (unl < 0 ? -1.0 : 1.0);
+		clb = alb;
+	} else {
+		unrb += alb * (unr < 0 ? -1.0 : 1.0);
+		crb = alb;
+	}
+	Lpb += clb * G / (2.0 * L.r * cl);
+	double Lrb = -clb * cl / (2.0 * L.r);
+	Rpb += crb * G / (2.0 * R.r * cr);
+	double Rrb = -crb * cr / (2.0 * R.r);
+	Lrb += rolb + rulb * L.u + rvlb * L.v + relb * 0.5 * (L.u * L.u + L.v * L.v);
+	Lub += rulb * L.r + relb * L.r * L.u;
+	Lvb += rvlb * L.r + relb * L.r * L.v;
+	Lpb += relb / (G - 1.0);
+	Rrb += rorb + rurb * R.u + rvrb * R.v + rerb * 0.5 * (R.u * R.u + R.v * R.v);
+	Rub += rurb * R.r + rerb * R.r * R.u;
+	Rvb += rvrb * R.r + rerb * R.r * R.v;
+	Rpb += rerb / (G - 1.0);
+	Lub += unlb * nx;
+	Lvb += unlb * ny;
+	Rub += unrb * nx;
+	Rvb += unrb * ny;
+	Lb[0] += Lrb; Lb[1] += Lpb; Lb[2] += Lub; Lb[3] += Lvb;
+	Rb[0] += Rrb; Rb[1] += Rpb; Rb[2] += Rub; Rb[3] += Rvb;
+}
+
+// one-direction forward-mode number for the HLLC Jacobian
+struct D1 {
+	double v, d;
+};
+__device__ __forceinline__ D1 operator+(D1 a, D1 b) { return D1{a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ D1 operator-(D1 a, D1 b) { return D1{a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ D1 operator*(D1 a, D1 b) { return D1{a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ D1 operator/(D1 a, D1 b) { return D1{a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+__device__ __forceinline__ D1 operator*(double a, D1 b) { return D1{a * b.v, a * b.d}; }
+__device__ __forceinline__ D1 operator+(D1 a, double b) { return D1{a.v + b, a.d}; }
+__device__ __forceinline__ D1 operator/(D1 a, double b) { return D1{a.v / b, a.d / b}; }
+__device__ __forceinline__ D1 dsqrt(D1 a)
+{
+	double s = sqrt(a.v);
+	return D1{s, a.d / (2.0 * s)};
+}
+
+// HLLC (the same formulas as flux_hllc) on one-direction duals
+__device__ __forceinline__ void hllc_d1(D1* f, const D1* L, const D1* R, double nx, double ny, double G)
+{
+	D1 unl = L[2] * D1{nx, 0} + L[3] * D1{ny, 0}, unr = R[2] * D1{nx, 0} + R[3] * D1{ny, 0};
+	D1 cl = dsqrt(G * L[1] / L[0]), cr = dsqrt(G * R[1] / R[0]);
+	D1 El = L[1] / (G - 1.0) + 0.5 * L[0] * (L[2] * L[2] + L[3] * L[3]);
+	D1 Er = R[1] / (G - 1.0) + 0.5 * R[0] * (R[2] * R[2] + R[3] * R[3]);
+	D1 a1 = unl - cl, a2 = unr - cr, b1 = unl + cl, b2 = unr + cr;
+	D1 sl = a1.v < a2.v ? a1 : a2;
+	D1 sr = b1.v > b2.v ? b1 : b2;
+	if (sl.v >= 0.0) {
+		f[0] = L[0] * unl;
+		f[1] = L[0] * L[2] * unl + L[1] * D1{nx, 0};
+		f[2] = L[0] * L[3] * unl + L[1] * D1{ny, 0};
+		f[3] = (El + L[1]) * unl;
+		return;
+	}
+	if (sr.v <= 0.0) {
+		f[0] = R[0] * unr;
+		f[1] = R[0] * R[2] * unr + R[1] * D1{nx, 0};
+		f[2] = R[0] * R[3] * unr + R[1] * D1{ny, 0};
+		f[3] = (Er + R[1]) * unr;
+		return;
+	}
+	D1 ml = L[0] * (sl - unl), mr = R[0] * (sr - unr);
+	D1 ss = (R[1] - L[1] + L[0] * unl * (sl - unl) - R[0] * unr * (sr - unr)) / (ml - mr);
+	bool lft = ss.v >= 0.0;
+	const D1* K = lft ? L : R;
+	D1 sk = lft ? sl : sr, unk = lft ? unl : unr, Ek = lft ? El : Er;
+	D1 k = (lft ? ml : mr) / (sk - ss);
+	D1 us0 = k, us1 = k * (K[2] + (ss - unk) * D1{nx, 0}), us2 = k * (K[3] + (ss - unk) * D1{ny, 0});
+	D1 us3 = k * (Ek / K[0] + (ss - unk) * (ss + K[1] / (K[0] * (sk - unk))));
+	D1 fk0 = K[0] * unk, fk1 = K[0] * K[2] * unk + K[1] * D1{nx, 0}, fk2 = K[0] * K[3] * unk + K[1] * D1{ny, 0};
+	D1 fk3 = (Ek + K[1]) * unk;
+	f[0] = fk0 + sk * (us0 - K[0]);
+	f[1] = fk1 + sk * (us1 - K[0] * K[2]);
+	f[2] = fk2 + sk * (us2 - K[0] * K[3]);
+	f[3] = fk3 + sk * (us3 - Ek);
+}
+
+__device__ __forceinline__ void hllc_vjp(const Prim& L, const Prim& R, double nx, double ny, double G, const double* fb,
+                                         double* Lb, double* Rb)
+{
+	const double lv[4] = {L.r, L.p, L.u, L.v}, rv[4] = {R.r, R.p, R.u, R.v};
+	for (int j = 0; j < 8; j++) {
+		D1 l[4], r[4], f[4];
+		for (int q = 0; q < 4; q++) {
+			l[q] = D1{lv[q], (j == q) ? 1.0 : 0.0};
+			r[q] = D1{rv[q], (j == 4 + q) ? 1.0 : 0.0};
+		}
+		hllc_d1(f, l, r, nx, ny, G);
+		double s = fb[0] * f[0].d + fb[1] * f[1].d + fb[2] * f[2].d + fb[3] * f[3].d;
+		if (j < 4) Lb[j] += s;
+		else Rb[j - 4] += s;
+	}
+}
+
+// Reverse of the edge pass for one stage: seed F_e-bar = (Rb_c2 - Rb_c1) l/2
+// with Rb = scale * tau/S * W; per Gauss point the flux VJP, the ghost VJP on
+// boundaries, and the reverse of the reconstruction into (P, G) adjoints of
+// both sides, stored per edge.
+template <bool MM, int FLUX>
+__global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, double scale)
+{
+	int e = xcd_block() * BLOCK + threadIdx.x;
+	if (e >= b.ne) return;
+	size_t ne = b.ne, n = b.nc;
+	int c1 = b.ec1[e], c2 = b.ec2[e];
+	double nx = b.enx[e], ny = b.eny[e];
+	double hl = b.el[e] * 0.5;
+	double fb[4];
+	for (int q = 0; q < 4; q++) {
+		double r1 = scale * b.dtS[c1] * a.Wn[q * n + c1];
+		double r2 = c2 >= 0 ? scale * b.dtS[c2] * a.Wn[q * n + c2] : 0.0;
+		fb[q] = (r2 - r1) * hl;
+	}
+	Prim p1 = load_prim(b, c1);
+	Prim p2 = c2 >= 0 ? load_prim(b, c2) : p1;
+	double out[EA];
+	for (int k = 0; k < EA; k++) out[k] = 0.0;
+	double parb[4] = {0.0, 0.0, 0.0, 0.0};
+	for (int gp = 0; gp < 2; gp++) {
+		double d1x = b.edl[(size_t)(2 * gp) * ne + e], d1y = b.edl[(size_t)(2 * gp + 1) * ne + e];
+		Prim L = p1, R;
+		recon(L, b, c1, d1x, d1y);
+		double d2x = 0.0, d2y = 0.0;
+		if (c2 >= 0) {
+			d2x = b.edl[(size_t)(4 + 2 * gp) * ne + e];
+			d2y = b.edl[(size_t)(5 + 2 * gp) * ne + e];
+			R = p2;
+			recon(R, b, c2, d2x, d2y);
+		} else {
+			R = ghost_of<MM>(b, ph, e, c1, L, p1);
+		}
+		double Lb[4] = {0, 0, 0, 0}, Rb[4] = {0, 0, 0, 0};
+		if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb);
+		else lax_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb);
+		if (c2 < 0) {
+			int bc = -c2 - 1;
+			double Cb[4] = {0, 0, 0, 0};
+			ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], nx, ny, L, p1, mat_of<MM>(b, ph, c1), Rb, Lb, Cb, parb);
+			for (int q = 0; q < 4; q++) out[q] += Cb[q];
+		} else {
+			for (int q = 0; q < 4; q++) {
+				out[12 + q] += Rb[q];
+				out[16 + 2 * q] += Rb[q] * d2x;
+				out[17 + 2 * q] += Rb[q] * d2y;
+			}
+		}
+		for (int q = 0; q < 4; q++) {
+			out[q] += Lb[q];
+			out[4 + 2 * q] += Lb[q] * d1x;
+			out[5 + 2 * q] += Lb[q] * d1y;
+		}
+	}
+	for (int k = 0; k < EA; k++) a.Ea[(size_t)k * ne + e] = out[k];
+	if (c2 < 0)
+		for (int q = 0; q < 4; q++) a.Ebc[(size_t)q * ne + e] = parb[q];
+}
+
+// each cell gathers the (P, G) adjoints of its side of its three edges
+__global__ __launch_bounds__(BLOCK) void k_adj_cell(Bufs b, AdjBufs a)
+{
+	int i = xcd_block() * BLOCK + threadIdx.x;
+	if (i >= b.nc) return;
+	size_t n = b.nc, ne = b.ne;
+	double acc[12];
+	for (int k = 0; k < 12; k++) acc[k] = 0.0;
+	for (int k = 0; k < 3; k++) {
+		int f = b.cface[k * n + i];
+		int e = f >> 1, off = (f & 1) ? 12 : 0;
+		for (int q = 0; q < 12; q++) acc[q] += a.Ea[(size_t)(off + q) * ne + e];
+	}
+	for (int q = 0; q < 4; q++) a.Pb[q * n + i] = acc[q];
+	for (int q = 0; q < 8; q++) a.Gb[q * n + i] = acc[4 + q];
+}
+
+// Reverse of the gradient operator and of the primitive conversion, then the
+// RK bookkeeping: Ws = b W + (dP/dU)^T Pb, and acc0 per mode
+// (0: Acc = a W, 1: Acc += a W, 2: Ws += Acc, the step's first stage).
+template <bool MM, bool LSQ>
+__global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, double wa, double wb, int mode)
+{
+	int i = xcd_block() * BLOCK + threadIdx.x;
+	if (i >= b.nc) return;
+	size_t n = b.nc, ne = b.ne;
+	Prim pc = load_prim(b, i);
+	MatV m = mat_of<MM>(b, ph, i);
+	double pb[4];
+	for (int q = 0; q < 4; q++) pb[q] = a.Pb[q * n + i];
+	double gb[8];
+	for (int q = 0; q < 8; q++) gb[q] = a.Gb[q * n + i];
+	double si = b.S[i];
+	for (int k = 0; k < 3; k++) {
+		int f = b.cface[k * n + i];
+		int e = f >> 1, side = f & 1;
+		int j = b.cnb[k * n + i];
+		double nbb[4];  // adjoint of the neighbour value this face contributed
+		double self[4];
+		if (LSQ) {
+			double wx = b.lsqw[(size_t)(2 * k) * n + i], wy = b.lsqw[(size_t)(2 * k + 1) * n + i];
+			for (int q = 0; q < 4; q++) {
+				nbb[q] = wx * gb[2 * q] + wy * gb[2 * q + 1];
+				self[q] = -nbb[q];
+			}
+		} else {
+			double nx = b.enx[e], ny = b.eny[e], l = b.el[e];
+			double sg = side ? -1.0 : 1.0;
+			for (int q = 0; q < 4; q++) {
+				nbb[q] = sg * 0.5 * l * (nx * gb[2 * q] + ny * gb[2 * q + 1]) / si;
+				self[q] = nbb[q];
+			}
+		}
+		for (int q = 0; q < 4; q++) pb[q] += self[q];
+		if (j >= 0) {
+			// the neighbour's gradient depends on this cell's value
+			int s = a.cslot[k * n + i];
+			double gj[8];
+			for (int q = 0; q < 8; q++) gj[q] = a.Gb[q * n + j];
+			if (LSQ) {
+				double wx = b.lsqw[(size_t)(2 * s) * n + j], wy = b.lsqw[(size_t)(2 * s + 1) * n + j];
+				for (int q = 0; q < 4; q++) pb[q] += wx * gj[2 * q] + wy * gj[2 * q + 1];
+			} else {
+				double nx = b.enx[e], ny = b.eny[e], l = b.el[e];
+				double sg = side ? 1.0 : -1.0;  // the neighbour sits on the other side
+				double sj = b.S[j];
+				for (int q = 0; q < 4; q++) pb[q] += sg * 0.5 * l * (nx * gj[2 * q] + ny * gj[2 * q + 1]) / sj;
+			}
+		} else {
+			// ghost of the cell-centre state across a boundary edge
+			int bc = -b.ec2[e] - 1;
+			double Lb[4] = {0, 0, 0, 0}, Cb[4] = {0, 0, 0, 0}, parb[4] = {0, 0, 0, 0};
+			ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], b.enx[e], b.eny[e], pc, pc, m, nbb, Lb, Cb, parb);
+			for (int q = 0; q < 4; q++) {
+				pb[q] += Lb[q] + Cb[q];
+				a.Ebc2[(size_t)q * ne + e] = parb[q];
+			}
+		}
+	}
+	// P = (r, p, u, v)(U): the transposed Jacobian
+	double r = pc.r, u = pc.u, v = pc.v, g1 = m.gam - 1.0;
+	double ub[4];
+	ub[0] = pb[0] - pb[2] * u / r - pb[3] * v / r + pb[1] * g1 * 0.5 * (u * u + v * v);
+	ub[1] = pb[2] / r - pb[1] * g1 * u;
+	ub[2] = pb[3] / r - pb[1] * g1 * v;
+	ub[3] = pb[1] * g1;
+	for (int q = 0; q < 4; q++) {
+		double w = a.Wn[q * n + i];
+		double ws = wb * w + ub[q];
+		if (mode == 0) a.Acc[q * n + i] = wa * w;
+		else if (mode == 1) a.Acc[q * n + i] += wa * w;
+		else ws += a.Acc[q * n + i];
+		a.Ws[q * n + i] = ws;
+	}
+}
+
+// boundary-parameter adjoints: one workgroup sums the per-edge entries of
+// each boundary object in ascending edge order (deterministic)
+__global__ __launch_bounds__(BLOCK) void k_bc_reduce(Bufs b, AdjBufs a, int nbc)
+{
+	__shared__ double part[BLOCK][4];
+	size_t ne = b.ne;
+	for (int bc = 0; bc < nbc; bc++) {
+		double s[4] = {0, 0, 0, 0};
+		for (int k = threadIdx.x; k < a.nbedges; k += BLOCK) {
+			int e = a.bedges[k];
+			if (-b.ec2[e] - 1 != bc) continue;
+			for (int q = 0; q < 4; q++) s[q] += a.Ebc[(size_t)q * ne + e] + a.Ebc2[(size_t)q * ne + e];
+		}
+		for (int q = 0; q < 4; q++) part[threadIdx.x][q] = s[q];
+		__syncthreads();
+		if (threadIdx.x == 0) {
+			double t[4] = {0, 0, 0, 0};
+			for (int k = 0; k < BLOCK; k++)
+				for (int q = 0; q < 4; q++) t[q] += part[k][q];
+			for (int q = 0; q < 4; q++) a.bcbar[4 * bc + q] += t[q];
+		}
+		__syncthreads();
+	}
+}
+
+}  // namespace
+
+struct cfd2d_adj {
+	AdjBufs a{};
+	std::vector<void*> allocs;
+	std::vector<double*> ckpt;   // states every ckpt steps
+	std::vector<double*> stages; // stage input states of one segment
+};
+
+namespace {
+
+int adj_alloc(cfd2d_adj* A, double** p, size_t count)
+{
+	void* q = nullptr;
+	hipError_t e = hipMalloc(&q, (count ? count : 1) * sizeof(double));
+	if (e != hipSuccess) return hip_fail(e, "hipMalloc (adjoint)");
+	A->allocs.push_back(q);
+	*p = (double*)q;
+	return 0;
+}
+
+int adj_alloc_int(cfd2d_adj* A, int** p, size_t count)
+{
+	void* q = nullptr;
+	hipError_t e = hipMalloc(&q, (count ? count : 1) * sizeof(int));
+	if (e != hipSuccess) return hip_fail(e, "hipMalloc (adjoint)");
+	A->allocs.push_back(q);
+	*p = (int*)q;
+	return 0;
+}
+
+void adj_free(cfd2d_adj* A)
+{
+	if (!A) return;
+	for (void* p : A->allocs) (void)hipFree(p);
+	delete A;
+}
+
+template <bool MM>
+void launch_adj_stage(cfd2d_ctx* c, cfd2d_adj* A, double wa, double wb, int mode)
+{
+	// primitives and gradients of the stage input (state in c->b.U)
+	launch_prim<MM>(c);
+	launch_gradient<MM>(c);
+	if (c->p.flux == CFD2D_FLUX_HLLC)
+		hipLaunchKernelGGL((k_adj_edge<MM, CFD2D_FLUX_HLLC>), dim3(grid_of(c->ne)), dim3(BLOCK), 0, c->st, c->b, c->ph,
+		                   A->a, wb);
+	else
+		hipLaunchKernelGGL((k_adj_edge<MM, CFD2D_FLUX_LAX>), dim3(grid_of(c->ne)), dim3(BLOCK), 0, c->st, c->b, c->ph,
+		                   A->a, wb);
+	hipLaunchKernelGGL(k_adj_cell, dim3(grid_of(c->nc)), dim3(BLOCK), 0, c->st, c->b, A->a);
+	if (c->p.gradient == CFD2D_GRAD_LSQ)
+		hipLaunchKernelGGL((k_adj_grad<MM, true>), dim3(grid_of(c->nc)), dim3(BLOCK), 0, c->st, c->b, c->ph, A->a, wa,
+		                   wb, mode);
+	else
+		hipLaunchKernelGGL((k_adj_grad<MM, false>), dim3(grid_of(c->nc)), dim3(BLOCK), 0, c->st, c->b, c->ph, A->a, wa,
+		                   wb, mode);
+	if (A->a.nbedges > 0)
+		hipLaunchKernelGGL(k_bc_reduce, dim3(1), dim3(BLOCK), 0, c->st, c->b, A->a, c->nbc);
+}
+
+// one forward step that records the input state of every RK stage
+template <bool MM>
+void step_record(cfd2d_ctx* c, double* const* rec)
+{
+	size_t bytes = 4 * (size_t)c->nc * sizeof(double);
+	int S = c->p.rk_stages;
+	const int modes2[2] = {UPD_FIRST, UPD_RK2_LAST}, modes3[3] = {UPD_FIRST, UPD_RK3_MID, UPD_RK3_LAST};
+	for (int s = 0; s < S; s++) {
+		(void)hipMemcpyAsync(rec[s], c->b.U, bytes, hipMemcpyDeviceToDevice, c->st);
+		launch_gradient<MM>(c);
+		launch_edge<MM>(c);
+		launch_update<MM>(c, S == 3 ? modes3[s] : modes2[s]);
+	}
+	hipLaunchKernelGGL((k_remediate<MM>), dim3(1), dim3(BLOCK), 0, c->st, c->b, c->ph);
+}
+
+using AdjGuard = std::unique_ptr<cfd2d_adj, void (*)(cfd2d_adj*)>;
+
+template <bool MM>
+int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, double* dJdU0, double* dJdbc)
+{
+	int nc = c->nc, ne = c->ne, S = c->p.rk_stages;
+	size_t n4 = 4 * (size_t)nc, bytes = n4 * sizeof(double);
+	auto* A0 = new (std::nothrow) cfd2d_adj;
+	if (!A0) return fail(CFD2D_ERR_ARG, "out of host memory");
+	AdjGuard A(A0, adj_free);
+	int rc = 0;
+	int nck = (nsteps + ck - 1) / ck;
+	int seg = std::min(ck, nsteps);
+	A->ckpt.resize(nck);
+	A->stages.resize((size_t)seg * S);
+	for (int k = 0; k < nck && !rc; k++) rc = adj_alloc(A.get(), &A->ckpt[k], n4);
+	for (size_t k = 0; k < A->stages.size() && !rc; k++) rc = adj_alloc(A.get(), &A->stages[k], n4);
+	AdjBufs& a = A->a;
+	if (!rc) rc = adj_alloc(A.get(), &a.Wn, n4);
+	if (!rc) rc = adj_alloc(A.get(), &a.Ws, n4);
+	if (!rc) rc = adj_alloc(A.get(), &a.Acc, n4);
+	if (!rc) rc = adj_alloc(A.get(), &a.Pb, n4);
+	if (!rc) rc = adj_alloc(A.get(), &a.Gb, 2 * n4);
+	if (!rc) rc = adj_alloc(A.get(), &a.Ea, (size_t)EA * ne);
+	if (!rc) rc = adj_alloc(A.get(), &a.Ebc, 4 * (size_t)ne);
+	if (!rc) rc = adj_alloc(A.get(), &a.Ebc2, 4 * (size_t)ne);
+	if (!rc) rc = adj_alloc(A.get(), &a.bcbar, 4 * (size_t)(c->nbc > 0 ? c->nbc : 1));
+	if (rc) return rc;
+	// host-side topology: the slot of each cell in its neighbours' face
+	// lists, and the boundary edges in ascending order
+	std::vector<int> cface(3 * (size_t)nc), cnb(3 * (size_t)nc), ec2(ne);
+	HIPCHK(hipMemcpy(cface.data(), c->b.cface, cface.size() * sizeof(int), hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(cnb.data(), c->b.cnb, cnb.size() * sizeof(int), hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(ec2.data(), c->b.ec2, ec2.size() * sizeof(int), hipMemcpyDeviceToHost));
+	std::vector<int> cslot(3 * (size_t)nc, 0), bed;
+	for (int i = 0; i < nc; i++)
+		for (int k = 0; k < 3; k++) {
+			int j = cnb[(size_t)k * nc + i];
+			if (j < 0) continue;
+			int e = cface[(size_t)k * nc + i] >> 1;
+			int found = -1;
+			for (int s = 0; s < 3; s++)
+				if ((cface[(size_t)s * nc + j] >> 1) == e) found = s;
+			if (found < 0) return fail(CFD2D_ERR_MESH, "adjoint: inconsistent face lists");
+			cslot[(size_t)k * nc + i] = found;
+		}
+	for (int e = 0; e < ne; e++)
+		if (ec2[e] < 0) bed.push_back(e);
+	if ((rc = adj_alloc_int(A.get(), &a.cslot, cslot.size()))) return rc;
+	if ((rc = adj_alloc_int(A.get(), &a.bedges, bed.size()))) return rc;
+	a.nbedges = (int)bed.size();
+	HIPCHK(hipMemcpy(a.cslot, cslot.data(), cslot.size() * sizeof(int), hipMemcpyHostToDevice));
+	if (!bed.empty()) HIPCHK(hipMemcpy(a.bedges, bed.data(), bed.size() * sizeof(int), hipMemcpyHostToDevice));
+	std::vector<double> zb(4 * (size_t)(c->nbc > 0 ? c->nbc : 1), 0.0);
+	HIPCHK(hipMemcpy(a.bcbar, zb.data(), zb.size() * sizeof(double), hipMemcpyHostToDevice));
+	std::vector<double> ze(4 * (size_t)ne, 0.0);
+	HIPCHK(hipMemcpy(a.Ebc, ze.data(), ze.size() * sizeof(double), hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(a.Ebc2, ze.data(), ze.size() * sizeof(double), hipMemcpyHostToDevice));
+
+	// forward pass, keeping the state every ck steps
+	for (int s = 0; s < nsteps; s++) {
+		if (s % ck == 0) HIPCHK(hipMemcpyAsync(A->ckpt[s / ck], c->b.U, bytes, hipMemcpyDeviceToDevice, c->st));
+		step_once(c);
+	}
+	HIPCHK(hipStreamSynchronize(c->st));
+	if ((rc = check_launch("adjoint forward pass"))) return rc;
+	// objective and its gradient at the final state (p of c1 = (gamma-1)(E - |m|^2/(2 r)))
+	std::vector<double> U(n4), P(n4), W(n4, 0.0), el(ne), eny(ne), mt(5 * (size_t)c->p.nmat);
+	std::vector<int> c1(ne), cm(nc);
+	std::vector<unsigned> fl(nc);
+	HIPCHK(hipMemcpy(U.data(), c->b.U, bytes, hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(P.data(), c->b.P, bytes, hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(el.data(), c->b.el, ne * sizeof(double), hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(eny.data(), c->b.eny, ne * sizeof(double), hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(c1.data(), c->b.ec1, ne * sizeof(int), hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(fl.data(), c->b.flags, nc * sizeof(unsigned), hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(mt.data(), c->b.mtab, mt.size() * sizeof(double), hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(cm.data(), c->b.cmat, nc * sizeof(int), hipMemcpyDeviceToHost));
+	for (int i = 0; i < nc; i++)
+		if (fl[i] & 0x2u) return fail(CFD2D_ERR_CONFIG, "adjoint: a cell left the bounds (LIM flag)");
+	double Jv = 0.0;
+	for (int e = 0; e < ne; e++) {
+		if (!mask[e]) continue;
+		int i = c1[e];
+		Jv += P[(size_t)nc + i] * el[e] * eny[e];
+		double g1 = mt[5 * (size_t)cm[i] + 2] - 1.0;
+		double u = P[2 * (size_t)nc + i], v = P[3 * (size_t)nc + i];
+		double w = el[e] * eny[e];
+		W[i] += w * g1 * 0.5 * (u * u + v * v);
+		W[(size_t)nc + i] += -w * g1 * u;
+		W[2 * (size_t)nc + i] += -w * g1 * v;
+		W[3 * (size_t)nc + i] += w * g1;
+	}
+	if (J) *J = Jv;
+	HIPCHK(hipMemcpy(a.Wn, W.data(), bytes, hipMemcpyHostToDevice));
+	// reverse sweep, one segment at a time: recompute the segment from its
+	// checkpoint recording every stage input, then walk it backwards
+	const double wa2[2] = {0.0, 0.5}, wb2[2] = {1.0, 0.5};
+	const double wa3[3] = {0.0, 0.75, 1.0 / 3.0}, wb3[3] = {1.0, 0.25, 2.0 / 3.0};
+	for (int g = nck - 1; g >= 0; g--) {
+		int s0 = g * ck, s1 = std::min(nsteps, s0 + ck);
+		HIPCHK(hipMemcpyAsync(c->b.U, A->ckpt[g], bytes, hipMemcpyDeviceToDevice, c->st));
+		do_prim(c);
+		for (int s = s0; s < s1; s++) step_record<MM>(c, &A->stages[(size_t)(s - s0) * S]);
+		for (int s = s1 - 1; s >= s0; s--) {
+			for (int st = S - 1; st >= 0; st--) {
+				HIPCHK(hipMemcpyAsync(c->b.U, A->stages[(size_t)(s - s0) * S + st], bytes, hipMemcpyDeviceToDevice,
+				                      c->st));
+				double wa = S == 3 ? wa3[st] : wa2[st], wb = S == 3 ? wb3[st] : wb2[st];
+				int mode = st == 0 ? 2 : (st == S - 1 ? 0 : 1);
+				launch_adj_stage<MM>(c, A.get(), wa, wb, mode);
+				std::swap(a.Wn, a.Ws);
+			}
+		}
+	}
+	if ((rc = check_launch("adjoint reverse sweep"))) return rc;
+	if (dJdU0) HIPCHK(hipMemcpyAsync(dJdU0, a.Wn, bytes, hipMemcpyDeviceToHost, c->st));
+	if (dJdbc && c->nbc > 0)
+		HIPCHK(hipMemcpyAsync(dJdbc, a.bcbar, 4 * (size_t)c->nbc * sizeof(double), hipMemcpyDeviceToHost, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	// leave the context at the final forward state
+	HIPCHK(hipMemcpy(c->b.U, U.data(), bytes, hipMemcpyHostToDevice));
+	do_prim(c);
+	HIPCHK(hipStreamSynchronize(c->st));
+	return check_launch("adjoint");
+}
+
+}  // namespace
+
+extern "C" int cfd2d_adjoint(cfd2d_ctx* c, int nsteps, int ckpt_every, const int* edge_mask, double* J,
+                             double* dJdU0, double* dJdbc)
+{
+	if (!c || nsteps < 1 || ckpt_every < 1 || !edge_mask) return fail(CFD2D_ERR_ARG, "cfd2d_adjoint: bad arguments");
+	if ((c->p.flux != CFD2D_FLUX_LAX && c->p.flux != CFD2D_FLUX_HLLC) || c->p.limiter != CFD2D_LIMITER_NONE ||
+	    c->p.viscous || c->p.steady)
+		return fail(CFD2D_ERR_CONFIG, "adjoint: needs FLUX LAX or HLLC, no limiter, inviscid, unsteady");
+	bool timing = c->timing;
+	c->timing = false;  // the adjoint's launches are not part of the step timing
+	int rc = c->mm ? adjoint_run<true>(c, nsteps, ckpt_every, edge_mask, J, dJdU0, dJdbc)
+	               : adjoint_run<false>(c, nsteps, ckpt_every, edge_mask, J, dJdU0, dJdbc);
+	c->timing = timing;
+	return rc;
+}

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -1167,3 +1168,5 @@ int cfd2d_ctx_stream(cfd2d_ctx* c, void** s)
 }
 
 }  // extern "C"
+
+#include "adjoint.inl"

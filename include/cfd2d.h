@@ -143,6 +143,18 @@ int  cfd2d_timing(cfd2d_ctx* c, int enable);
 int  cfd2d_timing_read(cfd2d_ctx* c, int kernel, double* total_ms, long long* launches);
 int  cfd2d_ctx_info(cfd2d_ctx* c, int* n_cells, int* n_edges, int* n_cases);
 int  cfd2d_ctx_stream(cfd2d_ctx* c, void** hip_stream);
+/* Discrete adjoint of nsteps steps (extension; no reference counterpart: the
+ * reference's steady implicit method FVM_TVD_IMPLICIT only evaluates this
+ * objective, calcLiftForce at fvm_tvd_implicit.cpp:1088-1110).  Objective
+ * J = sum over edges with edge_mask != 0 of p(c1) * l * n_y after the steps;
+ * on return *J, dJ_dU0 = dJ/d(state at the call) as double[4][nc], and
+ * dJ_dbc = dJ/d(parameters of each boundary object) as double[n_bc][4] (NULL
+ * skips either).  Forward states are kept every ckpt_every steps and each
+ * segment is recomputed once in the reverse sweep.  Supports FLUX LAX and
+ * HLLC, no limiter, inviscid, unsteady; the time step is held fixed.  The
+ * context is left at the final state, as after cfd2d_step(c, nsteps). */
+int  cfd2d_adjoint(cfd2d_ctx* c, int nsteps, int ckpt_every, const int* edge_mask, double* J, double* dJ_dU0,
+                   double* dJ_dbc);
 
 #ifdef __cplusplus
 }

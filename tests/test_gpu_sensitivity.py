@@ -1,0 +1,141 @@
+"""Discrete adjoint on the GPU (cfd2d_adjoint) against the tangent-linear
+model of the oracle (oracle/tangent.py) and against finite differences of the
+GPU's own forward map.
+
+Both sides differentiate the same discrete map exactly, so they agree to
+round-off: <dJ/dU0, v> + <dJ/dpar, w> within 1e-9 relative of the tangent,
+each boundary-parameter derivative within 1e-9 of the largest one, J within
+1e-12.  At 21,600 cells the check is against central differences (h = 1e-6,
+relative 1e-5), which holds where no max()/min() switch sits near a tie.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import tangent
+from cfd2d import case as cases
+from cfd2d import native
+
+pytestmark = pytest.mark.gpu
+
+
+def bump(flux, gradient, rk, ny=6):
+    return cases.bump_channel(ny, height=0.06, mach=0.8, seed=3, flux=flux, gradient=gradient, limiter="NONE",
+                              rk_stages=rk)
+
+
+def pulse(flux, gradient, rk, n=12):
+    return cases.periodic_pulse(n, seed=2, flux=flux, gradient=gradient, rk_stages=rk, viscous=False,
+                                cfl_mode="REFERENCE")
+
+
+def objective_mask(s, kind):
+    if kind == "bump":
+        return (s.geometry["edge_group"] == s.case.mesh.group_index("bump")).astype(np.int32)
+    c1 = s.geometry["edge_int"][:, 2]
+    return (c1 < s.nc // 4).astype(np.int32)  # a band of interior edges
+
+
+CONFIGS = [("bump", "LAX", "GREEN_GAUSS", 2), ("bump", "HLLC", "LSQ", 3), ("bump", "LAX", "LSQ", 3),
+           ("bump", "HLLC", "GREEN_GAUSS", 2), ("pulse", "HLLC", "LSQ", 2), ("pulse", "LAX", "GREEN_GAUSS", 3)]
+
+
+@pytest.mark.parametrize("kind,flux,gradient,rk", CONFIGS)
+def test_adjoint_matches_tangent(kind, flux, gradient, rk):
+    make = bump if kind == "bump" else pulse
+    c = make(flux, gradient, rk)
+    s = oracle.OracleSolver(c)
+    tau, _ = s.tau()
+    mask = objective_mask(s, kind)
+    n = 12
+    g = native.GpuSolver(c)
+    g.set_tau(tau)
+    J, gU, gb = g.adjoint(n, mask, ckpt_every=5)
+    g.close()
+    rng = np.random.default_rng(11)
+    U0 = c.initial_state()
+    v = rng.standard_normal(U0.shape) * np.abs(U0) * 1e-2
+    m = tangent.Model(s)
+    w = rng.standard_normal(m.bcpar0.shape) * np.abs(m.bcpar0) * 1e-2
+    Jm, dJ = tangent.tangent(s, U0, v, w, n, tau, mask)
+    assert abs(J - Jm) <= 1e-12 * abs(Jm)
+    ad = float(np.sum(gU * v)) + float(np.sum(gb * w))
+    assert abs(ad - dJ) <= 1e-9 * abs(dJ), (ad, dJ)
+    if kind == "bump":
+        # each inlet parameter (u, v, T, p) and the outlet pressure separately
+        zero = np.zeros_like(U0)
+        big = np.max(np.abs(gb))
+        for bc, k in [(0, 0), (0, 1), (0, 2), (0, 3), (1, 0)]:
+            e = np.zeros_like(m.bcpar0)
+            e[bc, k] = 1.0
+            _, d = tangent.tangent(s, U0, zero, e, n, tau, mask)
+            assert abs(gb[bc, k] - d) <= 1e-9 * big, (bc, k, gb[bc, k], d)
+
+
+def test_adjoint_checkpoint_interval_and_final_state():
+    """The reverse sweep recomputes each segment deterministically, so the
+    gradient is bitwise the same for any checkpoint interval, and the context
+    ends at the same state as plain steps."""
+    c = bump("HLLC", "LSQ", 3, ny=10)
+    n = 13
+    g = native.GpuSolver(c)
+    mask = np.zeros(g.ne, np.int32)
+    s = oracle.OracleSolver(c)
+    mask[:] = (s.geometry["edge_group"] == c.mesh.group_index("bump"))
+    res = []
+    for ck in (1, 4, 13, 50):
+        g.set_state(c.initial_state())
+        res.append(g.adjoint(n, mask, ckpt_every=ck))
+        res[-1] = res[-1] + (g.state(),)
+    ref = native.GpuSolver(c)
+    ref.run(n)
+    Uref = ref.state()
+    ref.close()
+    g.close()
+    for J, gU, gb, U in res[1:]:
+        assert J == res[0][0]
+        assert np.array_equal(gU, res[0][1])
+        assert np.array_equal(gb, res[0][2])
+        assert np.array_equal(U, res[0][3])
+    assert np.array_equal(res[0][3], Uref)
+
+
+def test_adjoint_finite_differences_21k_cells():
+    c = bump("HLLC", "LSQ", 3, ny=60)
+    s_or = oracle.OracleSolver(c)
+    mask = (s_or.geometry["edge_group"] == c.mesh.group_index("bump")).astype(np.int32)
+    del s_or
+    g = native.GpuSolver(c)
+    n = 30
+    U0 = c.initial_state()
+    J, gU, gb = g.adjoint(n, mask, ckpt_every=8)
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal(U0.shape) * np.abs(U0) * 1e-2
+
+    def Jof(h):
+        g.set_state(U0 + h * v)
+        g.step(n)
+        g.sync()
+        return g.wall_force(mask)[1]
+
+    h = 1e-6
+    fd = (Jof(h) - Jof(-h)) / (2 * h)
+    assert abs(Jof(0.0) - J) <= 1e-13 * abs(J)
+    ad = float(np.sum(gU * v))
+    g.close()
+    assert abs(ad - fd) <= 1e-5 * abs(fd), (ad, fd)
+
+
+@pytest.mark.parametrize("over", [dict(flux="GODUNOV"), dict(limiter="MINMOD")])
+def test_adjoint_rejects_unsupported(over):
+    cfg = dict(flux="LAX", gradient="GREEN_GAUSS", limiter="NONE", rk_stages=2)
+    cfg.update(over)
+    c = cases.bump_channel(6, height=0.06, mach=0.8, seed=3, **cfg)
+    g = native.GpuSolver(c)
+    with pytest.raises(native.Cfd2dError) as ei:
+        g.adjoint(2, np.zeros(g.ne, np.int32))
+    assert ei.value.code == 5
+    with pytest.raises(native.Cfd2dError) as ei:
+        g.adjoint(0, np.zeros(g.ne, np.int32))
+    assert ei.value.code == 1
+    g.close()
