@@ -128,6 +128,42 @@ def batched_vs_serial(n_cases, steps, seed=7):
                 unit="cell-updates/s")
 
 
+def adjoint_vs_forward(steps, ckpt, ny=289):
+    """Adjoint workload: bump channel with ny=289 (501,126 triangles), Mach
+    0.8, bump height 0.05; LAX flux, Green-Gauss gradients, Heun RK2, no
+    limiter; J = y pressure force on the bump.  Wall time of cfd2d_adjoint
+    (forward pass, checkpoints, recomputation, reverse sweep, gradient copied
+    to the host) over the wall time of the same number of plain steps."""
+    from cfd2d import case as cases
+    from cfd2d import native
+    c = cases.bump_channel(ny, height=0.05, mach=0.8, seed=3, flux="LAX", gradient="GREEN_GAUSS", limiter="NONE",
+                           rk_stages=2)
+    g = native.GpuSolver(c)
+    mask = g.edge_mask("bump")
+    U0 = c.initial_state()
+    g.run(5)
+    g.adjoint(4, mask, ckpt_every=2)  # first-call set-up (topology) outside the timing
+    g.set_state(U0)
+    g.sync()
+    t0 = time.perf_counter()
+    g.run(steps)
+    t_fwd = time.perf_counter() - t0
+    out = dict(cells=g.nc, edges=g.ne, steps=steps, workload="bump channel ny=%d, LAX/GG/RK2, Mach 0.8" % ny,
+               forward_s=round(t_fwd, 5))
+    for name, ck in (("ckpt%d" % ckpt, ckpt), ("auto", 0)):
+        g.set_state(U0)
+        g.sync()
+        t0 = time.perf_counter()
+        J, gU, gb = g.adjoint(steps, mask, ckpt_every=ck)
+        t = time.perf_counter() - t0
+        out[name] = dict(seconds=round(t, 5), ratio=round(t / t_fwd, 3))
+        out["J"] = J
+    g.close()
+    out["ratio"] = out["ckpt%d" % ckpt]["ratio"]
+    out["ratio_auto"] = out["auto"]["ratio"]
+    return out
+
+
 def load_traffic(workload):
     """HBM bytes per launch of each kernel from the committed rocprofv3 PMC summary."""
     path = os.path.join(REPO, "profiles", "traffic.json")
@@ -151,6 +187,9 @@ def main():
     ap.add_argument("--batch-cases", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--adj-steps", type=int, default=200)
+    ap.add_argument("--adj-ckpt", type=int, default=20)
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -211,6 +250,9 @@ def main():
     if not args.no_batch:
         out["batched"] = batched_vs_serial(args.batch_cases, args.batch_steps)
         log("batched: %s" % out["batched"])
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_vs_forward(args.adj_steps, args.adj_ckpt)
+        log("adjoint: %s" % out["adjoint"])
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case_fn, args.cpu_steps)
         ref = reference_cpu()

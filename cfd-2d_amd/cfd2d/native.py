@@ -296,7 +296,9 @@ class Context:
     def adjoint(self, nsteps, edge_mask, ckpt_every=20):
         """Runs nsteps steps and the reverse sweep for J = sum over masked
         edges of p(c1) * l * n_y; returns (J, dJ/dU0 as (4, nc), dJ/d(boundary
-        parameters) as (n_bc, 4)).  The context ends at the final state."""
+        parameters) as (n_bc, 4)).  The context ends at the final state.
+        ckpt_every <= 0 lets the library choose the interval (all stages
+        recorded when they fit in half of the free device memory)."""
         J = C.c_double()
         gU = np.zeros(4 * self.nc)
         gb = np.zeros(4 * max(self.n_bc, 1))
@@ -339,6 +341,14 @@ class GpuSolver:
 
     def __getattr__(self, name):
         return getattr(self.ctx, name)
+
+    def edge_mask(self, group):
+        """int32 mask of the edges in the named edge group (objective edges)."""
+        names = self.mesh.group_names(0)
+        if group not in names:
+            raise KeyError("no edge group %r (have %s)" % (group, names))
+        eg = self.mesh.export()["edge_group"]
+        return (eg == names.index(group)).astype(np.int32)
 
     def run(self, n, graph=False):
         if graph:

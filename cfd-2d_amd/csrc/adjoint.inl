@@ -11,36 +11,41 @@
 //
 // Per RK stage with input state U_s, U_{s+1} = a U0 + b (U_s + tau/S R(U_s)):
 //   given W = dJ/dU_{s+1}:  acc0 += a W,  dJ/dU_s = b W + (dR/dU_s)^T (b tau/S W)
-// The vector-Jacobian product of R runs as: primitives and gradients of U_s
-// (forward kernels), k_adj_edge (per edge: reverse of reconstruction, flux
-// and ghost into a per-edge buffer, both sides), k_adj_cell (each cell
-// gathers its faces' entries in ascending edge order), k_adj_grad (reverse of
-// the gradient operator: own and neighbours' gradient adjoints, ghost terms,
-// then the primitive->conserved Jacobian and the RK bookkeeping).  Boundary
-// parameter adjoints are summed per boundary object by one workgroup in edge
-// order.  No floating-point atomics: the result does not depend on
-// scheduling.
+// The vector-Jacobian product of R is three passes over the primitives P and
+// gradients G the forward pass recorded for that stage: k_adj_edge (two
+// lanes per edge, one Gauss point each: reverse of the flux, of the ghost on
+// boundaries and of the reconstruction, into a per-edge buffer for both
+// sides), k_adj_cell (each cell gathers its faces' entries in ascending edge
+// order into (P, G) adjoints), k_adj_grad (reverse of the gradient operator:
+// own and neighbours' gradient adjoints, ghost terms, then the
+// primitive->conserved Jacobian and the RK bookkeeping).  Boundary-parameter
+// adjoints accumulate per boundary edge and are summed per boundary object
+// once, in edge order.  No floating-point atomics: the result does not depend
+// on scheduling.
 //
-// Checkpoints: the forward pass keeps the state every ckpt steps; the reverse
-// sweep recomputes one segment at a time, recording the input state of every
-// RK stage, and walks it backwards.
+// Storage: the forward kernels write each stage's P and G into per-stage
+// slots (the update writes the next stage's P), so recording costs no extra
+// pass.  The state is checkpointed every ckpt steps; the last segment is
+// recorded during the forward pass itself and every earlier segment is
+// recomputed once from its checkpoint.  With ckpt >= nsteps nothing is
+// recomputed.
 
 namespace {
 
-constexpr int EA = 24;  // per edge: c1 side (P 4, G 8), c2 side (P 4, G 8)
+constexpr int EA = 16;  // per edge [side][Gauss point][r, p, u, v]; on a boundary the
+                        // c2 half holds the adjoint of the ghost's cell-centre input
 
 struct AdjBufs {
-	double* Wn;    // dJ/dU_{s+1}
-	double* Ws;    // dJ/dU_s
-	double* Acc;   // sum of a * W over the stages of the current step
-	double* Pb;    // [4][nc]
-	double* Gb;    // [8][nc]
-	double* Ea;    // [24][ne]
-	double* Ebc;   // [4][ne] boundary-parameter adjoints from the edge pass
-	double* Ebc2;  // [4][ne] ... from the gradient pass (ghosts of cell centres)
-	double* bcbar; // [nbc][4]
-	int* cslot;    // [3][nc] slot of this cell in its neighbour's face list
-	int* bedges;   // boundary edges in ascending order
+	double* Wn;     // dJ/dU_{s+1}
+	double* Ws;     // dJ/dU_s
+	double* Acc;    // sum of a * W over the stages of the current step
+	double* Pb;     // [4][nc]
+	double* Gb;     // [8][nc]
+	double* Ea;     // [16][ne]
+	double* Ebc;    // [4][ne] boundary-parameter adjoints, summed over stages per edge
+	double* bcbar;  // [nbc][4]
+	const int* cslot;   // [3][nc] slot of this cell in its neighbour's face list
+	const int* bedges;  // boundary edges in ascending order
 	int nbedges;
 };
 
@@ -216,84 +221,83 @@ __device__ __forceinline__ void hllc_vjp(const Prim& L, const Prim& R, double nx
 	}
 }
 
-// Reverse of the edge pass for one stage: seed F_e-bar = (Rb_c2 - Rb_c1) l/2
-// with Rb = scale * tau/S * W; per Gauss point the flux VJP, the ghost VJP on
-// boundaries, and the reverse of the reconstruction into (P, G) adjoints of
-// both sides, stored per edge.
+// Reverse of the edge pass for one stage.  Seed F_e-bar = (Rb_c2 - Rb_c1) l/2
+// with Rb = scale * tau/S * W; each lane takes one Gauss point: the flux VJP,
+// the ghost VJP on a boundary, then its (r, p, u, v) adjoints on both sides.
+// The boundary-parameter adjoints of the pair are added into Ebc.
 template <bool MM, int FLUX>
 __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, double scale)
 {
-	int e = xcd_block() * BLOCK + threadIdx.x;
-	if (e >= b.ne) return;
+	int t = xcd_block() * BLOCK + threadIdx.x;
+	int e = t >> 1, gp = t & 1;
+	if (e >= b.ne) return;  // both lanes of a pair leave together
 	size_t ne = b.ne, n = b.nc;
 	int c1 = b.ec1[e], c2 = b.ec2[e];
 	double nx = b.enx[e], ny = b.eny[e];
 	double hl = b.el[e] * 0.5;
 	double fb[4];
+	double s1 = scale * b.dtS[c1], s2 = c2 >= 0 ? scale * b.dtS[c2] : 0.0;
 	for (int q = 0; q < 4; q++) {
-		double r1 = scale * b.dtS[c1] * a.Wn[q * n + c1];
-		double r2 = c2 >= 0 ? scale * b.dtS[c2] * a.Wn[q * n + c2] : 0.0;
+		double r1 = s1 * a.Wn[q * n + c1];
+		double r2 = c2 >= 0 ? s2 * a.Wn[q * n + c2] : 0.0;
 		fb[q] = (r2 - r1) * hl;
 	}
 	Prim p1 = load_prim(b, c1);
-	Prim p2 = c2 >= 0 ? load_prim(b, c2) : p1;
-	double out[EA];
-	for (int k = 0; k < EA; k++) out[k] = 0.0;
-	double parb[4] = {0.0, 0.0, 0.0, 0.0};
-	for (int gp = 0; gp < 2; gp++) {
-		double d1x = b.edl[(size_t)(2 * gp) * ne + e], d1y = b.edl[(size_t)(2 * gp + 1) * ne + e];
-		Prim L = p1, R;
-		recon(L, b, c1, d1x, d1y);
-		double d2x = 0.0, d2y = 0.0;
-		if (c2 >= 0) {
-			d2x = b.edl[(size_t)(4 + 2 * gp) * ne + e];
-			d2y = b.edl[(size_t)(5 + 2 * gp) * ne + e];
-			R = p2;
-			recon(R, b, c2, d2x, d2y);
-		} else {
-			R = ghost_of<MM>(b, ph, e, c1, L, p1);
-		}
-		double Lb[4] = {0, 0, 0, 0}, Rb[4] = {0, 0, 0, 0};
-		if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb);
-		else lax_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb);
-		if (c2 < 0) {
-			int bc = -c2 - 1;
-			double Cb[4] = {0, 0, 0, 0};
-			ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], nx, ny, L, p1, mat_of<MM>(b, ph, c1), Rb, Lb, Cb, parb);
-			for (int q = 0; q < 4; q++) out[q] += Cb[q];
-		} else {
-			for (int q = 0; q < 4; q++) {
-				out[12 + q] += Rb[q];
-				out[16 + 2 * q] += Rb[q] * d2x;
-				out[17 + 2 * q] += Rb[q] * d2y;
-			}
-		}
-		for (int q = 0; q < 4; q++) {
-			out[q] += Lb[q];
-			out[4 + 2 * q] += Lb[q] * d1x;
-			out[5 + 2 * q] += Lb[q] * d1y;
-		}
+	Prim L = p1, R;
+	recon(L, b, c1, b.edl[(size_t)(2 * gp) * ne + e], b.edl[(size_t)(2 * gp + 1) * ne + e]);
+	if (c2 >= 0) {
+		R = load_prim(b, c2);
+		recon(R, b, c2, b.edl[(size_t)(4 + 2 * gp) * ne + e], b.edl[(size_t)(5 + 2 * gp) * ne + e]);
+	} else {
+		R = ghost_of<MM>(b, ph, e, c1, L, p1);
 	}
-	for (int k = 0; k < EA; k++) a.Ea[(size_t)k * ne + e] = out[k];
-	if (c2 < 0)
-		for (int q = 0; q < 4; q++) a.Ebc[(size_t)q * ne + e] = parb[q];
+	double Lb[4] = {0, 0, 0, 0}, Rb[4] = {0, 0, 0, 0}, parb[4] = {0, 0, 0, 0};
+	if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb);
+	else lax_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb);
+	if (c2 < 0) {
+		int bc = -c2 - 1;
+		double Cb[4] = {0, 0, 0, 0};
+		ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], nx, ny, L, p1, mat_of<MM>(b, ph, c1), Rb, Lb, Cb, parb);
+		for (int q = 0; q < 4; q++) Rb[q] = Cb[q];
+	}
+	for (int q = 0; q < 4; q++) {
+		a.Ea[(size_t)(4 * gp + q) * ne + e] = Lb[q];
+		a.Ea[(size_t)(8 + 4 * gp + q) * ne + e] = Rb[q];
+	}
+	double o[4];
+	for (int q = 0; q < 4; q++) o[q] = __shfl_xor(parb[q], 1);
+	if (c2 < 0 && gp == 0)
+		for (int q = 0; q < 4; q++) a.Ebc[(size_t)q * ne + e] += parb[q] + o[q];
 }
 
-// each cell gathers the (P, G) adjoints of its side of its three edges
+// each cell gathers its side of its three edges: P adjoint = sum of the
+// Gauss-point adjoints, G adjoint = sum of adjoint x Gauss-point offset
 __global__ __launch_bounds__(BLOCK) void k_adj_cell(Bufs b, AdjBufs a)
 {
 	int i = xcd_block() * BLOCK + threadIdx.x;
 	if (i >= b.nc) return;
 	size_t n = b.nc, ne = b.ne;
-	double acc[12];
-	for (int k = 0; k < 12; k++) acc[k] = 0.0;
+	double pb[4] = {0, 0, 0, 0};
+	double gb[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 	for (int k = 0; k < 3; k++) {
 		int f = b.cface[k * n + i];
-		int e = f >> 1, off = (f & 1) ? 12 : 0;
-		for (int q = 0; q < 12; q++) acc[q] += a.Ea[(size_t)(off + q) * ne + e];
+		int e = f >> 1, side = f & 1;
+		for (int gp = 0; gp < 2; gp++) {
+			double dx = b.edl[(size_t)(4 * side + 2 * gp) * ne + e];
+			double dy = b.edl[(size_t)(4 * side + 2 * gp + 1) * ne + e];
+			for (int q = 0; q < 4; q++) {
+				double v = a.Ea[(size_t)(8 * side + 4 * gp + q) * ne + e];
+				pb[q] += v;
+				gb[2 * q] += v * dx;
+				gb[2 * q + 1] += v * dy;
+			}
+		}
+		if (b.cnb[k * n + i] < 0)
+			for (int gp = 0; gp < 2; gp++)
+				for (int q = 0; q < 4; q++) pb[q] += a.Ea[(size_t)(8 + 4 * gp + q) * ne + e];
 	}
-	for (int q = 0; q < 4; q++) a.Pb[q * n + i] = acc[q];
-	for (int q = 0; q < 8; q++) a.Gb[q * n + i] = acc[4 + q];
+	for (int q = 0; q < 4; q++) a.Pb[q * n + i] = pb[q];
+	for (int q = 0; q < 8; q++) a.Gb[q * n + i] = gb[q];
 }
 
 // Reverse of the gradient operator and of the primitive conversion, then the
@@ -317,22 +321,20 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 		int e = f >> 1, side = f & 1;
 		int j = b.cnb[k * n + i];
 		double nbb[4];  // adjoint of the neighbour value this face contributed
-		double self[4];
 		if (LSQ) {
 			double wx = b.lsqw[(size_t)(2 * k) * n + i], wy = b.lsqw[(size_t)(2 * k + 1) * n + i];
 			for (int q = 0; q < 4; q++) {
 				nbb[q] = wx * gb[2 * q] + wy * gb[2 * q + 1];
-				self[q] = -nbb[q];
+				pb[q] -= nbb[q];
 			}
 		} else {
-			double nx = b.enx[e], ny = b.eny[e], l = b.el[e];
+			double l = b.el[e], nx = b.enx[e], ny = b.eny[e];
 			double sg = side ? -1.0 : 1.0;
 			for (int q = 0; q < 4; q++) {
 				nbb[q] = sg * 0.5 * l * (nx * gb[2 * q] + ny * gb[2 * q + 1]) / si;
-				self[q] = nbb[q];
+				pb[q] += nbb[q];
 			}
 		}
-		for (int q = 0; q < 4; q++) pb[q] += self[q];
 		if (j >= 0) {
 			// the neighbour's gradient depends on this cell's value
 			int s = a.cslot[k * n + i];
@@ -342,7 +344,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 				double wx = b.lsqw[(size_t)(2 * s) * n + j], wy = b.lsqw[(size_t)(2 * s + 1) * n + j];
 				for (int q = 0; q < 4; q++) pb[q] += wx * gj[2 * q] + wy * gj[2 * q + 1];
 			} else {
-				double nx = b.enx[e], ny = b.eny[e], l = b.el[e];
+				double l = b.el[e], nx = b.enx[e], ny = b.eny[e];
 				double sg = side ? 1.0 : -1.0;  // the neighbour sits on the other side
 				double sj = b.S[j];
 				for (int q = 0; q < 4; q++) pb[q] += sg * 0.5 * l * (nx * gj[2 * q] + ny * gj[2 * q + 1]) / sj;
@@ -354,7 +356,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 			ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], b.enx[e], b.eny[e], pc, pc, m, nbb, Lb, Cb, parb);
 			for (int q = 0; q < 4; q++) {
 				pb[q] += Lb[q] + Cb[q];
-				a.Ebc2[(size_t)q * ne + e] = parb[q];
+				a.Ebc[(size_t)q * ne + e] += parb[q];
 			}
 		}
 	}
@@ -376,7 +378,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 }
 
 // boundary-parameter adjoints: one workgroup sums the per-edge entries of
-// each boundary object in ascending edge order (deterministic)
+// each boundary object in ascending edge order
 __global__ __launch_bounds__(BLOCK) void k_bc_reduce(Bufs b, AdjBufs a, int nbc)
 {
 	__shared__ double part[BLOCK][4];
@@ -386,7 +388,7 @@ __global__ __launch_bounds__(BLOCK) void k_bc_reduce(Bufs b, AdjBufs a, int nbc)
 		for (int k = threadIdx.x; k < a.nbedges; k += BLOCK) {
 			int e = a.bedges[k];
 			if (-b.ec2[e] - 1 != bc) continue;
-			for (int q = 0; q < 4; q++) s[q] += a.Ebc[(size_t)q * ne + e] + a.Ebc2[(size_t)q * ne + e];
+			for (int q = 0; q < 4; q++) s[q] += a.Ebc[(size_t)q * ne + e];
 		}
 		for (int q = 0; q < 4; q++) part[threadIdx.x][q] = s[q];
 		__syncthreads();
@@ -394,10 +396,36 @@ __global__ __launch_bounds__(BLOCK) void k_bc_reduce(Bufs b, AdjBufs a, int nbc)
 			double t[4] = {0, 0, 0, 0};
 			for (int k = 0; k < BLOCK; k++)
 				for (int q = 0; q < 4; q++) t[q] += part[k][q];
-			for (int q = 0; q < 4; q++) a.bcbar[4 * bc + q] += t[q];
+			for (int q = 0; q < 4; q++) a.bcbar[4 * bc + q] = t[q];
 		}
 		__syncthreads();
 	}
+}
+
+__global__ __launch_bounds__(BLOCK) void k_fill(double* x, size_t count, double v)
+{
+	size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+	if (i < count) x[i] = v;
+}
+
+// objective seed at the final state: W(i) = (sum of l n_y over its masked
+// edges) * dp/dU(i), and p(i) for the objective's value
+template <bool MM>
+__global__ __launch_bounds__(BLOCK) void k_seed(Bufs b, Phys ph, AdjBufs a, const int* cells, const double* wsum,
+                                                int K, double* pout)
+{
+	int k = blockIdx.x * BLOCK + threadIdx.x;
+	if (k >= K) return;
+	size_t n = b.nc;
+	int i = cells[k];
+	Prim p = load_prim(b, i);
+	MatV m = mat_of<MM>(b, ph, i);
+	double g1 = m.gam - 1.0, w = wsum[k];
+	a.Wn[i] = w * g1 * 0.5 * (p.u * p.u + p.v * p.v);
+	a.Wn[n + i] = -w * g1 * p.u;
+	a.Wn[2 * n + i] = -w * g1 * p.v;
+	a.Wn[3 * n + i] = w * g1;
+	pout[k] = p.p;
 }
 
 }  // namespace
@@ -405,29 +433,25 @@ __global__ __launch_bounds__(BLOCK) void k_bc_reduce(Bufs b, AdjBufs a, int nbc)
 struct cfd2d_adj {
 	AdjBufs a{};
 	std::vector<void*> allocs;
-	std::vector<double*> ckpt;   // states every ckpt steps
-	std::vector<double*> stages; // stage input states of one segment
+	std::vector<double*> ckpt;  // state at the start of every segment but the last
+	double* Pst = nullptr;      // [nrec + 1][4][nc] stage primitives
+	double* Gst = nullptr;      // [nrec][8][nc] stage gradients
+	double* Ufin = nullptr;     // final state
+	int* seed_cells = nullptr;
+	double* seed_w = nullptr;
+	double* seed_p = nullptr;
 };
 
 namespace {
 
-int adj_alloc(cfd2d_adj* A, double** p, size_t count)
+template <typename T>
+int adj_alloc(cfd2d_adj* A, T** p, size_t count)
 {
 	void* q = nullptr;
-	hipError_t e = hipMalloc(&q, (count ? count : 1) * sizeof(double));
+	hipError_t e = hipMalloc(&q, (count ? count : 1) * sizeof(T));
 	if (e != hipSuccess) return hip_fail(e, "hipMalloc (adjoint)");
 	A->allocs.push_back(q);
-	*p = (double*)q;
-	return 0;
-}
-
-int adj_alloc_int(cfd2d_adj* A, int** p, size_t count)
-{
-	void* q = nullptr;
-	hipError_t e = hipMalloc(&q, (count ? count : 1) * sizeof(int));
-	if (e != hipSuccess) return hip_fail(e, "hipMalloc (adjoint)");
-	A->allocs.push_back(q);
-	*p = (int*)q;
+	*p = (T*)q;
 	return 0;
 }
 
@@ -438,75 +462,14 @@ void adj_free(cfd2d_adj* A)
 	delete A;
 }
 
-template <bool MM>
-void launch_adj_stage(cfd2d_ctx* c, cfd2d_adj* A, double wa, double wb, int mode)
-{
-	// primitives and gradients of the stage input (state in c->b.U)
-	launch_prim<MM>(c);
-	launch_gradient<MM>(c);
-	if (c->p.flux == CFD2D_FLUX_HLLC)
-		hipLaunchKernelGGL((k_adj_edge<MM, CFD2D_FLUX_HLLC>), dim3(grid_of(c->ne)), dim3(BLOCK), 0, c->st, c->b, c->ph,
-		                   A->a, wb);
-	else
-		hipLaunchKernelGGL((k_adj_edge<MM, CFD2D_FLUX_LAX>), dim3(grid_of(c->ne)), dim3(BLOCK), 0, c->st, c->b, c->ph,
-		                   A->a, wb);
-	hipLaunchKernelGGL(k_adj_cell, dim3(grid_of(c->nc)), dim3(BLOCK), 0, c->st, c->b, A->a);
-	if (c->p.gradient == CFD2D_GRAD_LSQ)
-		hipLaunchKernelGGL((k_adj_grad<MM, true>), dim3(grid_of(c->nc)), dim3(BLOCK), 0, c->st, c->b, c->ph, A->a, wa,
-		                   wb, mode);
-	else
-		hipLaunchKernelGGL((k_adj_grad<MM, false>), dim3(grid_of(c->nc)), dim3(BLOCK), 0, c->st, c->b, c->ph, A->a, wa,
-		                   wb, mode);
-	if (A->a.nbedges > 0)
-		hipLaunchKernelGGL(k_bc_reduce, dim3(1), dim3(BLOCK), 0, c->st, c->b, A->a, c->nbc);
-}
-
-// one forward step that records the input state of every RK stage
-template <bool MM>
-void step_record(cfd2d_ctx* c, double* const* rec)
-{
-	size_t bytes = 4 * (size_t)c->nc * sizeof(double);
-	int S = c->p.rk_stages;
-	const int modes2[2] = {UPD_FIRST, UPD_RK2_LAST}, modes3[3] = {UPD_FIRST, UPD_RK3_MID, UPD_RK3_LAST};
-	for (int s = 0; s < S; s++) {
-		(void)hipMemcpyAsync(rec[s], c->b.U, bytes, hipMemcpyDeviceToDevice, c->st);
-		launch_gradient<MM>(c);
-		launch_edge<MM>(c);
-		launch_update<MM>(c, S == 3 ? modes3[s] : modes2[s]);
-	}
-	hipLaunchKernelGGL((k_remediate<MM>), dim3(1), dim3(BLOCK), 0, c->st, c->b, c->ph);
-}
-
 using AdjGuard = std::unique_ptr<cfd2d_adj, void (*)(cfd2d_adj*)>;
 
-template <bool MM>
-int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, double* dJdU0, double* dJdbc)
+// the slot of each cell in its neighbours' face lists and the boundary edges
+// in ascending order, built once per context
+int adj_topology(cfd2d_ctx* c)
 {
-	int nc = c->nc, ne = c->ne, S = c->p.rk_stages;
-	size_t n4 = 4 * (size_t)nc, bytes = n4 * sizeof(double);
-	auto* A0 = new (std::nothrow) cfd2d_adj;
-	if (!A0) return fail(CFD2D_ERR_ARG, "out of host memory");
-	AdjGuard A(A0, adj_free);
-	int rc = 0;
-	int nck = (nsteps + ck - 1) / ck;
-	int seg = std::min(ck, nsteps);
-	A->ckpt.resize(nck);
-	A->stages.resize((size_t)seg * S);
-	for (int k = 0; k < nck && !rc; k++) rc = adj_alloc(A.get(), &A->ckpt[k], n4);
-	for (size_t k = 0; k < A->stages.size() && !rc; k++) rc = adj_alloc(A.get(), &A->stages[k], n4);
-	AdjBufs& a = A->a;
-	if (!rc) rc = adj_alloc(A.get(), &a.Wn, n4);
-	if (!rc) rc = adj_alloc(A.get(), &a.Ws, n4);
-	if (!rc) rc = adj_alloc(A.get(), &a.Acc, n4);
-	if (!rc) rc = adj_alloc(A.get(), &a.Pb, n4);
-	if (!rc) rc = adj_alloc(A.get(), &a.Gb, 2 * n4);
-	if (!rc) rc = adj_alloc(A.get(), &a.Ea, (size_t)EA * ne);
-	if (!rc) rc = adj_alloc(A.get(), &a.Ebc, 4 * (size_t)ne);
-	if (!rc) rc = adj_alloc(A.get(), &a.Ebc2, 4 * (size_t)ne);
-	if (!rc) rc = adj_alloc(A.get(), &a.bcbar, 4 * (size_t)(c->nbc > 0 ? c->nbc : 1));
-	if (rc) return rc;
-	// host-side topology: the slot of each cell in its neighbours' face
-	// lists, and the boundary edges in ascending order
+	if (c->adj_nbedges >= 0) return 0;
+	int nc = c->nc, ne = c->ne;
 	std::vector<int> cface(3 * (size_t)nc), cnb(3 * (size_t)nc), ec2(ne);
 	HIPCHK(hipMemcpy(cface.data(), c->b.cface, cface.size() * sizeof(int), hipMemcpyDeviceToHost));
 	HIPCHK(hipMemcpy(cnb.data(), c->b.cnb, cnb.size() * sizeof(int), hipMemcpyDeviceToHost));
@@ -525,83 +488,206 @@ int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, do
 		}
 	for (int e = 0; e < ne; e++)
 		if (ec2[e] < 0) bed.push_back(e);
-	if ((rc = adj_alloc_int(A.get(), &a.cslot, cslot.size()))) return rc;
-	if ((rc = adj_alloc_int(A.get(), &a.bedges, bed.size()))) return rc;
-	a.nbedges = (int)bed.size();
-	HIPCHK(hipMemcpy(a.cslot, cslot.data(), cslot.size() * sizeof(int), hipMemcpyHostToDevice));
-	if (!bed.empty()) HIPCHK(hipMemcpy(a.bedges, bed.data(), bed.size() * sizeof(int), hipMemcpyHostToDevice));
-	std::vector<double> zb(4 * (size_t)(c->nbc > 0 ? c->nbc : 1), 0.0);
-	HIPCHK(hipMemcpy(a.bcbar, zb.data(), zb.size() * sizeof(double), hipMemcpyHostToDevice));
-	std::vector<double> ze(4 * (size_t)ne, 0.0);
-	HIPCHK(hipMemcpy(a.Ebc, ze.data(), ze.size() * sizeof(double), hipMemcpyHostToDevice));
-	HIPCHK(hipMemcpy(a.Ebc2, ze.data(), ze.size() * sizeof(double), hipMemcpyHostToDevice));
+	int rc;
+	if ((rc = upload(c, &c->adj_cslot, cslot)) || (rc = upload(c, &c->adj_bedges, bed))) return rc;
+	c->adj_nbedges = (int)bed.size();
+	return 0;
+}
 
-	// forward pass, keeping the state every ck steps
+void launch_fill(cfd2d_ctx* c, double* x, size_t count)
+{
+	hipLaunchKernelGGL(k_fill, dim3((unsigned)((count + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, c->st, x, count, 0.0);
+}
+
+// nsteps forward steps writing every stage's P and G into the stage slots
+// (P slot 0 = primitives of the current state); U advances in place
+template <bool MM>
+void record_steps(cfd2d_ctx* c, cfd2d_adj* A, int nsteps)
+{
+	size_t n4 = 4 * (size_t)c->nc, n8 = 8 * (size_t)c->nc;
+	double *P0 = c->b.P, *Po0 = c->b.Po, *G0 = c->b.G;
+	int S = c->p.rk_stages;
+	const int modes2[2] = {UPD_FIRST, UPD_RK2_LAST}, modes3[3] = {UPD_FIRST, UPD_RK3_MID, UPD_RK3_LAST};
+	c->b.P = A->Pst;
+	c->b.Po = A->Pst;
+	launch_prim<MM>(c);
+	size_t t = 0;
 	for (int s = 0; s < nsteps; s++) {
+		for (int st = 0; st < S; st++, t++) {
+			c->b.P = A->Pst + t * n4;
+			c->b.Po = A->Pst + (t + 1) * n4;
+			c->b.G = A->Gst + t * n8;
+			launch_gradient<MM>(c);
+			launch_edge<MM>(c);
+			launch_update<MM>(c, S == 3 ? modes3[st] : modes2[st]);
+		}
+		c->b.P = A->Pst + t * n4;
+		c->b.Po = c->b.P;
+		hipLaunchKernelGGL((k_remediate<MM>), dim3(1), dim3(BLOCK), 0, c->st, c->b, c->ph);
+	}
+	c->b.P = P0;
+	c->b.Po = Po0;
+	c->b.G = G0;
+}
+
+template <bool MM>
+void launch_adj_stage(cfd2d_ctx* c, cfd2d_adj* A, double wa, double wb, int mode)
+{
+	if (c->p.flux == CFD2D_FLUX_HLLC)
+		hipLaunchKernelGGL((k_adj_edge<MM, CFD2D_FLUX_HLLC>), dim3(grid_of(2 * c->ne)), dim3(BLOCK), 0, c->st, c->b,
+		                   c->ph, A->a, wb);
+	else
+		hipLaunchKernelGGL((k_adj_edge<MM, CFD2D_FLUX_LAX>), dim3(grid_of(2 * c->ne)), dim3(BLOCK), 0, c->st, c->b,
+		                   c->ph, A->a, wb);
+	hipLaunchKernelGGL(k_adj_cell, dim3(grid_of(c->nc)), dim3(BLOCK), 0, c->st, c->b, A->a);
+	if (c->p.gradient == CFD2D_GRAD_LSQ)
+		hipLaunchKernelGGL((k_adj_grad<MM, true>), dim3(grid_of(c->nc)), dim3(BLOCK), 0, c->st, c->b, c->ph, A->a, wa,
+		                   wb, mode);
+	else
+		hipLaunchKernelGGL((k_adj_grad<MM, false>), dim3(grid_of(c->nc)), dim3(BLOCK), 0, c->st, c->b, c->ph, A->a, wa,
+		                   wb, mode);
+}
+
+// steps per segment when the caller leaves it to the library: the largest
+// interval whose stage records fit in half of the free device memory
+int auto_interval(cfd2d_ctx* c, int nsteps)
+{
+	size_t fr = 0, tot = 0;
+	if (hipMemGetInfo(&fr, &tot) != hipSuccess) return std::min(nsteps, 20);
+	double per_step = (double)c->p.rk_stages * 12.0 * 8.0 * c->nc;
+	double k = (0.5 * (double)fr - 64.0 * 8.0 * c->nc - 24.0 * 8.0 * c->ne) / per_step;
+	if (k < 1.0) return 1;
+	return k >= nsteps ? nsteps : (int)k;
+}
+
+template <bool MM>
+int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, double* dJdU0, double* dJdbc)
+{
+	int nc = c->nc, ne = c->ne, S = c->p.rk_stages;
+	size_t n4 = 4 * (size_t)nc, n8 = 8 * (size_t)nc, bytes = n4 * sizeof(double);
+	int rc = adj_topology(c);
+	if (rc) return rc;
+	if (ck <= 0) ck = auto_interval(c, nsteps);
+	ck = std::min(ck, nsteps);
+	auto* A0 = new (std::nothrow) cfd2d_adj;
+	if (!A0) return fail(CFD2D_ERR_ARG, "out of host memory");
+	AdjGuard A(A0, adj_free);
+	int nck = (nsteps + ck - 1) / ck;
+	size_t nrec = (size_t)ck * S;
+	A->ckpt.resize(nck - 1);
+	for (int k = 0; k < nck - 1 && !rc; k++) rc = adj_alloc(A.get(), &A->ckpt[k], n4);
+	AdjBufs& a = A->a;
+	if (!rc) rc = adj_alloc(A.get(), &A->Pst, (nrec + 1) * n4);
+	if (!rc) rc = adj_alloc(A.get(), &A->Gst, nrec * n8);
+	if (!rc) rc = adj_alloc(A.get(), &A->Ufin, n4);
+	if (!rc) rc = adj_alloc(A.get(), &a.Wn, n4);
+	if (!rc) rc = adj_alloc(A.get(), &a.Ws, n4);
+	if (!rc) rc = adj_alloc(A.get(), &a.Acc, n4);
+	if (!rc) rc = adj_alloc(A.get(), &a.Pb, n4);
+	if (!rc) rc = adj_alloc(A.get(), &a.Gb, n8);
+	if (!rc) rc = adj_alloc(A.get(), &a.Ea, (size_t)EA * ne);
+	if (!rc) rc = adj_alloc(A.get(), &a.Ebc, 4 * (size_t)ne);
+	if (!rc) rc = adj_alloc(A.get(), &a.bcbar, 4 * (size_t)(c->nbc > 0 ? c->nbc : 1));
+	if (rc) return rc;
+	a.cslot = c->adj_cslot;
+	a.bedges = c->adj_bedges;
+	a.nbedges = c->adj_nbedges;
+	// objective edges grouped by c1, ascending edge order within a cell
+	std::vector<std::pair<int, int>> me;
+	for (int e = 0; e < ne; e++)
+		if (mask[e]) me.emplace_back(c->hc1[e], e);
+	std::stable_sort(me.begin(), me.end(), [](const std::pair<int, int>& x, const std::pair<int, int>& y) {
+		return x.first < y.first;
+	});
+	std::vector<int> cells;
+	std::vector<double> wsum;
+	for (size_t k = 0; k < me.size(); k++) {
+		if (cells.empty() || cells.back() != me[k].first) {
+			cells.push_back(me[k].first);
+			wsum.push_back(0.0);
+		}
+		wsum.back() += c->hel[me[k].second] * c->hny[me[k].second];
+	}
+	int K = (int)cells.size();
+	if ((rc = adj_alloc(A.get(), &A->seed_cells, cells.size())) || (rc = adj_alloc(A.get(), &A->seed_w, wsum.size())) ||
+	    (rc = adj_alloc(A.get(), &A->seed_p, cells.size())))
+		return rc;
+	if (K > 0) {
+		HIPCHK(hipMemcpyAsync(A->seed_cells, cells.data(), K * sizeof(int), hipMemcpyHostToDevice, c->st));
+		HIPCHK(hipMemcpyAsync(A->seed_w, wsum.data(), K * sizeof(double), hipMemcpyHostToDevice, c->st));
+	}
+	launch_fill(c, a.Ebc, 4 * (size_t)ne);
+	HIPCHK(hipMemsetAsync(c->b.anyflag + 1, 0, sizeof(int), c->st));
+
+	// forward pass: checkpoints, then the last segment recorded in place
+	int last0 = (nck - 1) * ck;
+	for (int s = 0; s < last0; s++) {
 		if (s % ck == 0) HIPCHK(hipMemcpyAsync(A->ckpt[s / ck], c->b.U, bytes, hipMemcpyDeviceToDevice, c->st));
 		step_once(c);
 	}
+	record_steps<MM>(c, A.get(), nsteps - last0);
+	HIPCHK(hipMemcpyAsync(A->Ufin, c->b.U, bytes, hipMemcpyDeviceToDevice, c->st));
+	// seed dJ/dU at the final state (its primitives are the last P slot)
+	launch_fill(c, a.Wn, n4);
+	if (K > 0) {
+		Bufs bf = c->b;
+		bf.P = A->Pst + (size_t)(nsteps - last0) * S * n4;
+		hipLaunchKernelGGL((k_seed<MM>), dim3(grid_of(K)), dim3(BLOCK), 0, c->st, bf, c->ph, a, A->seed_cells,
+		                   A->seed_w, K, A->seed_p);
+	}
+	std::vector<double> pk(K > 0 ? K : 1);
+	int lim = 0;
+	if (K > 0) HIPCHK(hipMemcpyAsync(pk.data(), A->seed_p, K * sizeof(double), hipMemcpyDeviceToHost, c->st));
+	HIPCHK(hipMemcpyAsync(&lim, c->b.anyflag + 1, sizeof(int), hipMemcpyDeviceToHost, c->st));
 	HIPCHK(hipStreamSynchronize(c->st));
 	if ((rc = check_launch("adjoint forward pass"))) return rc;
-	// objective and its gradient at the final state (p of c1 = (gamma-1)(E - |m|^2/(2 r)))
-	std::vector<double> U(n4), P(n4), W(n4, 0.0), el(ne), eny(ne), mt(5 * (size_t)c->p.nmat);
-	std::vector<int> c1(ne), cm(nc);
-	std::vector<unsigned> fl(nc);
-	HIPCHK(hipMemcpy(U.data(), c->b.U, bytes, hipMemcpyDeviceToHost));
-	HIPCHK(hipMemcpy(P.data(), c->b.P, bytes, hipMemcpyDeviceToHost));
-	HIPCHK(hipMemcpy(el.data(), c->b.el, ne * sizeof(double), hipMemcpyDeviceToHost));
-	HIPCHK(hipMemcpy(eny.data(), c->b.eny, ne * sizeof(double), hipMemcpyDeviceToHost));
-	HIPCHK(hipMemcpy(c1.data(), c->b.ec1, ne * sizeof(int), hipMemcpyDeviceToHost));
-	HIPCHK(hipMemcpy(fl.data(), c->b.flags, nc * sizeof(unsigned), hipMemcpyDeviceToHost));
-	HIPCHK(hipMemcpy(mt.data(), c->b.mtab, mt.size() * sizeof(double), hipMemcpyDeviceToHost));
-	HIPCHK(hipMemcpy(cm.data(), c->b.cmat, nc * sizeof(int), hipMemcpyDeviceToHost));
-	for (int i = 0; i < nc; i++)
-		if (fl[i] & 0x2u) return fail(CFD2D_ERR_CONFIG, "adjoint: a cell left the bounds (LIM flag)");
-	double Jv = 0.0;
-	for (int e = 0; e < ne; e++) {
-		if (!mask[e]) continue;
-		int i = c1[e];
-		Jv += P[(size_t)nc + i] * el[e] * eny[e];
-		double g1 = mt[5 * (size_t)cm[i] + 2] - 1.0;
-		double u = P[2 * (size_t)nc + i], v = P[3 * (size_t)nc + i];
-		double w = el[e] * eny[e];
-		W[i] += w * g1 * 0.5 * (u * u + v * v);
-		W[(size_t)nc + i] += -w * g1 * u;
-		W[2 * (size_t)nc + i] += -w * g1 * v;
-		W[3 * (size_t)nc + i] += w * g1;
+	if (lim) return fail(CFD2D_ERR_CONFIG, "adjoint: a cell left the bounds (LIM flag) during the steps");
+	if (J) {
+		// ascending edge order, as cfd2d_wall_force sums it
+		double Jv = 0.0;
+		for (int e = 0; e < ne; e++) {
+			if (!mask[e]) continue;
+			size_t k = std::lower_bound(cells.begin(), cells.end(), c->hc1[e]) - cells.begin();
+			Jv += pk[k] * c->hel[e] * c->hny[e];
+		}
+		*J = Jv;
 	}
-	if (J) *J = Jv;
-	HIPCHK(hipMemcpy(a.Wn, W.data(), bytes, hipMemcpyHostToDevice));
-	// reverse sweep, one segment at a time: recompute the segment from its
-	// checkpoint recording every stage input, then walk it backwards
+	// reverse sweep, segment by segment (the last one is already recorded)
 	const double wa2[2] = {0.0, 0.5}, wb2[2] = {1.0, 0.5};
 	const double wa3[3] = {0.0, 0.75, 1.0 / 3.0}, wb3[3] = {1.0, 0.25, 2.0 / 3.0};
+	double *P0 = c->b.P, *G0 = c->b.G;
 	for (int g = nck - 1; g >= 0; g--) {
-		int s0 = g * ck, s1 = std::min(nsteps, s0 + ck);
-		HIPCHK(hipMemcpyAsync(c->b.U, A->ckpt[g], bytes, hipMemcpyDeviceToDevice, c->st));
-		do_prim(c);
-		for (int s = s0; s < s1; s++) step_record<MM>(c, &A->stages[(size_t)(s - s0) * S]);
-		for (int s = s1 - 1; s >= s0; s--) {
-			for (int st = S - 1; st >= 0; st--) {
-				HIPCHK(hipMemcpyAsync(c->b.U, A->stages[(size_t)(s - s0) * S + st], bytes, hipMemcpyDeviceToDevice,
-				                      c->st));
-				double wa = S == 3 ? wa3[st] : wa2[st], wb = S == 3 ? wb3[st] : wb2[st];
-				int mode = st == 0 ? 2 : (st == S - 1 ? 0 : 1);
-				launch_adj_stage<MM>(c, A.get(), wa, wb, mode);
-				std::swap(a.Wn, a.Ws);
-			}
+		int s0 = g * ck, len = std::min(nsteps, s0 + ck) - s0;
+		if (g != nck - 1) {
+			HIPCHK(hipMemcpyAsync(c->b.U, A->ckpt[g], bytes, hipMemcpyDeviceToDevice, c->st));
+			record_steps<MM>(c, A.get(), len);
 		}
+		for (int t = len * S - 1; t >= 0; t--) {
+			int st = t % S;
+			c->b.P = A->Pst + (size_t)t * n4;
+			c->b.G = A->Gst + (size_t)t * n8;
+			double wa = S == 3 ? wa3[st] : wa2[st], wb = S == 3 ? wb3[st] : wb2[st];
+			int mode = st == 0 ? 2 : (st == S - 1 ? 0 : 1);
+			launch_adj_stage<MM>(c, A.get(), wa, wb, mode);
+			std::swap(a.Wn, a.Ws);
+		}
+		c->b.P = P0;
+		c->b.G = G0;
 	}
-	if ((rc = check_launch("adjoint reverse sweep"))) return rc;
+	if (c->nbc > 0 && a.nbedges > 0)
+		hipLaunchKernelGGL(k_bc_reduce, dim3(1), dim3(BLOCK), 0, c->st, c->b, a, c->nbc);
 	if (dJdU0) HIPCHK(hipMemcpyAsync(dJdU0, a.Wn, bytes, hipMemcpyDeviceToHost, c->st));
-	if (dJdbc && c->nbc > 0)
-		HIPCHK(hipMemcpyAsync(dJdbc, a.bcbar, 4 * (size_t)c->nbc * sizeof(double), hipMemcpyDeviceToHost, c->st));
+	if (dJdbc && c->nbc > 0) {
+		if (a.nbedges > 0)
+			HIPCHK(hipMemcpyAsync(dJdbc, a.bcbar, 4 * (size_t)c->nbc * sizeof(double), hipMemcpyDeviceToHost, c->st));
+		else
+			std::fill(dJdbc, dJdbc + 4 * (size_t)c->nbc, 0.0);
+	}
+	// leave the context at the final state
+	HIPCHK(hipMemcpyAsync(c->b.U, A->Ufin, bytes, hipMemcpyDeviceToDevice, c->st));
+	launch_prim<MM>(c);
 	HIPCHK(hipStreamSynchronize(c->st));
-	// leave the context at the final forward state
-	HIPCHK(hipMemcpy(c->b.U, U.data(), bytes, hipMemcpyHostToDevice));
-	do_prim(c);
-	HIPCHK(hipStreamSynchronize(c->st));
-	return check_launch("adjoint");
+	return check_launch("adjoint reverse sweep");
 }
 
 }  // namespace
@@ -609,7 +695,7 @@ int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, do
 extern "C" int cfd2d_adjoint(cfd2d_ctx* c, int nsteps, int ckpt_every, const int* edge_mask, double* J,
                              double* dJdU0, double* dJdbc)
 {
-	if (!c || nsteps < 1 || ckpt_every < 1 || !edge_mask) return fail(CFD2D_ERR_ARG, "cfd2d_adjoint: bad arguments");
+	if (!c || nsteps < 1 || !edge_mask) return fail(CFD2D_ERR_ARG, "cfd2d_adjoint: bad arguments");
 	if ((c->p.flux != CFD2D_FLUX_LAX && c->p.flux != CFD2D_FLUX_HLLC) || c->p.limiter != CFD2D_LIMITER_NONE ||
 	    c->p.viscous || c->p.steady)
 		return fail(CFD2D_ERR_CONFIG, "adjoint: needs FLUX LAX or HLLC, no limiter, inviscid, unsteady");

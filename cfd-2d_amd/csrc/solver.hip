@@ -3,7 +3,9 @@
 //
 // Data layout (structure of arrays, FP64, cells in the reference's order):
 //   cells  U[4][nc] conserved, U0[4][nc] step start, P[4][nc] primitives
-//          (r, p, u, v) of U, G[8][nc] gradients, dtS[nc] = tau_i / S_i,
+//          (r, p, u, v) of U (Po: where the update writes them, = P except
+//          while the adjoint records stages), G[8][nc] gradients,
+//          dtS[nc] = tau_i / S_i,
 //          S[nc], faces cface[3][nc] = edge*2 + side in ascending edge order,
 //          cnb[3][nc] neighbour across each face (-1 on a boundary),
 //          lsqw[6][nc] least-squares weights, flags[nc]
@@ -38,11 +40,12 @@ constexpr int NXCD = 8;
 
 struct Bufs {
 	int nc, ne, nblocks, nmat;
-	double *U, *U0, *P, *G, *dtS, *S, *lsqw, *loc, *R;
+	double *U, *U0, *P, *Po, *G, *dtS, *S, *lsqw, *loc, *R;
 	int *cface, *cnb, *cmat;
 	unsigned* flags;
 	int* blockflag;
-	int* anyflag;  // set by a final-stage update block holding a LIM cell
+	int* anyflag;  // [0] set by a final-stage update block holding a LIM cell,
+	               // cleared by remediation; [1] the same, cleared only by the host
 	int *ec1, *ec2;
 	double *enx, *eny, *el, *edl, *F;
 	int* bctype;
@@ -85,14 +88,15 @@ __device__ __forceinline__ Prim load_prim(const Bufs& b, int i)
 	return Prim{b.P[i], b.P[n + i], b.P[2 * n + i], b.P[3 * n + i]};
 }
 
-__device__ __forceinline__ void store_prim(const Bufs& b, int i, const Prim& q)
+__device__ __forceinline__ void store_prim_to(double* P, size_t n, int i, const Prim& q)
 {
-	size_t n = b.nc;
-	b.P[i] = q.r;
-	b.P[n + i] = q.p;
-	b.P[2 * n + i] = q.u;
-	b.P[3 * n + i] = q.v;
+	P[i] = q.r;
+	P[n + i] = q.p;
+	P[2 * n + i] = q.u;
+	P[3 * n + i] = q.v;
 }
+
+__device__ __forceinline__ void store_prim(const Bufs& b, int i, const Prim& q) { store_prim_to(b.P, b.nc, i, q); }
 
 // ghost of cell i's (cell-centre or reconstructed) state across boundary edge e
 template <bool MM>
@@ -368,7 +372,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(Bufs b, Phys ph)
 			if (!finite) atomicAdd(&b.err[1], 1ull);
 			MatV m = mat_of<MM>(b, ph, i);
 			Prim p = cons_to_prim(u[0], u[1], u[2], u[3], m);
-			store_prim(b, i, p);
+			store_prim_to(b.Po, n, i, p);
 			if (MODE == UPD_RK2_LAST || MODE == UPD_RK3_LAST) {
 				bool out = p.r < ph.ro_min || p.r > ph.ro_max || p.p < ph.p_min || p.p > ph.p_max ||
 				           fabs(p.u) > ph.u_max || fabs(p.v) > ph.u_max;
@@ -377,6 +381,8 @@ __global__ __launch_bounds__(BLOCK) void k_update(Bufs b, Phys ph)
 					b.flags[i] = fl;
 				}
 			}
+		} else if (b.Po != b.P) {
+			store_prim_to(b.Po, n, i, load_prim(b, i));  // frozen cell, primitives to the output slot
 		}
 		flagged = (fl & 0x2u) != 0;
 	}
@@ -384,7 +390,10 @@ __global__ __launch_bounds__(BLOCK) void k_update(Bufs b, Phys ph)
 		int any = __syncthreads_or(flagged);
 		if (threadIdx.x == 0) {
 			b.blockflag[lb] = any;
-			if (any) atomicOr(b.anyflag, 1);
+			if (any) {
+				atomicOr(b.anyflag, 1);
+				atomicOr(b.anyflag + 1, 1);
+			}
 		}
 	}
 }
@@ -495,6 +504,8 @@ struct cfd2d_ctx {
 	Bufs b{};
 	hipStream_t st = nullptr;
 	std::vector<double> hS;
+	std::vector<int> hc1;  // edge c1, length, normal (host copies for the force integrals)
+	std::vector<double> hel, hnx, hny;
 	std::vector<int> case_off;
 	std::vector<double> case_tau;
 	// timing
@@ -509,6 +520,10 @@ struct cfd2d_ctx {
 	hipGraphExec_t graph_exec = nullptr;
 	bool capturing = false;
 	std::vector<void*> allocs;
+	// adjoint topology, built on the first cfd2d_adjoint call
+	int* adj_cslot = nullptr;
+	int* adj_bedges = nullptr;
+	int adj_nbedges = -1;
 };
 
 namespace {
@@ -808,6 +823,10 @@ int cfd2d_ctx_create(const cfd2d_mesh* mesh, const cfd2d_params* p, int n_bc, co
 	c->case_off = m.case_cell_offset;
 	c->case_tau.assign(c->ncase, p->tau);
 	c->hS = m.S;
+	c->hc1 = m.c1;
+	c->hel = m.l;
+	c->hnx = m.nx;
+	c->hny = m.ny;
 	std::vector<double> mtab(5 * (size_t)p->nmat);
 	for (int k = 0; k < p->nmat; k++) {
 		double Cv = p->Cp[k] - GAS_R / p->M[k];
@@ -871,7 +890,7 @@ int cfd2d_ctx_create(const cfd2d_mesh* mesh, const cfd2d_params* p, int n_bc, co
 	    (rc = dalloc(c, &b.G, 2 * n4)) || (rc = dalloc(c, &b.dtS, nc)) || (rc = upload(c, &b.S, m.S)) ||
 	    (rc = dalloc(c, &b.loc, nc)) || (rc = dalloc(c, &b.R, n4)) || (rc = upload(c, &b.cface, cface)) ||
 	    (rc = upload(c, &b.cnb, cnb)) || (rc = upload(c, &b.cmat, cmat)) || (rc = dalloc(c, &b.flags, nc)) ||
-	    (rc = dalloc(c, &b.blockflag, b.nblocks)) || (rc = dalloc(c, &b.anyflag, 1)) || (rc = upload(c, &b.ec1, m.c1)) ||
+	    (rc = dalloc(c, &b.blockflag, b.nblocks)) || (rc = dalloc(c, &b.anyflag, 2)) || (rc = upload(c, &b.ec1, m.c1)) ||
 	    (rc = upload(c, &b.ec2, ec2)) || (rc = upload(c, &b.enx, m.nx)) || (rc = upload(c, &b.eny, m.ny)) ||
 	    (rc = upload(c, &b.el, m.l)) || (rc = upload(c, &b.edl, edl)) || (rc = dalloc(c, &b.F, 4 * (size_t)ne)) ||
 	    (rc = upload(c, &b.bctype, bct)) || (rc = upload(c, &b.bcpar, bcp)) || (rc = upload(c, &b.mtab, mtab)) ||
@@ -894,12 +913,14 @@ int cfd2d_ctx_create(const cfd2d_mesh* mesh, const cfd2d_params* p, int n_bc, co
 		destroy_ctx(c);
 		return hip_fail(e, "hipStreamCreate");
 	}
+	b.Po = b.P;
 	std::vector<unsigned> zf(nc, 0u);
 	std::vector<int> zb(b.nblocks, 0);
+	int zany[2] = {0, 0};
 	unsigned long long zerr[2] = {0, 0};
 	if (hipMemcpy(b.flags, zf.data(), nc * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess ||
 	    hipMemcpy(b.blockflag, zb.data(), b.nblocks * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-	    hipMemcpy(b.anyflag, zb.data(), sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(b.anyflag, zany, sizeof(zany), hipMemcpyHostToDevice) != hipSuccess ||
 	    hipMemcpy(b.err, zerr, sizeof(zerr), hipMemcpyHostToDevice) != hipSuccess) {
 		destroy_ctx(c);
 		return fail(CFD2D_ERR_HIP, "initial upload failed");
@@ -1108,20 +1129,15 @@ int cfd2d_status(cfd2d_ctx* c, long long* hits, int* nonfinite)
 int cfd2d_wall_force(cfd2d_ctx* c, const int* mask, double* fx, double* fy)
 {
 	if (!c || !mask) return fail(CFD2D_ERR_ARG, "null");
-	std::vector<double> P(4 * (size_t)c->nc), el(c->ne), nx(c->ne), ny(c->ne);
-	std::vector<int> c1(c->ne);
+	std::vector<double> pr(c->nc);
 	HIPCHK(hipStreamSynchronize(c->st));
-	HIPCHK(hipMemcpy(P.data(), c->b.P, P.size() * sizeof(double), hipMemcpyDeviceToHost));
-	HIPCHK(hipMemcpy(el.data(), c->b.el, el.size() * sizeof(double), hipMemcpyDeviceToHost));
-	HIPCHK(hipMemcpy(nx.data(), c->b.enx, nx.size() * sizeof(double), hipMemcpyDeviceToHost));
-	HIPCHK(hipMemcpy(ny.data(), c->b.eny, ny.size() * sizeof(double), hipMemcpyDeviceToHost));
-	HIPCHK(hipMemcpy(c1.data(), c->b.ec1, c1.size() * sizeof(int), hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(pr.data(), c->b.P + c->nc, pr.size() * sizeof(double), hipMemcpyDeviceToHost));
 	double X = 0.0, Y = 0.0;
 	for (int e = 0; e < c->ne; e++) {
 		if (!mask[e]) continue;
-		double p = P[(size_t)c->nc + c1[e]];
-		X += p * el[e] * nx[e];
-		Y += p * el[e] * ny[e];
+		double p = pr[c->hc1[e]];
+		X += p * c->hel[e] * c->hnx[e];
+		Y += p * c->hel[e] * c->hny[e];
 	}
 	if (fx) *fx = X;
 	if (fy) *fy = Y;

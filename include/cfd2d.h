@@ -150,7 +150,9 @@ int  cfd2d_ctx_stream(cfd2d_ctx* c, void** hip_stream);
  * on return *J, dJ_dU0 = dJ/d(state at the call) as double[4][nc], and
  * dJ_dbc = dJ/d(parameters of each boundary object) as double[n_bc][4] (NULL
  * skips either).  Forward states are kept every ckpt_every steps and each
- * segment is recomputed once in the reverse sweep.  Supports FLUX LAX and
+ * segment but the last is recomputed once in the reverse sweep; ckpt_every
+ * <= 0 picks the largest interval whose stage records (96 B per cell per RK
+ * stage) fit in half of the free device memory.  Supports FLUX LAX and
  * HLLC, no limiter, inviscid, unsteady; the time step is held fixed.  The
  * context is left at the final state, as after cfd2d_step(c, nsteps). */
 int  cfd2d_adjoint(cfd2d_ctx* c, int nsteps, int ckpt_every, const int* edge_mask, double* J, double* dJ_dU0,

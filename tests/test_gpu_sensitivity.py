@@ -6,7 +6,8 @@ Both sides differentiate the same discrete map exactly, so they agree to
 round-off: <dJ/dU0, v> + <dJ/dpar, w> within 1e-9 relative of the tangent,
 each boundary-parameter derivative within 1e-9 of the largest one, J within
 1e-12.  At 21,600 cells the check is against central differences (h = 1e-6,
-relative 1e-5), which holds where no max()/min() switch sits near a tie.
+relative 1e-6; measured 3e-8 with HLLC), which holds where no max()/min()
+switch sits near a tie.
 """
 import numpy as np
 import pytest
@@ -83,7 +84,7 @@ def test_adjoint_checkpoint_interval_and_final_state():
     s = oracle.OracleSolver(c)
     mask[:] = (s.geometry["edge_group"] == c.mesh.group_index("bump"))
     res = []
-    for ck in (1, 4, 13, 50):
+    for ck in (1, 4, 13, 50, 0):
         g.set_state(c.initial_state())
         res.append(g.adjoint(n, mask, ckpt_every=ck))
         res[-1] = res[-1] + (g.state(),)
@@ -123,11 +124,14 @@ def test_adjoint_finite_differences_21k_cells():
     assert abs(Jof(0.0) - J) <= 1e-13 * abs(J)
     ad = float(np.sum(gU * v))
     g.close()
-    assert abs(ad - fd) <= 1e-5 * abs(fd), (ad, fd)
+    print("adjoint %.15e fd %.15e rel %.3e" % (ad, fd, abs(ad - fd) / abs(fd)))
+    assert abs(ad - fd) <= 1e-6 * abs(fd), (ad, fd)
 
 
-@pytest.mark.parametrize("over", [dict(flux="GODUNOV"), dict(limiter="MINMOD")])
+@pytest.mark.parametrize("over", [dict(flux="GODUNOV"), dict(limiter="MINMOD"), dict(u_max=1e-3)])
 def test_adjoint_rejects_unsupported(over):
+    """Non-smooth configurations are refused (code 5): the exact Riemann
+    solver, the limiter, and a run in which a cell leaves the bounds."""
     cfg = dict(flux="LAX", gradient="GREEN_GAUSS", limiter="NONE", rk_stages=2)
     cfg.update(over)
     c = cases.bump_channel(6, height=0.06, mach=0.8, seed=3, **cfg)
