@@ -11,14 +11,16 @@
 //
 // Per RK stage with input state U_s, U_{s+1} = a U0 + b (U_s + tau/S R(U_s)):
 //   given W = dJ/dU_{s+1}:  acc0 += a W,  dJ/dU_s = b W + (dR/dU_s)^T (b tau/S W)
-// The vector-Jacobian product of R is three passes over the primitives P and
-// gradients G the forward pass recorded for that stage: k_adj_edge (two
-// lanes per edge, one Gauss point each: reverse of the flux, of the ghost on
-// boundaries and of the reconstruction, into a per-edge buffer for both
-// sides), k_adj_cell (each cell gathers its faces' entries in ascending edge
-// order into (P, G) adjoints), k_adj_grad (reverse of the gradient operator:
-// own and neighbours' gradient adjoints, ghost terms, then the
-// primitive->conserved Jacobian and the RK bookkeeping).  Boundary-parameter
+// The vector-Jacobian product of R runs over the primitives P and gradients
+// G the forward pass recorded for that stage.  First the reverse of the flux,
+// of the ghost on boundaries and of the reconstruction into (P, G) adjoints
+// per cell: for HLLC (a VALU-heavy VJP) once per edge, k_adj_edge (two lanes
+// per edge, one Gauss point each, into a per-edge buffer for both sides) then
+// k_adj_cell (each cell gathers its faces' entries in ascending edge order);
+// for LAX k_adj_face (each cell evaluates its three faces from its own side,
+// no edge buffer).  Then k_adj_grad: reverse of the gradient operator (own
+// and neighbours' gradient adjoints, ghost terms), the primitive->conserved
+// Jacobian and the RK bookkeeping.  Boundary-parameter
 // adjoints accumulate per boundary edge and are summed per boundary object
 // once, in edge order.  No floating-point atomics: the result does not depend
 // on scheduling.
@@ -42,7 +44,8 @@ struct AdjBufs {
 	double* Pb;     // [4][nc]
 	double* Gb;     // [8][nc]
 	double* Ea;     // [16][ne]
-	double* Ebc;    // [4][ne] boundary-parameter adjoints, summed over stages per edge
+	double* Ebc;    // [8][ne] boundary-parameter adjoints per edge, summed over stages
+	                // (two slots of 4: one per Gauss point of the edge pass)
 	double* bcbar;  // [nbc][4]
 	const int* cslot;   // [3][nc] slot of this cell in its neighbour's face list
 	const int* bedges;  // boundary edges in ascending order
@@ -92,57 +95,53 @@ __device__ __forceinline__ void ghost_vjp(int type, const double* par, double nx
 }
 
 // reverse of flux_lax: accumulates into Lb, Rb (r, p, u, v) the product of
-// fb with the flux Jacobian
-__device__ __forceinline__ void lax_vjp(const Prim& L, const Prim& R, double nx, double ny, double G, const double* fb,
-                                        double* Lb, double* Rb)
+// fb with the flux Jacobian.  Written for few divisions: 1/r per side, and
+// dc/dp = c/(2p), dc/dr = -c/(2r) on the side that sets the wave speed.
+__device__ __forceinline__ void lax_vjp(const Prim& L, const Prim& R, double nx, double ny, double G, double igm1,
+                                        const double* fb, double* Lb, double* Rb)
 {
 	double unl = L.u * nx + L.v * ny, unr = R.u * nx + R.v * ny;
-	double cl = sqrt(G * L.p / L.r), cr = sqrt(G * R.p / R.r);
+	double irl = 1.0 / L.r, irr = 1.0 / R.r;
+	double cl = sqrt(G * L.p * irl), cr = sqrt(G * R.p * irr);
 	double al = fabs(unl) + cl, ar = fabs(unr) + cr;
 	bool left = al > ar;
 	double alpha = left ? al : ar;
-	double rol = L.r, rul = L.r * L.u, rvl = L.r * L.v, rel = L.p / (G - 1.0) + 0.5 * L.r * (L.u * L.u + L.v * L.v);
-	double ror = R.r, rur = R.r * R.u, rvr = R.r * R.v, rer = R.p / (G - 1.0) + 0.5 * R.r * (R.u * R.u + R.v * R.v);
-	double frl = rol * unl, frr = ror * unr;
+	double kl = 0.5 * (L.u * L.u + L.v * L.v), kr = 0.5 * (R.u * R.u + R.v * R.v);
+	double rel = L.p * igm1 + L.r * kl, rer = R.p * igm1 + R.r * kr;
+	double frl = L.r * unl, frr = R.r * unr;
 	double h0 = 0.5 * fb[0], h1 = 0.5 * fb[1], h2 = 0.5 * fb[2], h3 = 0.5 * fb[3];
-	double alb = -(h0 * (ror - rol) + h1 * (rur - rul) + h2 * (rvr - rvl) + h3 * (rer - rel));
-	double frrb = h0 + h1 * R.u + h2 * R.v;
-	double frlb = h0 + h1 * L.u + h2 * L.v;
-	double Rub = h1 * frr, Lub = h1 * frl, Rvb = h2 * frr, Lvb = h2 * frl;
-	double Rpb = h1 * nx + h2 * ny + h3 * unr, Lpb = h1 * nx + h2 * ny + h3 * unl;
-	double unrb = h3 * (rer + R.p), unlb = h3 * (rel + L.p);
-	double rorb = -h0 * alpha, rolb = h0 * alpha;
+	double alb = -(h0 * (R.r - L.r) + h1 * (R.r * R.u - L.r * L.u) + h2 * (R.r * R.v - L.r * L.v) + h3 * (rer - rel));
+	double frrb = h0 + h1 * R.u + h2 * R.v, frlb = h0 + h1 * L.u + h2 * L.v;
+	double unrb = h3 * (rer + R.p) + frrb * R.r, unlb = h3 * (rel + L.p) + frlb * L.r;
+	// adjoints of the conserved variables (r, ru, rv, re) of each side
+	double rorb = frrb * unr - h0 * alpha, rolb = frlb * unl + h0 * alpha;
 	double rurb = -h1 * alpha, rulb = h1 * alpha;
 	double rvrb = -h2 * alpha, rvlb = h2 * alpha;
-	double rerb = h3 * unr - h3 * alpha, relb = h3 * unl + h3 * alpha;
-	rorb += frrb * unr;
-	unrb += frrb * ror;
-	rolb += frlb * unl;
-	unlb += frlb * rol;
-	double clb = 0.0, crb = 0.0;
+	double rerb = h3 * (unr - alpha), relb = h3 * (unl + alpha);
+	double Rub = h1 * frr, Lub = h1 * frl, Rvb = h2 * frr, Lvb = h2 * frl;
+	double Rpb = h1 * nx + h2 * ny + h3 * unr, Lpb = h1 * nx + h2 * ny + h3 * unl;
+	double Lrb = 0.0, Rrb = 0.0;
+	// alpha = |u.n| + c of the side with the larger value
+	double cs = left ? cl : cr, ps = left ? L.p : R.p, irs = left ? irl : irr;
+	double ts = alb * 0.5 * cs;
+	double psb = ts / ps, rsb = -ts * irs;
 	if (left) {
 		unlb += alb * (unl < 0 ? -1.0 : 1.0);
-		clb = alb;
+		Lpb += psb;
+		Lrb += rsb;
 	} else {
 		unrb += alb * (unr < 0 ? -1.0 : 1.0);
-		crb = alb;
+		Rpb += psb;
+		Rrb += rsb;
 	}
-	Lpb += clb * G / (2.0 * L.r * cl);
-	double Lrb = -clb * cl / (2.0 * L.r);
-	Rpb += crb * G / (2.0 * R.r * cr);
-	double Rrb = -crb * cr / (2.0 * R.r);
-	Lrb += rolb + rulb * L.u + rvlb * L.v + relb * 0.5 * (L.u * L.u + L.v * L.v);
-	Lub += rulb * L.r + relb * L.r * L.u;
-	Lvb += rvlb * L.r + relb * L.r * L.v;
-	Lpb += relb / (G - 1.0);
-	Rrb += rorb + rurb * R.u + rvrb * R.v + rerb * 0.5 * (R.u * R.u + R.v * R.v);
-	Rub += rurb * R.r + rerb * R.r * R.u;
-	Rvb += rvrb * R.r + rerb * R.r * R.v;
-	Rpb += rerb / (G - 1.0);
-	Lub += unlb * nx;
-	Lvb += unlb * ny;
-	Rub += unrb * nx;
-	Rvb += unrb * ny;
+	Lrb += rolb + rulb * L.u + rvlb * L.v + relb * kl;
+	Lub += rulb * L.r + relb * L.r * L.u + unlb * nx;
+	Lvb += rvlb * L.r + relb * L.r * L.v + unlb * ny;
+	Lpb += relb * igm1;
+	Rrb += rorb + rurb * R.u + rvrb * R.v + rerb * kr;
+	Rub += rurb * R.r + rerb * R.r * R.u + unrb * nx;
+	Rvb += rvrb * R.r + rerb * R.r * R.v + unrb * ny;
+	Rpb += rerb * igm1;
 	Lb[0] += Lrb; Lb[1] += Lpb; Lb[2] += Lub; Lb[3] += Lvb;
 	Rb[0] += Rrb; Rb[1] += Rpb; Rb[2] += Rub; Rb[3] += Rvb;
 }
@@ -224,7 +223,7 @@ __device__ __forceinline__ void hllc_vjp(const Prim& L, const Prim& R, double nx
 // Reverse of the edge pass for one stage.  Seed F_e-bar = (Rb_c2 - Rb_c1) l/2
 // with Rb = scale * tau/S * W; each lane takes one Gauss point: the flux VJP,
 // the ghost VJP on a boundary, then its (r, p, u, v) adjoints on both sides.
-// The boundary-parameter adjoints of the pair are added into Ebc.
+// Each lane adds its boundary-parameter adjoints into its own Ebc slot.
 template <bool MM, int FLUX>
 __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, double scale)
 {
@@ -253,7 +252,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, 
 	}
 	double Lb[4] = {0, 0, 0, 0}, Rb[4] = {0, 0, 0, 0}, parb[4] = {0, 0, 0, 0};
 	if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb);
-	else lax_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb);
+	else lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb);
 	if (c2 < 0) {
 		int bc = -c2 - 1;
 		double Cb[4] = {0, 0, 0, 0};
@@ -264,10 +263,88 @@ __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, 
 		a.Ea[(size_t)(4 * gp + q) * ne + e] = Lb[q];
 		a.Ea[(size_t)(8 + 4 * gp + q) * ne + e] = Rb[q];
 	}
-	double o[4];
-	for (int q = 0; q < 4; q++) o[q] = __shfl_xor(parb[q], 1);
-	if (c2 < 0 && gp == 0)
-		for (int q = 0; q < 4; q++) a.Ebc[(size_t)q * ne + e] += parb[q] + o[q];
+	if (c2 < 0)
+		for (int q = 0; q < 4; q++) a.Ebc[(size_t)(4 * gp + q) * ne + e] += parb[q];
+}
+
+// Cell-centric reverse of the edge pass for fluxes whose VJP is cheap (LAX):
+// each cell evaluates the flux VJP of its three faces and keeps its own side,
+// so every interior face is evaluated by both of its cells.  Twice the
+// arithmetic of k_adj_edge, but no per-edge buffer and no second gather:
+// (P, G) adjoints go straight to Pb, Gb.  Two lanes per cell, one Gauss point
+// each, summed by a lane shuffle.  A cell owns its boundary faces and adds
+// their parameter adjoints into Ebc (one slot per Gauss point).
+template <bool MM, int FLUX>
+__global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, double scale)
+{
+	int t = xcd_block() * BLOCK + threadIdx.x;
+	int i = t >> 1, gp = t & 1;
+	if (i >= b.nc) return;  // both lanes of a pair leave together
+	size_t n = b.nc, ne = b.ne;
+	Prim pi = load_prim(b, i);
+	double wi[4];
+	double si = scale * b.dtS[i];
+	for (int q = 0; q < 4; q++) wi[q] = si * a.Wn[q * n + i];
+	double pb[4] = {0, 0, 0, 0};
+	double gb[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	for (int k = 0; k < 3; k++) {
+		int f = b.cface[k * n + i];
+		int e = f >> 1, side = f & 1;
+		int j = b.cnb[k * n + i];
+		double nx = b.enx[e], ny = b.eny[e];
+		double hl = b.el[e] * 0.5;
+		// F_e-bar = (Rb_c2 - Rb_c1) l/2; this cell is c1 (side 0) or c2 (side 1)
+		double fb[4];
+		Prim pj = pi;
+		if (j >= 0) {
+			pj = load_prim(b, j);
+			double sj = scale * b.dtS[j];
+			for (int q = 0; q < 4; q++) {
+				double wj = sj * a.Wn[q * n + j];
+				fb[q] = (side ? wi[q] - wj : wj - wi[q]) * hl;
+			}
+		} else {
+			for (int q = 0; q < 4; q++) fb[q] = -wi[q] * hl;
+		}
+		int c1 = side ? j : i, c2 = side ? i : j;
+		double d1x = b.edl[(size_t)(2 * gp) * ne + e], d1y = b.edl[(size_t)(2 * gp + 1) * ne + e];
+		Prim L = side ? pj : pi, R;
+		recon(L, b, c1, d1x, d1y);
+		double d2x = 0.0, d2y = 0.0;
+		if (j >= 0) {
+			d2x = b.edl[(size_t)(4 + 2 * gp) * ne + e];
+			d2y = b.edl[(size_t)(5 + 2 * gp) * ne + e];
+			R = side ? pi : pj;
+			recon(R, b, c2, d2x, d2y);
+		} else {
+			R = ghost_of<MM>(b, ph, e, i, L, pi);
+		}
+		double Lb[4] = {0, 0, 0, 0}, Rb[4] = {0, 0, 0, 0};
+		if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb);
+		else lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb);
+		if (j < 0) {
+			int bc = -b.ec2[e] - 1;
+			double Cb[4] = {0, 0, 0, 0}, parb[4] = {0, 0, 0, 0};
+			ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], nx, ny, L, pi, mat_of<MM>(b, ph, i), Rb, Lb, Cb, parb);
+			for (int q = 0; q < 4; q++) {
+				pb[q] += Cb[q];
+				a.Ebc[(size_t)(4 * gp + q) * ne + e] += parb[q];
+			}
+		}
+		double dx = side ? d2x : d1x, dy = side ? d2y : d1y;
+		for (int q = 0; q < 4; q++) {
+			double o = side ? Rb[q] : Lb[q];
+			pb[q] += o;
+			gb[2 * q] += o * dx;
+			gb[2 * q + 1] += o * dy;
+		}
+	}
+	for (int q = 0; q < 4; q++) pb[q] += __shfl_xor(pb[q], 1);
+	for (int q = 0; q < 8; q++) gb[q] += __shfl_xor(gb[q], 1);
+	if (gp == 0) {
+		for (int q = 0; q < 4; q++) a.Pb[q * n + i] = pb[q];
+		for (int q = 0; q < 8; q++) a.Gb[q * n + i] = gb[q];
+	}
 }
 
 // each cell gathers its side of its three edges: P adjoint = sum of the
@@ -388,7 +465,7 @@ __global__ __launch_bounds__(BLOCK) void k_bc_reduce(Bufs b, AdjBufs a, int nbc)
 		for (int k = threadIdx.x; k < a.nbedges; k += BLOCK) {
 			int e = a.bedges[k];
 			if (-b.ec2[e] - 1 != bc) continue;
-			for (int q = 0; q < 4; q++) s[q] += a.Ebc[(size_t)q * ne + e];
+			for (int q = 0; q < 4; q++) s[q] += a.Ebc[(size_t)q * ne + e] + a.Ebc[(size_t)(4 + q) * ne + e];
 		}
 		for (int q = 0; q < 4; q++) part[threadIdx.x][q] = s[q];
 		__syncthreads();
@@ -428,11 +505,10 @@ __global__ __launch_bounds__(BLOCK) void k_seed(Bufs b, Phys ph, AdjBufs a, cons
 	pout[k] = p.p;
 }
 
-}  // namespace
-
-struct cfd2d_adj {
+// device buffers of one adjoint call, carved from the context's adjoint
+// arena (kept between calls, grown when a call needs more)
+struct AdjWork {
 	AdjBufs a{};
-	std::vector<void*> allocs;
 	std::vector<double*> ckpt;  // state at the start of every segment but the last
 	double* Pst = nullptr;      // [nrec + 1][4][nc] stage primitives
 	double* Gst = nullptr;      // [nrec][8][nc] stage gradients
@@ -442,27 +518,53 @@ struct cfd2d_adj {
 	double* seed_p = nullptr;
 };
 
-namespace {
+struct Carve {
+	char* base;
+	size_t off = 0;
+	template <typename T>
+	T* take(size_t count)
+	{
+		off = (off + 255) & ~(size_t)255;
+		T* p = base ? (T*)(base + off) : nullptr;
+		off += (count ? count : 1) * sizeof(T);
+		return p;
+	}
+};
 
-template <typename T>
-int adj_alloc(cfd2d_adj* A, T** p, size_t count)
+void carve_work(AdjWork& W, Carve& cv, int nck, size_t nrec, size_t nc, size_t ne, int nbc, bool edge_buffer,
+                size_t K)
 {
-	void* q = nullptr;
-	hipError_t e = hipMalloc(&q, (count ? count : 1) * sizeof(T));
-	if (e != hipSuccess) return hip_fail(e, "hipMalloc (adjoint)");
-	A->allocs.push_back(q);
-	*p = (T*)q;
+	size_t n4 = 4 * nc, n8 = 8 * nc;
+	W.ckpt.assign(nck - 1, nullptr);
+	for (auto& p : W.ckpt) p = cv.take<double>(n4);
+	W.Pst = cv.take<double>((nrec + 1) * n4);
+	W.Gst = cv.take<double>(nrec * n8);
+	W.Ufin = cv.take<double>(n4);
+	W.a.Wn = cv.take<double>(n4);
+	W.a.Ws = cv.take<double>(n4);
+	W.a.Acc = cv.take<double>(n4);
+	W.a.Pb = cv.take<double>(n4);
+	W.a.Gb = cv.take<double>(n8);
+	W.a.Ea = edge_buffer ? cv.take<double>((size_t)EA * ne) : nullptr;
+	W.a.Ebc = cv.take<double>(8 * ne);
+	W.a.bcbar = cv.take<double>(4 * (size_t)(nbc > 0 ? nbc : 1));
+	W.seed_cells = cv.take<int>(K);
+	W.seed_w = cv.take<double>(K);
+	W.seed_p = cv.take<double>(K);
+}
+
+int ensure_arena(cfd2d_ctx* c, size_t bytes)
+{
+	if (bytes <= c->adj_ws_bytes) return 0;
+	if (c->adj_ws) {
+		(void)hipFree(c->adj_ws);
+		c->adj_ws = nullptr;
+		c->adj_ws_bytes = 0;
+	}
+	HIPCHK(hipMalloc(&c->adj_ws, bytes));
+	c->adj_ws_bytes = bytes;
 	return 0;
 }
-
-void adj_free(cfd2d_adj* A)
-{
-	if (!A) return;
-	for (void* p : A->allocs) (void)hipFree(p);
-	delete A;
-}
-
-using AdjGuard = std::unique_ptr<cfd2d_adj, void (*)(cfd2d_adj*)>;
 
 // the slot of each cell in its neighbours' face lists and the boundary edges
 // in ascending order, built once per context
@@ -502,7 +604,7 @@ void launch_fill(cfd2d_ctx* c, double* x, size_t count)
 // nsteps forward steps writing every stage's P and G into the stage slots
 // (P slot 0 = primitives of the current state); U advances in place
 template <bool MM>
-void record_steps(cfd2d_ctx* c, cfd2d_adj* A, int nsteps)
+void record_steps(cfd2d_ctx* c, AdjWork* A, int nsteps)
 {
 	size_t n4 = 4 * (size_t)c->nc, n8 = 8 * (size_t)c->nc;
 	double *P0 = c->b.P, *Po0 = c->b.Po, *G0 = c->b.G;
@@ -531,15 +633,18 @@ void record_steps(cfd2d_ctx* c, cfd2d_adj* A, int nsteps)
 }
 
 template <bool MM>
-void launch_adj_stage(cfd2d_ctx* c, cfd2d_adj* A, double wa, double wb, int mode)
+void launch_adj_stage(cfd2d_ctx* c, AdjWork* A, double wa, double wb, int mode)
 {
-	if (c->p.flux == CFD2D_FLUX_HLLC)
+	if (c->p.flux == CFD2D_FLUX_HLLC) {
+		// VALU-heavy VJP: once per edge, through the edge buffer
 		hipLaunchKernelGGL((k_adj_edge<MM, CFD2D_FLUX_HLLC>), dim3(grid_of(2 * c->ne)), dim3(BLOCK), 0, c->st, c->b,
 		                   c->ph, A->a, wb);
-	else
-		hipLaunchKernelGGL((k_adj_edge<MM, CFD2D_FLUX_LAX>), dim3(grid_of(2 * c->ne)), dim3(BLOCK), 0, c->st, c->b,
+		hipLaunchKernelGGL(k_adj_cell, dim3(grid_of(c->nc)), dim3(BLOCK), 0, c->st, c->b, A->a);
+	} else {
+		// cheap VJP: evaluated from both sides, no edge buffer
+		hipLaunchKernelGGL((k_adj_face<MM, CFD2D_FLUX_LAX>), dim3(grid_of(2 * c->nc)), dim3(BLOCK), 0, c->st, c->b,
 		                   c->ph, A->a, wb);
-	hipLaunchKernelGGL(k_adj_cell, dim3(grid_of(c->nc)), dim3(BLOCK), 0, c->st, c->b, A->a);
+	}
 	if (c->p.gradient == CFD2D_GRAD_LSQ)
 		hipLaunchKernelGGL((k_adj_grad<MM, true>), dim3(grid_of(c->nc)), dim3(BLOCK), 0, c->st, c->b, c->ph, A->a, wa,
 		                   wb, mode);
@@ -549,11 +654,13 @@ void launch_adj_stage(cfd2d_ctx* c, cfd2d_adj* A, double wa, double wb, int mode
 }
 
 // steps per segment when the caller leaves it to the library: the largest
-// interval whose stage records fit in half of the free device memory
+// interval whose stage records fit in half of the free device memory (the
+// adjoint arena of earlier calls counted as free)
 int auto_interval(cfd2d_ctx* c, int nsteps)
 {
 	size_t fr = 0, tot = 0;
 	if (hipMemGetInfo(&fr, &tot) != hipSuccess) return std::min(nsteps, 20);
+	fr += c->adj_ws_bytes;  // the arena of earlier calls is reused
 	double per_step = (double)c->p.rk_stages * 12.0 * 8.0 * c->nc;
 	double k = (0.5 * (double)fr - 64.0 * 8.0 * c->nc - 24.0 * 8.0 * c->ne) / per_step;
 	if (k < 1.0) return 1;
@@ -569,29 +676,8 @@ int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, do
 	if (rc) return rc;
 	if (ck <= 0) ck = auto_interval(c, nsteps);
 	ck = std::min(ck, nsteps);
-	auto* A0 = new (std::nothrow) cfd2d_adj;
-	if (!A0) return fail(CFD2D_ERR_ARG, "out of host memory");
-	AdjGuard A(A0, adj_free);
 	int nck = (nsteps + ck - 1) / ck;
 	size_t nrec = (size_t)ck * S;
-	A->ckpt.resize(nck - 1);
-	for (int k = 0; k < nck - 1 && !rc; k++) rc = adj_alloc(A.get(), &A->ckpt[k], n4);
-	AdjBufs& a = A->a;
-	if (!rc) rc = adj_alloc(A.get(), &A->Pst, (nrec + 1) * n4);
-	if (!rc) rc = adj_alloc(A.get(), &A->Gst, nrec * n8);
-	if (!rc) rc = adj_alloc(A.get(), &A->Ufin, n4);
-	if (!rc) rc = adj_alloc(A.get(), &a.Wn, n4);
-	if (!rc) rc = adj_alloc(A.get(), &a.Ws, n4);
-	if (!rc) rc = adj_alloc(A.get(), &a.Acc, n4);
-	if (!rc) rc = adj_alloc(A.get(), &a.Pb, n4);
-	if (!rc) rc = adj_alloc(A.get(), &a.Gb, n8);
-	if (!rc) rc = adj_alloc(A.get(), &a.Ea, (size_t)EA * ne);
-	if (!rc) rc = adj_alloc(A.get(), &a.Ebc, 4 * (size_t)ne);
-	if (!rc) rc = adj_alloc(A.get(), &a.bcbar, 4 * (size_t)(c->nbc > 0 ? c->nbc : 1));
-	if (rc) return rc;
-	a.cslot = c->adj_cslot;
-	a.bedges = c->adj_bedges;
-	a.nbedges = c->adj_nbedges;
 	// objective edges grouped by c1, ascending edge order within a cell
 	std::vector<std::pair<int, int>> me;
 	for (int e = 0; e < ne; e++)
@@ -609,14 +695,23 @@ int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, do
 		wsum.back() += c->hel[me[k].second] * c->hny[me[k].second];
 	}
 	int K = (int)cells.size();
-	if ((rc = adj_alloc(A.get(), &A->seed_cells, cells.size())) || (rc = adj_alloc(A.get(), &A->seed_w, wsum.size())) ||
-	    (rc = adj_alloc(A.get(), &A->seed_p, cells.size())))
-		return rc;
+	AdjWork work;
+	AdjWork* A = &work;
+	bool edge_buffer = c->p.flux == CFD2D_FLUX_HLLC;
+	Carve sizing{nullptr};
+	carve_work(work, sizing, nck, nrec, nc, ne, c->nbc, edge_buffer, K);
+	if ((rc = ensure_arena(c, sizing.off))) return rc;
+	Carve cv{(char*)c->adj_ws};
+	carve_work(work, cv, nck, nrec, nc, ne, c->nbc, edge_buffer, K);
+	AdjBufs& a = A->a;
+	a.cslot = c->adj_cslot;
+	a.bedges = c->adj_bedges;
+	a.nbedges = c->adj_nbedges;
 	if (K > 0) {
 		HIPCHK(hipMemcpyAsync(A->seed_cells, cells.data(), K * sizeof(int), hipMemcpyHostToDevice, c->st));
 		HIPCHK(hipMemcpyAsync(A->seed_w, wsum.data(), K * sizeof(double), hipMemcpyHostToDevice, c->st));
 	}
-	launch_fill(c, a.Ebc, 4 * (size_t)ne);
+	launch_fill(c, a.Ebc, 8 * (size_t)ne);
 	HIPCHK(hipMemsetAsync(c->b.anyflag + 1, 0, sizeof(int), c->st));
 
 	// forward pass: checkpoints, then the last segment recorded in place
@@ -625,7 +720,7 @@ int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, do
 		if (s % ck == 0) HIPCHK(hipMemcpyAsync(A->ckpt[s / ck], c->b.U, bytes, hipMemcpyDeviceToDevice, c->st));
 		step_once(c);
 	}
-	record_steps<MM>(c, A.get(), nsteps - last0);
+	record_steps<MM>(c, A, nsteps - last0);
 	HIPCHK(hipMemcpyAsync(A->Ufin, c->b.U, bytes, hipMemcpyDeviceToDevice, c->st));
 	// seed dJ/dU at the final state (its primitives are the last P slot)
 	launch_fill(c, a.Wn, n4);
@@ -660,7 +755,7 @@ int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, do
 		int s0 = g * ck, len = std::min(nsteps, s0 + ck) - s0;
 		if (g != nck - 1) {
 			HIPCHK(hipMemcpyAsync(c->b.U, A->ckpt[g], bytes, hipMemcpyDeviceToDevice, c->st));
-			record_steps<MM>(c, A.get(), len);
+			record_steps<MM>(c, A, len);
 		}
 		for (int t = len * S - 1; t >= 0; t--) {
 			int st = t % S;
@@ -668,7 +763,7 @@ int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, do
 			c->b.G = A->Gst + (size_t)t * n8;
 			double wa = S == 3 ? wa3[st] : wa2[st], wb = S == 3 ? wb3[st] : wb2[st];
 			int mode = st == 0 ? 2 : (st == S - 1 ? 0 : 1);
-			launch_adj_stage<MM>(c, A.get(), wa, wb, mode);
+			launch_adj_stage<MM>(c, A, wa, wb, mode);
 			std::swap(a.Wn, a.Ws);
 		}
 		c->b.P = P0;

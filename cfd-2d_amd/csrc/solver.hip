@@ -57,7 +57,7 @@ struct Bufs {
 struct Phys {
 	MatV m0;
 	RimConst rk;
-	double gam_flux, cfl;
+	double gam_flux, igm1, cfl;  // igm1 = 1 / (gam_flux - 1)
 	double ro_min, ro_max, p_min, p_max, u_max;
 	int max_newton, cfl_mode, steady;
 };
@@ -524,6 +524,10 @@ struct cfd2d_ctx {
 	int* adj_cslot = nullptr;
 	int* adj_bedges = nullptr;
 	int adj_nbedges = -1;
+	// adjoint arena: checkpoints, stage records and work arrays, kept between
+	// calls and grown on demand
+	void* adj_ws = nullptr;
+	size_t adj_ws_bytes = 0;
 };
 
 namespace {
@@ -754,6 +758,7 @@ void destroy_ctx(cfd2d_ctx* c)
 	for (auto e : c->ev0) (void)hipEventDestroy(e);
 	for (auto e : c->ev1) (void)hipEventDestroy(e);
 	for (void* p : c->allocs) (void)hipFree(p);
+	if (c->adj_ws) (void)hipFree(c->adj_ws);
 	if (c->st) (void)hipStreamDestroy(c->st);
 	delete c;
 }
@@ -840,6 +845,7 @@ int cfd2d_ctx_create(const cfd2d_mesh* mesh, const cfd2d_params* p, int n_bc, co
 	}
 	c->ph.m0 = MatV{mtab[0], mtab[1], mtab[2], mtab[3], mtab[4]};
 	c->ph.gam_flux = p->gam_flux;
+	c->ph.igm1 = 1.0 / (p->gam_flux - 1.0);
 	c->ph.rk = rim_const(p->gam_flux);
 	c->ph.cfl = p->cfl;
 	c->ph.ro_min = p->ro_min;
