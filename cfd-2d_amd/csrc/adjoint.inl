@@ -38,11 +38,14 @@ constexpr int EA = 16;  // per edge [side][Gauss point][r, p, u, v]; on a bounda
                         // c2 half holds the adjoint of the ghost's cell-centre input
 
 struct AdjBufs {
-	double* Wn;     // dJ/dU_{s+1}
-	double* Ws;     // dJ/dU_s
-	double* Acc;    // sum of a * W over the stages of the current step
-	double* Pb;     // [4][nc]
-	double* Gb;     // [8][nc]
+	double* Wn;     // [nc][4] dJ/dU_{s+1}
+	double* Ws;     // [nc][4] dJ/dU_s
+	double* Acc;    // [nc][4] sum of a * W over the stages of the current step
+	double* Pb;     // [4][nc] adjoint of the primitives
+	double* Gb;     // [8][nc] adjoint of the gradients
+	                // (separate arrays: held as per-cell records, the gfx950
+	                // build of k_adj_grad returned wrong boundary-cell values
+	                // at -O2/-O3, correct at -O1 or -fno-unroll-loops)
 	double* Ea;     // [16][ne]
 	double* Ebc;    // [8][ne] boundary-parameter adjoints per edge, summed over stages
 	                // (two slots of 4: one per Gauss point of the edge pass)
@@ -52,50 +55,66 @@ struct AdjBufs {
 	int nbedges;
 };
 
-// d(ghost)/d(inside L, centre C, parameters) applied to gb (r, p, u, v order):
-// the reverse of ghost_state
+// d(ghost)/d(inside L, centre C, parameters) applied to gb (r, p, u, v
+// order): the reverse of ghost_state.  Every output component is assigned on
+// every path (the callers add them where they belong).
 __device__ __forceinline__ void ghost_vjp(int type, const double* par, double nx, double ny, const Prim& L,
-                                          const Prim& C, const MatV& m, const double* gb, double* Lb, double* Cb,
-                                          double* parb)
+                                          const Prim& C, const MatV& m, const double* gb, double* dL, double* dC,
+                                          double* dpar)
 {
 	if (type == BND_INLET) {
+		// u, v, T, p from the parameters; r = P M / (T gR)
 		double T = par[2], P = par[3];
-		parb[0] += gb[2];
-		parb[1] += gb[3];
-		parb[3] += gb[1] + gb[0] * m.M / (T * GAS_R);
-		parb[2] += -gb[0] * P * m.M / (T * T * GAS_R);
+		for (int q = 0; q < 4; q++) {
+			dL[q] = 0.0;
+			dC[q] = 0.0;
+		}
+		dpar[0] = gb[2];
+		dpar[1] = gb[3];
+		dpar[2] = -gb[0] * P * m.M / (T * T * GAS_R);
+		dpar[3] = gb[1] + gb[0] * m.M / (T * GAS_R);
 		return;
 	}
 	if (type == BND_PRESSURE_OUTLET) {
+		// r, u, v copied; p fixed while |u.n| < c of the cell centre
 		double un = L.u * nx + L.v * ny;
 		double czc = sqrt(m.gam * C.p / C.r);
-		Lb[0] += gb[0];
-		Lb[2] += gb[2];
-		Lb[3] += gb[3];
-		if (fabs(un) < czc) parb[0] += gb[1];
-		else Lb[1] += gb[1];
+		bool sub = fabs(un) < czc;
+		dL[0] = gb[0];
+		dL[1] = sub ? 0.0 : gb[1];
+		dL[2] = gb[2];
+		dL[3] = gb[3];
+		for (int q = 0; q < 4; q++) dC[q] = 0.0;
+		dpar[0] = sub ? gb[1] : 0.0;
+		dpar[1] = 0.0;
+		dpar[2] = 0.0;
+		dpar[3] = 0.0;
 		return;
 	}
-	// outlet and walls: r = pL M / (Tc gR), Tc = pc / (rc (gamma-1) Cv)
+	// outlet and walls: p copied, r = pL M / (Tc gR), Tc = pc / (rc (gamma-1) Cv);
+	// walls mirror the normal velocity
 	double Tc = temperature(C, m);
 	double k = m.M / GAS_R;
-	Lb[1] += gb[0] * k / Tc + gb[1];
 	double Tb = -gb[0] * L.p * k / (Tc * Tc);
 	double den = C.r * (m.gam - 1) * m.Cv;
-	Cb[1] += Tb / den;
-	Cb[0] += -Tb * C.p / (C.r * den);
+	dL[0] = 0.0;
+	dL[1] = gb[0] * k / Tc + gb[1];
 	if (type == BND_OUTLET) {
-		Lb[2] += gb[2];
-		Lb[3] += gb[3];
+		dL[2] = gb[2];
+		dL[3] = gb[3];
 	} else {
 		double Unb = -2.0 * (nx * gb[2] + ny * gb[3]);
-		Lb[2] += gb[2] + Unb * nx;
-		Lb[3] += gb[3] + Unb * ny;
+		dL[2] = gb[2] + Unb * nx;
+		dL[3] = gb[3] + Unb * ny;
 	}
+	dC[0] = -Tb * C.p / (C.r * den);
+	dC[1] = Tb / den;
+	dC[2] = 0.0;
+	dC[3] = 0.0;
+	for (int q = 0; q < 4; q++) dpar[q] = 0.0;
 }
 
-// reverse of flux_lax: accumulates into Lb, Rb (r, p, u, v) the product of
-// fb with the flux Jacobian.  Written for few divisions: 1/r per side, and
+// reverse of flux_lax: Lb, Rb (r, p, u, v) = fb times the flux Jacobian.  Written for few divisions: 1/r per side, and
 // dc/dp = c/(2p), dc/dr = -c/(2r) on the side that sets the wave speed.
 __device__ __forceinline__ void lax_vjp(const Prim& L, const Prim& R, double nx, double ny, double G, double igm1,
                                         const double* fb, double* Lb, double* Rb)
@@ -142,8 +161,8 @@ __device__ __forceinline__ void lax_vjp(const Prim& L, const Prim& R, double nx,
 	Rub += rurb * R.r + rerb * R.r * R.u + unrb * nx;
 	Rvb += rvrb * R.r + rerb * R.r * R.v + unrb * ny;
 	Rpb += rerb * igm1;
-	Lb[0] += Lrb; Lb[1] += Lpb; Lb[2] += Lub; Lb[3] += Lvb;
-	Rb[0] += Rrb; Rb[1] += Rpb; Rb[2] += Rub; Rb[3] += Rvb;
+	Lb[0] = Lrb; Lb[1] = Lpb; Lb[2] = Lub; Lb[3] = Lvb;
+	Rb[0] = Rrb; Rb[1] = Rpb; Rb[2] = Rub; Rb[3] = Rvb;
 }
 
 // one-direction forward-mode number for the HLLC Jacobian
@@ -215,8 +234,8 @@ __device__ __forceinline__ void hllc_vjp(const Prim& L, const Prim& R, double nx
 		}
 		hllc_d1(f, l, r, nx, ny, G);
 		double s = fb[0] * f[0].d + fb[1] * f[1].d + fb[2] * f[2].d + fb[3] * f[3].d;
-		if (j < 4) Lb[j] += s;
-		else Rb[j - 4] += s;
+		if (j < 4) Lb[j] = s;
+		else Rb[j - 4] = s;
 	}
 }
 
@@ -230,15 +249,18 @@ __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, 
 	int t = xcd_block() * BLOCK + threadIdx.x;
 	int e = t >> 1, gp = t & 1;
 	if (e >= b.ne) return;  // both lanes of a pair leave together
-	size_t ne = b.ne, n = b.nc;
+	size_t ne = b.ne;
 	int c1 = b.ec1[e], c2 = b.ec2[e];
 	double nx = b.enx[e], ny = b.eny[e];
 	double hl = b.el[e] * 0.5;
 	double fb[4];
 	double s1 = scale * b.dtS[c1], s2 = c2 >= 0 ? scale * b.dtS[c2] : 0.0;
+	double w1[4], w2[4] = {0, 0, 0, 0};
+	ld4(a.Wn, c1, w1);
+	if (c2 >= 0) ld4(a.Wn, c2, w2);
 	for (int q = 0; q < 4; q++) {
-		double r1 = s1 * a.Wn[q * n + c1];
-		double r2 = c2 >= 0 ? s2 * a.Wn[q * n + c2] : 0.0;
+		double r1 = s1 * w1[q];
+		double r2 = c2 >= 0 ? s2 * w2[q] : 0.0;
 		fb[q] = (r2 - r1) * hl;
 	}
 	Prim p1 = load_prim(b, c1);
@@ -250,21 +272,24 @@ __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, 
 	} else {
 		R = ghost_of<MM>(b, ph, e, c1, L, p1);
 	}
-	double Lb[4] = {0, 0, 0, 0}, Rb[4] = {0, 0, 0, 0}, parb[4] = {0, 0, 0, 0};
+	double Lb[4], Rb[4];
 	if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb);
 	else lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb);
 	if (c2 < 0) {
+		// the ghost's inputs: the reconstructed L, the cell centre, the parameters
 		int bc = -c2 - 1;
-		double Cb[4] = {0, 0, 0, 0};
-		ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], nx, ny, L, p1, mat_of<MM>(b, ph, c1), Rb, Lb, Cb, parb);
-		for (int q = 0; q < 4; q++) Rb[q] = Cb[q];
+		double dL[4], dC[4], dpar[4];
+		ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], nx, ny, L, p1, mat_of<MM>(b, ph, c1), Rb, dL, dC, dpar);
+		for (int q = 0; q < 4; q++) {
+			Lb[q] += dL[q];
+			Rb[q] = dC[q];
+			a.Ebc[(size_t)(4 * gp + q) * ne + e] += dpar[q];
+		}
 	}
 	for (int q = 0; q < 4; q++) {
 		a.Ea[(size_t)(4 * gp + q) * ne + e] = Lb[q];
 		a.Ea[(size_t)(8 + 4 * gp + q) * ne + e] = Rb[q];
 	}
-	if (c2 < 0)
-		for (int q = 0; q < 4; q++) a.Ebc[(size_t)(4 * gp + q) * ne + e] += parb[q];
 }
 
 // Cell-centric reverse of the edge pass for fluxes whose VJP is cheap (LAX):
@@ -284,7 +309,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 	Prim pi = load_prim(b, i);
 	double wi[4];
 	double si = scale * b.dtS[i];
-	for (int q = 0; q < 4; q++) wi[q] = si * a.Wn[q * n + i];
+	ld4(a.Wn, i, wi);
+	for (int q = 0; q < 4; q++) wi[q] *= si;
 	double pb[4] = {0, 0, 0, 0};
 	double gb[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 	for (int k = 0; k < 3; k++) {
@@ -299,8 +325,10 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 		if (j >= 0) {
 			pj = load_prim(b, j);
 			double sj = scale * b.dtS[j];
+			double wv[4];
+			ld4(a.Wn, j, wv);
 			for (int q = 0; q < 4; q++) {
-				double wj = sj * a.Wn[q * n + j];
+				double wj = sj * wv[q];
 				fb[q] = (side ? wi[q] - wj : wj - wi[q]) * hl;
 			}
 		} else {
@@ -319,16 +347,17 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 		} else {
 			R = ghost_of<MM>(b, ph, e, i, L, pi);
 		}
-		double Lb[4] = {0, 0, 0, 0}, Rb[4] = {0, 0, 0, 0};
+		double Lb[4], Rb[4];
 		if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb);
 		else lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb);
 		if (j < 0) {
 			int bc = -b.ec2[e] - 1;
-			double Cb[4] = {0, 0, 0, 0}, parb[4] = {0, 0, 0, 0};
-			ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], nx, ny, L, pi, mat_of<MM>(b, ph, i), Rb, Lb, Cb, parb);
+			double dL[4], dC[4], dpar[4];
+			ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], nx, ny, L, pi, mat_of<MM>(b, ph, i), Rb, dL, dC, dpar);
 			for (int q = 0; q < 4; q++) {
-				pb[q] += Cb[q];
-				a.Ebc[(size_t)(4 * gp + q) * ne + e] += parb[q];
+				Lb[q] += dL[q];
+				pb[q] += dC[q];
+				a.Ebc[(size_t)(4 * gp + q) * ne + e] += dpar[q];
 			}
 		}
 		double dx = side ? d2x : d1x, dy = side ? d2y : d1y;
@@ -342,8 +371,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 	for (int q = 0; q < 4; q++) pb[q] += __shfl_xor(pb[q], 1);
 	for (int q = 0; q < 8; q++) gb[q] += __shfl_xor(gb[q], 1);
 	if (gp == 0) {
-		for (int q = 0; q < 4; q++) a.Pb[q * n + i] = pb[q];
-		for (int q = 0; q < 8; q++) a.Gb[q * n + i] = gb[q];
+		sta4(a.Pb, b.nc, i, pb);
+		sta8(a.Gb, b.nc, i, gb);
 	}
 }
 
@@ -373,15 +402,15 @@ __global__ __launch_bounds__(BLOCK) void k_adj_cell(Bufs b, AdjBufs a)
 			for (int gp = 0; gp < 2; gp++)
 				for (int q = 0; q < 4; q++) pb[q] += a.Ea[(size_t)(8 + 4 * gp + q) * ne + e];
 	}
-	for (int q = 0; q < 4; q++) a.Pb[q * n + i] = pb[q];
-	for (int q = 0; q < 8; q++) a.Gb[q * n + i] = gb[q];
+	sta4(a.Pb, b.nc, i, pb);
+	sta8(a.Gb, b.nc, i, gb);
 }
 
 // Reverse of the gradient operator and of the primitive conversion, then the
-// RK bookkeeping: Ws = b W + (dP/dU)^T Pb, and acc0 per mode
+// RK bookkeeping: Ws = b W + (dP/dU)^T Pb, and acc0 per MODE
 // (0: Acc = a W, 1: Acc += a W, 2: Ws += Acc, the step's first stage).
-template <bool MM, bool LSQ>
-__global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, double wa, double wb, int mode)
+template <bool MM, bool LSQ, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, double wa, double wb)
 {
 	int i = xcd_block() * BLOCK + threadIdx.x;
 	if (i >= b.nc) return;
@@ -389,9 +418,9 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 	Prim pc = load_prim(b, i);
 	MatV m = mat_of<MM>(b, ph, i);
 	double pb[4];
-	for (int q = 0; q < 4; q++) pb[q] = a.Pb[q * n + i];
+	lda4(a.Pb, b.nc, i, pb);
 	double gb[8];
-	for (int q = 0; q < 8; q++) gb[q] = a.Gb[q * n + i];
+	lda8(a.Gb, b.nc, i, gb);
 	double si = b.S[i];
 	for (int k = 0; k < 3; k++) {
 		int f = b.cface[k * n + i];
@@ -416,7 +445,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 			// the neighbour's gradient depends on this cell's value
 			int s = a.cslot[k * n + i];
 			double gj[8];
-			for (int q = 0; q < 8; q++) gj[q] = a.Gb[q * n + j];
+			lda8(a.Gb, b.nc, j, gj);
 			if (LSQ) {
 				double wx = b.lsqw[(size_t)(2 * s) * n + j], wy = b.lsqw[(size_t)(2 * s + 1) * n + j];
 				for (int q = 0; q < 4; q++) pb[q] += wx * gj[2 * q] + wy * gj[2 * q + 1];
@@ -429,11 +458,11 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 		} else {
 			// ghost of the cell-centre state across a boundary edge
 			int bc = -b.ec2[e] - 1;
-			double Lb[4] = {0, 0, 0, 0}, Cb[4] = {0, 0, 0, 0}, parb[4] = {0, 0, 0, 0};
-			ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], b.enx[e], b.eny[e], pc, pc, m, nbb, Lb, Cb, parb);
+			double dL[4], dC[4], dpar[4];
+			ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], b.enx[e], b.eny[e], pc, pc, m, nbb, dL, dC, dpar);
 			for (int q = 0; q < 4; q++) {
-				pb[q] += Lb[q] + Cb[q];
-				a.Ebc[(size_t)q * ne + e] += parb[q];
+				pb[q] += dL[q] + dC[q];
+				a.Ebc[(size_t)q * ne + e] += dpar[q];
 			}
 		}
 	}
@@ -444,14 +473,24 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 	ub[1] = pb[2] / r - pb[1] * g1 * u;
 	ub[2] = pb[3] / r - pb[1] * g1 * v;
 	ub[3] = pb[1] * g1;
-	for (int q = 0; q < 4; q++) {
-		double w = a.Wn[q * n + i];
-		double ws = wb * w + ub[q];
-		if (mode == 0) a.Acc[q * n + i] = wa * w;
-		else if (mode == 1) a.Acc[q * n + i] += wa * w;
-		else ws += a.Acc[q * n + i];
-		a.Ws[q * n + i] = ws;
+	double w[4], ws[4];
+	ld4(a.Wn, i, w);
+	for (int q = 0; q < 4; q++) ws[q] = wb * w[q] + ub[q];
+	if (MODE == 0) {
+		double acc[4];
+		for (int q = 0; q < 4; q++) acc[q] = wa * w[q];
+		st4(a.Acc, i, acc);
+	} else if (MODE == 1) {
+		double acc[4];
+		ld4(a.Acc, i, acc);
+		for (int q = 0; q < 4; q++) acc[q] += wa * w[q];
+		st4(a.Acc, i, acc);
+	} else {
+		double acc[4];
+		ld4(a.Acc, i, acc);
+		for (int q = 0; q < 4; q++) ws[q] += acc[q];
 	}
+	st4(a.Ws, i, ws);
 }
 
 // boundary-parameter adjoints: one workgroup sums the per-edge entries of
@@ -493,15 +532,12 @@ __global__ __launch_bounds__(BLOCK) void k_seed(Bufs b, Phys ph, AdjBufs a, cons
 {
 	int k = blockIdx.x * BLOCK + threadIdx.x;
 	if (k >= K) return;
-	size_t n = b.nc;
 	int i = cells[k];
 	Prim p = load_prim(b, i);
 	MatV m = mat_of<MM>(b, ph, i);
 	double g1 = m.gam - 1.0, w = wsum[k];
-	a.Wn[i] = w * g1 * 0.5 * (p.u * p.u + p.v * p.v);
-	a.Wn[n + i] = -w * g1 * p.u;
-	a.Wn[2 * n + i] = -w * g1 * p.v;
-	a.Wn[3 * n + i] = w * g1;
+	double v[4] = {w * g1 * 0.5 * (p.u * p.u + p.v * p.v), -w * g1 * p.u, -w * g1 * p.v, w * g1};
+	st4(a.Wn, i, v);
 	pout[k] = p.p;
 }
 
@@ -632,6 +668,15 @@ void record_steps(cfd2d_ctx* c, AdjWork* A, int nsteps)
 	c->b.G = G0;
 }
 
+template <bool MM, bool LSQ>
+void launch_adj_grad(cfd2d_ctx* c, AdjWork* A, double wa, double wb, int mode)
+{
+	dim3 g(grid_of(c->nc)), blk(BLOCK);
+	if (mode == 0) hipLaunchKernelGGL((k_adj_grad<MM, LSQ, 0>), g, blk, 0, c->st, c->b, c->ph, A->a, wa, wb);
+	else if (mode == 1) hipLaunchKernelGGL((k_adj_grad<MM, LSQ, 1>), g, blk, 0, c->st, c->b, c->ph, A->a, wa, wb);
+	else hipLaunchKernelGGL((k_adj_grad<MM, LSQ, 2>), g, blk, 0, c->st, c->b, c->ph, A->a, wa, wb);
+}
+
 template <bool MM>
 void launch_adj_stage(cfd2d_ctx* c, AdjWork* A, double wa, double wb, int mode)
 {
@@ -645,12 +690,8 @@ void launch_adj_stage(cfd2d_ctx* c, AdjWork* A, double wa, double wb, int mode)
 		hipLaunchKernelGGL((k_adj_face<MM, CFD2D_FLUX_LAX>), dim3(grid_of(2 * c->nc)), dim3(BLOCK), 0, c->st, c->b,
 		                   c->ph, A->a, wb);
 	}
-	if (c->p.gradient == CFD2D_GRAD_LSQ)
-		hipLaunchKernelGGL((k_adj_grad<MM, true>), dim3(grid_of(c->nc)), dim3(BLOCK), 0, c->st, c->b, c->ph, A->a, wa,
-		                   wb, mode);
-	else
-		hipLaunchKernelGGL((k_adj_grad<MM, false>), dim3(grid_of(c->nc)), dim3(BLOCK), 0, c->st, c->b, c->ph, A->a, wa,
-		                   wb, mode);
+	if (c->p.gradient == CFD2D_GRAD_LSQ) launch_adj_grad<MM, true>(c, A, wa, wb, mode);
+	else launch_adj_grad<MM, false>(c, A, wa, wb, mode);
 }
 
 // steps per segment when the caller leaves it to the library: the largest
@@ -771,7 +812,8 @@ int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, do
 	}
 	if (c->nbc > 0 && a.nbedges > 0)
 		hipLaunchKernelGGL(k_bc_reduce, dim3(1), dim3(BLOCK), 0, c->st, c->b, a, c->nbc);
-	if (dJdU0) HIPCHK(hipMemcpyAsync(dJdU0, a.Wn, bytes, hipMemcpyDeviceToHost, c->st));
+	std::vector<double> rec(dJdU0 ? n4 : 0);
+	if (dJdU0) HIPCHK(hipMemcpyAsync(rec.data(), a.Wn, bytes, hipMemcpyDeviceToHost, c->st));
 	if (dJdbc && c->nbc > 0) {
 		if (a.nbedges > 0)
 			HIPCHK(hipMemcpyAsync(dJdbc, a.bcbar, 4 * (size_t)c->nbc * sizeof(double), hipMemcpyDeviceToHost, c->st));
@@ -782,6 +824,9 @@ int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, do
 	HIPCHK(hipMemcpyAsync(c->b.U, A->Ufin, bytes, hipMemcpyDeviceToDevice, c->st));
 	launch_prim<MM>(c);
 	HIPCHK(hipStreamSynchronize(c->st));
+	if (dJdU0)  // per-cell records to the API's [4][nc]
+		for (int i = 0; i < nc; i++)
+			for (int q = 0; q < 4; q++) dJdU0[(size_t)q * nc + i] = rec[4 * (size_t)i + q];
 	return check_launch("adjoint reverse sweep");
 }
 

@@ -2,10 +2,13 @@
 // and the context half of the C ABI.
 //
 // Data layout (structure of arrays, FP64, cells in the reference's order):
-//   cells  U[4][nc] conserved, U0[4][nc] step start, P[4][nc] primitives
-//          (r, p, u, v) of U (Po: where the update writes them, = P except
-//          while the adjoint records stages), G[8][nc] gradients,
-//          dtS[nc] = tau_i / S_i,
+//   cells  U[4][nc] conserved, U0[4][nc] step start (structure of arrays:
+//          streamed), P[nc][4] primitives (r, p, u, v) of U (Po: where the
+//          update writes them, = P except while the adjoint records stages)
+//          and G[nc][8] gradients (rx ry px py ux uy vx vy) as one 32 B / 64 B
+//          record per cell, because they are gathered by neighbours: one
+//          record is two or four 16-byte loads instead of four or eight
+//          8-byte gathers; dtS[nc] = tau_i / S_i,
 //          S[nc], faces cface[3][nc] = edge*2 + side in ascending edge order,
 //          cnb[3][nc] neighbour across each face (-1 on a boundary),
 //          lsqw[6][nc] least-squares weights, flags[nc]
@@ -82,18 +85,70 @@ __device__ __forceinline__ MatV mat_of(const Bufs& b, const Phys& ph, int i)
 	return MatV{t[0], t[1], t[2], t[3], t[4]};
 }
 
-__device__ __forceinline__ Prim load_prim(const Bufs& b, int i)
+// 32-byte records: four doubles per cell (buffers are 256-byte aligned, so
+// each record is 32-byte aligned and loads/stores as 16-byte pieces)
+__device__ __forceinline__ void ld4(const double* base, size_t i, double* v)
 {
-	size_t n = b.nc;
-	return Prim{b.P[i], b.P[n + i], b.P[2 * n + i], b.P[3 * n + i]};
+	const double* p = static_cast<const double*>(__builtin_assume_aligned(base + 4 * i, 32));
+	v[0] = p[0];
+	v[1] = p[1];
+	v[2] = p[2];
+	v[3] = p[3];
 }
 
-__device__ __forceinline__ void store_prim_to(double* P, size_t n, int i, const Prim& q)
+__device__ __forceinline__ void st4(double* base, size_t i, const double* v)
 {
-	P[i] = q.r;
-	P[n + i] = q.p;
-	P[2 * n + i] = q.u;
-	P[3 * n + i] = q.v;
+	double* p = static_cast<double*>(__builtin_assume_aligned(base + 4 * i, 32));
+	p[0] = v[0];
+	p[1] = v[1];
+	p[2] = v[2];
+	p[3] = v[3];
+}
+
+__device__ __forceinline__ void ld8(const double* base, size_t i, double* v)
+{
+	ld4(base, 2 * i, v);
+	ld4(base, 2 * i + 1, v + 4);
+}
+
+__device__ __forceinline__ void st8(double* base, size_t i, const double* v)
+{
+	st4(base, 2 * i, v);
+	st4(base, 2 * i + 1, v + 4);
+}
+
+// the same quantities as separate arrays ([4][n] / [8][n])
+__device__ __forceinline__ void lda4(const double* base, size_t n, size_t i, double* v)
+{
+	for (int q = 0; q < 4; q++) v[q] = base[q * n + i];
+}
+
+__device__ __forceinline__ void sta4(double* base, size_t n, size_t i, const double* v)
+{
+	for (int q = 0; q < 4; q++) base[q * n + i] = v[q];
+}
+
+__device__ __forceinline__ void lda8(const double* base, size_t n, size_t i, double* v)
+{
+	for (int q = 0; q < 8; q++) v[q] = base[q * n + i];
+}
+
+__device__ __forceinline__ void sta8(double* base, size_t n, size_t i, const double* v)
+{
+	for (int q = 0; q < 8; q++) base[q * n + i] = v[q];
+}
+
+__device__ __forceinline__ Prim load_prim(const Bufs& b, int i)
+{
+	double v[4];
+	ld4(b.P, i, v);
+	return Prim{v[0], v[1], v[2], v[3]};
+}
+
+__device__ __forceinline__ void store_prim_to(double* P, size_t, int i, const Prim& q)
+{
+	double v[4] = {q.r, q.p, q.u, q.v};
+	st4(P, i, v);
 }
 
 __device__ __forceinline__ void store_prim(const Bufs& b, int i, const Prim& q) { store_prim_to(b.P, b.nc, i, q); }
@@ -212,25 +267,27 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(Bufs b, Phys ph)
 		for (int q = 0; q < 8; q++) g[q] = g[q] / si;
 	}
 	if (LIM) minmod_limit(b, i, face, pi, nb, g);
-	for (int q = 0; q < 8; q++) b.G[q * n + i] = g[q];
+	st8(b.G, i, g);
 }
 
 __device__ __forceinline__ void recon(Prim& q, const Bufs& b, int c, double dx, double dy)
 {
-	size_t n = b.nc;
-	q.r += b.G[c] * dx + b.G[n + c] * dy;
-	q.p += b.G[2 * n + c] * dx + b.G[3 * n + c] * dy;
-	q.u += b.G[4 * n + c] * dx + b.G[5 * n + c] * dy;
-	q.v += b.G[6 * n + c] * dx + b.G[7 * n + c] * dy;
+	double g[8];
+	ld8(b.G, c, g);
+	q.r += g[0] * dx + g[1] * dy;
+	q.p += g[2] * dx + g[3] * dy;
+	q.u += g[4] * dx + g[5] * dy;
+	q.v += g[6] * dx + g[7] * dy;
 }
 
 __device__ __forceinline__ void grad_T(const Bufs& b, int c, const Prim& q, const MatV& m, double& tx, double& ty)
 {
-	size_t n = b.nc;
+	double g[4];
+	ld4(b.G, 2 * (size_t)c, g);  // rx ry px py
 	double Rg = GAS_R / m.M;
 	double pr = q.p / q.r;
-	tx = (b.G[2 * n + c] - pr * b.G[c]) / (q.r * Rg);
-	ty = (b.G[3 * n + c] - pr * b.G[n + c]) / (q.r * Rg);
+	tx = (g[2] - pr * g[0]) / (q.r * Rg);
+	ty = (g[3] - pr * g[1]) / (q.r * Rg);
 }
 
 // Edge flux pass, FVM_TVD::run edge loop + reconstruct + calcFlux
@@ -273,18 +330,20 @@ __global__ __launch_bounds__(BLOCK) void k_edge_flux(Bufs b, Phys ph)
 		// face gradient = mean of the two cell gradients (the cell's own on a
 		// boundary); face velocity = mean of the reconstructed states
 		MatV m = mat_of<MM>(b, ph, c1);
-		size_t n = b.nc;
-		double ux = b.G[4 * n + c1], uy = b.G[5 * n + c1], vx = b.G[6 * n + c1], vy = b.G[7 * n + c1];
+		double gv[4];
+		ld4(b.G, 2 * (size_t)c1 + 1, gv);  // ux uy vx vy
+		double ux = gv[0], uy = gv[1], vx = gv[2], vy = gv[3];
 		double tx, ty;
 		grad_T(b, c1, p1, m, tx, ty);
 		if (c2 >= 0) {
 			MatV m2 = mat_of<MM>(b, ph, c2);
 			double tx2, ty2;
 			grad_T(b, c2, p2, m2, tx2, ty2);
-			ux = 0.5 * (ux + b.G[4 * n + c2]);
-			uy = 0.5 * (uy + b.G[5 * n + c2]);
-			vx = 0.5 * (vx + b.G[6 * n + c2]);
-			vy = 0.5 * (vy + b.G[7 * n + c2]);
+			ld4(b.G, 2 * (size_t)c2 + 1, gv);
+			ux = 0.5 * (ux + gv[0]);
+			uy = 0.5 * (uy + gv[1]);
+			vx = 0.5 * (vx + gv[2]);
+			vy = 0.5 * (vy + gv[3]);
 			tx = 0.5 * (tx + tx2);
 			ty = 0.5 * (ty + ty2);
 		}
@@ -1053,7 +1112,11 @@ int cfd2d_gradients(cfd2d_ctx* c, double* g)
 	do_gradient(c);
 	int rc = check_launch("k_gradient");
 	if (rc) return rc;
-	HIPCHK(hipMemcpyAsync(g, c->b.G, 8 * (size_t)c->nc * sizeof(double), hipMemcpyDeviceToHost, c->st));
+	std::vector<double> rec(8 * (size_t)c->nc);
+	HIPCHK(hipMemcpyAsync(rec.data(), c->b.G, rec.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	for (int i = 0; i < c->nc; i++)
+		for (int q = 0; q < 8; q++) g[(size_t)q * c->nc + i] = rec[8 * (size_t)i + q];
 	HIPCHK(hipStreamSynchronize(c->st));
 	return CFD2D_OK;
 }
@@ -1135,13 +1198,13 @@ int cfd2d_status(cfd2d_ctx* c, long long* hits, int* nonfinite)
 int cfd2d_wall_force(cfd2d_ctx* c, const int* mask, double* fx, double* fy)
 {
 	if (!c || !mask) return fail(CFD2D_ERR_ARG, "null");
-	std::vector<double> pr(c->nc);
+	std::vector<double> P(4 * (size_t)c->nc);
 	HIPCHK(hipStreamSynchronize(c->st));
-	HIPCHK(hipMemcpy(pr.data(), c->b.P + c->nc, pr.size() * sizeof(double), hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(P.data(), c->b.P, P.size() * sizeof(double), hipMemcpyDeviceToHost));
 	double X = 0.0, Y = 0.0;
 	for (int e = 0; e < c->ne; e++) {
 		if (!mask[e]) continue;
-		double p = pr[c->hc1[e]];
+		double p = P[4 * (size_t)c->hc1[e] + 1];
 		X += p * c->hel[e] * c->hnx[e];
 		Y += p * c->hel[e] * c->hny[e];
 	}
