@@ -412,85 +412,86 @@ __global__ __launch_bounds__(BLOCK) void k_adj_cell(Bufs b, AdjBufs a)
 template <bool MM, bool LSQ, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, double wa, double wb)
 {
-	int i = xcd_block() * BLOCK + threadIdx.x;
-	if (i >= b.nc) return;
-	size_t n = b.nc, ne = b.ne;
-	Prim pc = load_prim(b, i);
-	MatV m = mat_of<MM>(b, ph, i);
-	double pb[4];
-	lda4(a.Pb, b.nc, i, pb);
-	double gb[8];
-	lda8(a.Gb, b.nc, i, gb);
-	double si = b.S[i];
-	for (int k = 0; k < 3; k++) {
-		int f = b.cface[k * n + i];
-		int e = f >> 1, side = f & 1;
-		int j = b.cnb[k * n + i];
-		double nbb[4];  // adjoint of the neighbour value this face contributed
-		if (LSQ) {
-			double wx = b.lsqw[(size_t)(2 * k) * n + i], wy = b.lsqw[(size_t)(2 * k + 1) * n + i];
-			for (int q = 0; q < 4; q++) {
-				nbb[q] = wx * gb[2 * q] + wy * gb[2 * q + 1];
-				pb[q] -= nbb[q];
-			}
-		} else {
-			double l = b.el[e], nx = b.enx[e], ny = b.eny[e];
-			double sg = side ? -1.0 : 1.0;
-			for (int q = 0; q < 4; q++) {
-				nbb[q] = sg * 0.5 * l * (nx * gb[2 * q] + ny * gb[2 * q + 1]) / si;
-				pb[q] += nbb[q];
-			}
-		}
-		if (j >= 0) {
-			// the neighbour's gradient depends on this cell's value
-			int s = a.cslot[k * n + i];
-			double gj[8];
-			lda8(a.Gb, b.nc, j, gj);
+	__shared__ double lds_w[BLOCK * 5], lds_a[BLOCK * 5];
+	int c0 = xcd_block() * BLOCK;
+	int i = c0 + threadIdx.x;
+	double ws[4] = {0, 0, 0, 0}, acc[4] = {0, 0, 0, 0};
+	if (i < b.nc) {
+		size_t n = b.nc, ne = b.ne;
+		Prim pc = load_prim(b, i);
+		MatV m = mat_of<MM>(b, ph, i);
+		double pb[4];
+		lda4(a.Pb, b.nc, i, pb);
+		double gb[8];
+		lda8(a.Gb, b.nc, i, gb);
+		double si = b.S[i];
+		for (int k = 0; k < 3; k++) {
+			int f = b.cface[k * n + i];
+			int e = f >> 1, side = f & 1;
+			int j = b.cnb[k * n + i];
+			double nbb[4];  // adjoint of the neighbour value this face contributed
 			if (LSQ) {
-				double wx = b.lsqw[(size_t)(2 * s) * n + j], wy = b.lsqw[(size_t)(2 * s + 1) * n + j];
-				for (int q = 0; q < 4; q++) pb[q] += wx * gj[2 * q] + wy * gj[2 * q + 1];
+				double wx = b.lsqw[(size_t)(2 * k) * n + i], wy = b.lsqw[(size_t)(2 * k + 1) * n + i];
+				for (int q = 0; q < 4; q++) {
+					nbb[q] = wx * gb[2 * q] + wy * gb[2 * q + 1];
+					pb[q] -= nbb[q];
+				}
 			} else {
 				double l = b.el[e], nx = b.enx[e], ny = b.eny[e];
-				double sg = side ? 1.0 : -1.0;  // the neighbour sits on the other side
-				double sj = b.S[j];
-				for (int q = 0; q < 4; q++) pb[q] += sg * 0.5 * l * (nx * gj[2 * q] + ny * gj[2 * q + 1]) / sj;
+				double sg = side ? -1.0 : 1.0;
+				for (int q = 0; q < 4; q++) {
+					nbb[q] = sg * 0.5 * l * (nx * gb[2 * q] + ny * gb[2 * q + 1]) / si;
+					pb[q] += nbb[q];
+				}
 			}
-		} else {
-			// ghost of the cell-centre state across a boundary edge
-			int bc = -b.ec2[e] - 1;
-			double dL[4], dC[4], dpar[4];
-			ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], b.enx[e], b.eny[e], pc, pc, m, nbb, dL, dC, dpar);
-			for (int q = 0; q < 4; q++) {
-				pb[q] += dL[q] + dC[q];
-				a.Ebc[(size_t)q * ne + e] += dpar[q];
+			if (j >= 0) {
+				// the neighbour's gradient depends on this cell's value
+				int s = a.cslot[k * n + i];
+				double gj[8];
+				lda8(a.Gb, b.nc, j, gj);
+				if (LSQ) {
+					double wx = b.lsqw[(size_t)(2 * s) * n + j], wy = b.lsqw[(size_t)(2 * s + 1) * n + j];
+					for (int q = 0; q < 4; q++) pb[q] += wx * gj[2 * q] + wy * gj[2 * q + 1];
+				} else {
+					double l = b.el[e], nx = b.enx[e], ny = b.eny[e];
+					double sg = side ? 1.0 : -1.0;  // the neighbour sits on the other side
+					double sj = b.S[j];
+					for (int q = 0; q < 4; q++) pb[q] += sg * 0.5 * l * (nx * gj[2 * q] + ny * gj[2 * q + 1]) / sj;
+				}
+			} else {
+				// ghost of the cell-centre state across a boundary edge
+				int bc = -b.ec2[e] - 1;
+				double dL[4], dC[4], dpar[4];
+				ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], b.enx[e], b.eny[e], pc, pc, m, nbb, dL, dC, dpar);
+				for (int q = 0; q < 4; q++) {
+					pb[q] += dL[q] + dC[q];
+					a.Ebc[(size_t)q * ne + e] += dpar[q];
+				}
 			}
 		}
+		// P = (r, p, u, v)(U): the transposed Jacobian
+		double r = pc.r, u = pc.u, v = pc.v, g1 = m.gam - 1.0;
+		double ub[4];
+		ub[0] = pb[0] - pb[2] * u / r - pb[3] * v / r + pb[1] * g1 * 0.5 * (u * u + v * v);
+		ub[1] = pb[2] / r - pb[1] * g1 * u;
+		ub[2] = pb[3] / r - pb[1] * g1 * v;
+		ub[3] = pb[1] * g1;
+		double w[4];
+		ld4(a.Wn, i, w);
+		for (int q = 0; q < 4; q++) ws[q] = wb * w[q] + ub[q];
+		if (MODE == 0) {
+			for (int q = 0; q < 4; q++) acc[q] = wa * w[q];
+		} else if (MODE == 1) {
+			ld4(a.Acc, i, acc);
+			for (int q = 0; q < 4; q++) acc[q] += wa * w[q];
+		} else {
+			ld4(a.Acc, i, acc);
+			for (int q = 0; q < 4; q++) ws[q] += acc[q];
+		}
 	}
-	// P = (r, p, u, v)(U): the transposed Jacobian
-	double r = pc.r, u = pc.u, v = pc.v, g1 = m.gam - 1.0;
-	double ub[4];
-	ub[0] = pb[0] - pb[2] * u / r - pb[3] * v / r + pb[1] * g1 * 0.5 * (u * u + v * v);
-	ub[1] = pb[2] / r - pb[1] * g1 * u;
-	ub[2] = pb[3] / r - pb[1] * g1 * v;
-	ub[3] = pb[1] * g1;
-	double w[4], ws[4];
-	ld4(a.Wn, i, w);
-	for (int q = 0; q < 4; q++) ws[q] = wb * w[q] + ub[q];
-	if (MODE == 0) {
-		double acc[4];
-		for (int q = 0; q < 4; q++) acc[q] = wa * w[q];
-		st4(a.Acc, i, acc);
-	} else if (MODE == 1) {
-		double acc[4];
-		ld4(a.Acc, i, acc);
-		for (int q = 0; q < 4; q++) acc[q] += wa * w[q];
-		st4(a.Acc, i, acc);
-	} else {
-		double acc[4];
-		ld4(a.Acc, i, acc);
-		for (int q = 0; q < 4; q++) ws[q] += acc[q];
-	}
-	st4(a.Ws, i, ws);
+	int nv = min(BLOCK, b.nc - c0);
+	store_records<4>(a.Ws, c0, nv, ws, lds_w);
+	if (MODE != 2) store_records<4>(a.Acc, c0, nv, acc, lds_a);
 }
 
 // boundary-parameter adjoints: one workgroup sums the per-edge entries of

@@ -111,10 +111,25 @@ __device__ __forceinline__ void ld8(const double* base, size_t i, double* v)
 	ld4(base, 2 * i + 1, v + 4);
 }
 
-__device__ __forceinline__ void st8(double* base, size_t i, const double* v)
+// Stores the W-double records of the block's cells [c0, c0 + nvalid) held in
+// registers (lane t: cell c0 + t) through LDS, so that each store instruction
+// covers contiguous memory instead of one 16-byte piece every W * 8 bytes.
+// Every thread of the block calls it; lds holds BLOCK * (W + 1) doubles.
+template <int W>
+__device__ __forceinline__ void store_records(double* rec, int c0, int nvalid, const double* v, double* lds)
 {
-	st4(base, 2 * i, v);
-	st4(base, 2 * i + 1, v + 4);
+	constexpr int PAD = W + 1;  // odd row length spreads the lanes over the LDS banks
+	int t = threadIdx.x;
+	for (int q = 0; q < W; q++) lds[t * PAD + q] = v[q];
+	__syncthreads();
+	double* dst = rec + (size_t)c0 * W;
+	int nd = nvalid * W;
+	for (int d = 2 * t; d < nd; d += 2 * BLOCK) {
+		int c = d / W, q = d % W;  // W even: d and d + 1 lie in one record
+		double* o = static_cast<double*>(__builtin_assume_aligned(dst + d, 16));
+		o[0] = lds[c * PAD + q];
+		o[1] = lds[c * PAD + q + 1];
+	}
 }
 
 // the same quantities as separate arrays ([4][n] / [8][n])
@@ -170,11 +185,20 @@ __device__ __forceinline__ Prim ghost_of(const Bufs& b, const Phys& ph, int e, i
 template <bool MM>
 __global__ __launch_bounds__(BLOCK) void k_prim(Bufs b, Phys ph)
 {
-	int i = xcd_block() * BLOCK + threadIdx.x;
-	if (i >= b.nc) return;
-	size_t n = b.nc;
-	MatV m = mat_of<MM>(b, ph, i);
-	store_prim(b, i, cons_to_prim(b.U[i], b.U[n + i], b.U[2 * n + i], b.U[3 * n + i], m));
+	__shared__ double lds[BLOCK * 5];
+	int c0 = xcd_block() * BLOCK;
+	int i = c0 + threadIdx.x;
+	double v[4] = {0.0, 0.0, 0.0, 0.0};
+	if (i < b.nc) {
+		size_t n = b.nc;
+		MatV m = mat_of<MM>(b, ph, i);
+		Prim p = cons_to_prim(b.U[i], b.U[n + i], b.U[2 * n + i], b.U[3 * n + i], m);
+		v[0] = p.r;
+		v[1] = p.p;
+		v[2] = p.u;
+		v[3] = p.v;
+	}
+	store_records<4>(b.P, c0, min(BLOCK, b.nc - c0), v, lds);
 }
 
 // Barth-Jespersen scaling with minmod(1, r) at the cell's six Gauss points
@@ -224,50 +248,53 @@ __device__ __forceinline__ void minmod_limit(const Bufs& b, int i, const int* fa
 template <bool MM, bool LSQ, bool LIM>
 __global__ __launch_bounds__(BLOCK) void k_gradient(Bufs b, Phys ph)
 {
-	int i = xcd_block() * BLOCK + threadIdx.x;
-	if (i >= b.nc) return;
-	size_t n = b.nc;
-	Prim pi = load_prim(b, i);
-	int face[3];
-	Prim nb[3];
-	for (int k = 0; k < 3; k++) {
-		face[k] = b.cface[k * n + i];
-		int j = b.cnb[k * n + i];
-		if (j >= 0) nb[k] = load_prim(b, j);
-		else nb[k] = ghost_of<MM>(b, ph, face[k] >> 1, i, pi, pi);
-	}
+	__shared__ double lds[BLOCK * 9];
+	int c0 = xcd_block() * BLOCK;
+	int i = c0 + threadIdx.x;
 	double g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-	if (LSQ) {
+	if (i < b.nc) {
+		size_t n = b.nc;
+		Prim pi = load_prim(b, i);
+		int face[3];
+		Prim nb[3];
 		for (int k = 0; k < 3; k++) {
-			double wx = b.lsqw[(size_t)(2 * k) * n + i], wy = b.lsqw[(size_t)(2 * k + 1) * n + i];
-			double dphi[4] = {nb[k].r - pi.r, nb[k].p - pi.p, nb[k].u - pi.u, nb[k].v - pi.v};
-			for (int q = 0; q < 4; q++) {
-				g[2 * q] += wx * dphi[q];
-				g[2 * q + 1] += wy * dphi[q];
-			}
+			face[k] = b.cface[k * n + i];
+			int j = b.cnb[k * n + i];
+			if (j >= 0) nb[k] = load_prim(b, j);
+			else nb[k] = ghost_of<MM>(b, ph, face[k] >> 1, i, pi, pi);
 		}
-	} else {
-		for (int k = 0; k < 3; k++) {
-			int e = face[k] >> 1, side = face[k] & 1;
-			double nx = b.enx[e], ny = b.eny[e], l = b.el[e];
-			double s[4] = {pi.r + nb[k].r, pi.p + nb[k].p, pi.u + nb[k].u, pi.v + nb[k].v};
-			for (int q = 0; q < 4; q++) {
-				double tx = s[q] / 2 * nx * l;
-				double ty = s[q] / 2 * ny * l;
-				if (side == 0) {
-					g[2 * q] += tx;
-					g[2 * q + 1] += ty;
-				} else {
-					g[2 * q] -= tx;
-					g[2 * q + 1] -= ty;
+		if (LSQ) {
+			for (int k = 0; k < 3; k++) {
+				double wx = b.lsqw[(size_t)(2 * k) * n + i], wy = b.lsqw[(size_t)(2 * k + 1) * n + i];
+				double dphi[4] = {nb[k].r - pi.r, nb[k].p - pi.p, nb[k].u - pi.u, nb[k].v - pi.v};
+				for (int q = 0; q < 4; q++) {
+					g[2 * q] += wx * dphi[q];
+					g[2 * q + 1] += wy * dphi[q];
 				}
 			}
+		} else {
+			for (int k = 0; k < 3; k++) {
+				int e = face[k] >> 1, side = face[k] & 1;
+				double nx = b.enx[e], ny = b.eny[e], l = b.el[e];
+				double s[4] = {pi.r + nb[k].r, pi.p + nb[k].p, pi.u + nb[k].u, pi.v + nb[k].v};
+				for (int q = 0; q < 4; q++) {
+					double tx = s[q] / 2 * nx * l;
+					double ty = s[q] / 2 * ny * l;
+					if (side == 0) {
+						g[2 * q] += tx;
+						g[2 * q + 1] += ty;
+					} else {
+						g[2 * q] -= tx;
+						g[2 * q + 1] -= ty;
+					}
+				}
+			}
+			double si = b.S[i];
+			for (int q = 0; q < 8; q++) g[q] = g[q] / si;
 		}
-		double si = b.S[i];
-		for (int q = 0; q < 8; q++) g[q] = g[q] / si;
+		if (LIM) minmod_limit(b, i, face, pi, nb, g);
 	}
-	if (LIM) minmod_limit(b, i, face, pi, nb, g);
-	st8(b.G, i, g);
+	store_records<8>(b.G, c0, min(BLOCK, b.nc - c0), g, lds);
 }
 
 __device__ __forceinline__ void recon(Prim& q, const Bufs& b, int c, double dx, double dy)
@@ -399,9 +426,11 @@ enum { UPD_FIRST = 0, UPD_RK2_LAST = 2, UPD_RK3_MID = 3, UPD_RK3_LAST = 4 };
 template <bool MM, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_update(Bufs b, Phys ph)
 {
+	__shared__ double lds[BLOCK * 5];
 	int lb = xcd_block();
 	int i = lb * BLOCK + threadIdx.x;
 	int flagged = 0;
+	double pv[4] = {0.0, 0.0, 0.0, 0.0};  // primitives written for this cell
 	if (i < b.nc) {
 		size_t n = b.nc;
 		unsigned fl = b.flags[i];
@@ -431,7 +460,10 @@ __global__ __launch_bounds__(BLOCK) void k_update(Bufs b, Phys ph)
 			if (!finite) atomicAdd(&b.err[1], 1ull);
 			MatV m = mat_of<MM>(b, ph, i);
 			Prim p = cons_to_prim(u[0], u[1], u[2], u[3], m);
-			store_prim_to(b.Po, n, i, p);
+			pv[0] = p.r;
+			pv[1] = p.p;
+			pv[2] = p.u;
+			pv[3] = p.v;
 			if (MODE == UPD_RK2_LAST || MODE == UPD_RK3_LAST) {
 				bool out = p.r < ph.ro_min || p.r > ph.ro_max || p.p < ph.p_min || p.p > ph.p_max ||
 				           fabs(p.u) > ph.u_max || fabs(p.v) > ph.u_max;
@@ -440,11 +472,12 @@ __global__ __launch_bounds__(BLOCK) void k_update(Bufs b, Phys ph)
 					b.flags[i] = fl;
 				}
 			}
-		} else if (b.Po != b.P) {
-			store_prim_to(b.Po, n, i, load_prim(b, i));  // frozen cell, primitives to the output slot
+		} else {
+			ld4(b.P, i, pv);  // frozen cell: its primitives unchanged
 		}
 		flagged = (fl & 0x2u) != 0;
 	}
+	store_records<4>(b.Po, lb * BLOCK, min(BLOCK, b.nc - lb * BLOCK), pv, lds);
 	if (MODE == UPD_RK2_LAST || MODE == UPD_RK3_LAST) {
 		int any = __syncthreads_or(flagged);
 		if (threadIdx.x == 0) {
