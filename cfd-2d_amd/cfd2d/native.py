@@ -68,6 +68,7 @@ SIGNATURES = {
     "cfd2d_calc_tau": (C.c_int, [C.c_void_p]),
     "cfd2d_get_tau": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_void_p]),
     "cfd2d_set_tau": (C.c_int, [C.c_void_p, C.c_double]),
+    "cfd2d_set_bc_par": (C.c_int, [C.c_void_p, _dp]),
     "cfd2d_gradients": (C.c_int, [C.c_void_p, _dp]),
     "cfd2d_edge_fluxes": (C.c_int, [C.c_void_p, _dp]),
     "cfd2d_residual": (C.c_int, [C.c_void_p, _dp]),
@@ -258,6 +259,13 @@ class Context:
 
     def set_tau(self, tau):
         check(lib().cfd2d_set_tau(self._h, tau))
+
+    def set_bc_par(self, par):
+        """Parameters of every non-periodic boundary object, (n_bc, 4)."""
+        par = np.ascontiguousarray(par, np.float64).reshape(-1)
+        if par.size != 4 * self.n_bc:
+            raise ValueError("need %d x 4 boundary parameters" % self.n_bc)
+        check(lib().cfd2d_set_bc_par(self._h, par if par.size else np.zeros(4)))
 
     def gradients(self):
         g = np.zeros(8 * self.nc)

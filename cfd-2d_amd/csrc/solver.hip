@@ -1128,6 +1128,15 @@ int cfd2d_get_tau(cfd2d_ctx* c, double* tau0, double* ctau)
 	return CFD2D_OK;
 }
 
+int cfd2d_set_bc_par(cfd2d_ctx* c, const double* bc_par)
+{
+	if (!c || (c->nbc > 0 && !bc_par)) return fail(CFD2D_ERR_ARG, "cfd2d_set_bc_par: bad arguments");
+	if (c->nbc == 0) return CFD2D_OK;
+	HIPCHK(hipMemcpyAsync(c->b.bcpar, bc_par, 4 * (size_t)c->nbc * sizeof(double), hipMemcpyHostToDevice, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	return CFD2D_OK;
+}
+
 int cfd2d_set_tau(cfd2d_ctx* c, double tau)
 {
 	if (!c || !(tau > 0.0)) return fail(CFD2D_ERR_ARG, "bad tau");

@@ -126,6 +126,11 @@ int  cfd2d_get_flags(cfd2d_ctx* c, unsigned* flags);
 int  cfd2d_calc_tau(cfd2d_ctx* c);
 int  cfd2d_get_tau(cfd2d_ctx* c, double* tau_case0, double* ctau /* nc, may be NULL */);
 int  cfd2d_set_tau(cfd2d_ctx* c, double tau);
+/* Replace the parameters of every boundary object (double[n_bc][4], the
+ * layout of cfd2d_ctx_create's bc_par).  Extension: the reference reads them
+ * once from the task file (CFDBoundary::create, bnd_cond.cpp:10-41); the
+ * adjoint's callers change inflow parameters between runs. */
+int  cfd2d_set_bc_par(cfd2d_ctx* c, const double* bc_par);
 /* stage diagnostics on the current state (parity checks) */
 int  cfd2d_gradients(cfd2d_ctx* c, double* g);      /* double[8][nc]: rx ry px py ux uy vx vy */
 int  cfd2d_edge_fluxes(cfd2d_ctx* c, double* f);    /* double[ne][4]: (sum of Gauss-point fluxes) * l/2 */

@@ -32,7 +32,11 @@ def pulse(flux, gradient, rk, n=12):
 
 def objective_mask(s, kind):
     if kind == "bump":
-        return (s.geometry["edge_group"] == s.case.mesh.group_index("bump")).astype(np.int32)
+        # bump and bottom wall: the bottom wall meets the inlet and the outlet,
+        # so every boundary parameter acts on J within the run
+        eg = s.geometry["edge_group"]
+        m = s.case.mesh
+        return ((eg == m.group_index("bump")) | (eg == m.group_index("wall_bottom"))).astype(np.int32)
     c1 = s.geometry["edge_int"][:, 2]
     return (c1 < s.nc // 4).astype(np.int32)  # a band of interior edges
 
@@ -66,6 +70,7 @@ def test_adjoint_matches_tangent(kind, flux, gradient, rk):
         # each inlet parameter (u, v, T, p) and the outlet pressure separately
         zero = np.zeros_like(U0)
         big = np.max(np.abs(gb))
+        assert big > 1e-8 * abs(J)
         for bc, k in [(0, 0), (0, 1), (0, 2), (0, 3), (1, 0)]:
             e = np.zeros_like(m.bcpar0)
             e[bc, k] = 1.0
