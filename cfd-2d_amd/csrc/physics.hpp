@@ -79,12 +79,18 @@ __device__ __forceinline__ Prim ghost_state(int type, const double* par, double 
 // from the acoustic guess with the P >= 1e-5 clamp, stopping when
 // |DU - F1 - F2| <= 1e-5 is seen before the update; powers as
 // exp(log(x) k).  max_iter > 0 caps the loop and reports it in *hit.
+// Divisions by loop invariants (1/RB, 1/RE, 1/PB, 1/PE, the gamma constants)
+// are formed once and multiplied; the shock branches share one reciprocal of
+// Z * PK for F and F'; the post-loop shock state uses 1/(R (U - D)) for both
+// the velocity and the energy.  These change the rounding of single
+// operations only (results agree with the reference to round-off, not bit
+// for bit); the FP64 division is the costliest operation of this solver.
 // Outputs the state sampled at x/t = 0: density RI, internal energy EI,
 // pressure PI, normal velocity UI, tangential velocity VI.
 // The gamma-dependent constants of rim_orig (src/global.cpp:236-250),
 // formed once on the host with the same expressions instead of per call.
 struct RimConst {
-	double GAM, AGAM, DGAM, GGAM, HGAM, FGAM, OGAM, QGAM, PGAM, RGAM, SGAM, TGAM, DGAM1, IAGAM, DGGAM;
+	double GAM, AGAM, DGAM, GGAM, HGAM, FGAM, OGAM, QGAM, PGAM, RGAM, SGAM, TGAM, DGAM1, IAGAM, DGGAM, ISGAM, IRGAM;
 	int gamma14;  // OGAM = 1/7 and 1/AGAM = 5/2 to rounding (GAM = 1.4, the reference's value)
 };
 
@@ -106,6 +112,8 @@ __host__ __device__ inline RimConst rim_const(double GAM)
 	k.DGAM1 = (1 + k.DGAM);
 	k.IAGAM = (1 / k.AGAM);
 	k.DGGAM = k.DGAM * k.GGAM;
+	k.ISGAM = 1.0 / k.SGAM;
+	k.IRGAM = 1.0 / k.RGAM;
 	k.gamma14 = fabs(7.0 * k.OGAM - 1.0) < 1e-14 && fabs(k.IAGAM - 2.5) < 1e-14;
 	return k;
 }
@@ -115,17 +123,28 @@ __host__ __device__ inline RimConst rim_const(double GAM)
 // precision estimate of x^(1/7) refined by two Newton steps on y^7 = x,
 // y <- y + (x / y^6 - y) / 7, which is quadratically convergent from the
 // ~1e-7 estimate to within an ulp or two (about 40 instructions instead of
-// the ~160 of the double-precision log and exp).  Other gammas, and x
-// outside [1e-30, 1e30], take exp(log(x) * OGAM).
+// the ~160 of the double-precision log and exp).  1/y^6 comes from the
+// hardware reciprocal with Newton-Raphson refinement (r <- r (2 - y^6 r),
+// once in the first step, whose error the root iteration removes, twice in
+// the second) instead of a full IEEE division.  Other gammas, and x outside
+// [1e-30, 1e30], take exp(log(x) * OGAM).
 __device__ __forceinline__ double pow_ogam(double x, const RimConst& K)
 {
 	if (K.gamma14 && x > 1e-30 && x < 1e30) {
 		float xf = (float)x;
 		double y = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf(xf) * (1.0f / 7.0f));
-		for (int it = 0; it < 2; it++) {
+		{
 			double y2 = y * y;
 			double y6 = y2 * y2 * y2;
-			y = y + (x / y6 - y) * (1.0 / 7.0);
+			double r = __builtin_amdgcn_rcp(y6);
+			r = r * (2.0 - y6 * r);
+			y = y + (x * r - y) * (1.0 / 7.0);
+			y2 = y * y;
+			y6 = y2 * y2 * y2;
+			r = __builtin_amdgcn_rcp(y6);
+			r = r * (2.0 - y6 * r);
+			r = r * (2.0 - y6 * r);
+			y = y + (x * r - y) * (1.0 / 7.0);
 		}
 		return y;
 	}
@@ -145,15 +164,17 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
                                     const RimConst& K, int max_iter, int& hit)
 {
 	const double GAM = K.GAM, AGAM = K.AGAM, DGAM = K.DGAM, HGAM = K.HGAM, FGAM = K.FGAM, OGAM = K.OGAM;
-	const double QGAM = K.QGAM, PGAM = K.PGAM, RGAM = K.RGAM, SGAM = K.SGAM, TGAM = K.TGAM;
+	const double QGAM = K.QGAM, PGAM = K.PGAM, SGAM = K.SGAM, TGAM = K.TGAM;
 	const double eps = 1.0e-5;
 
 	double US = 0.0, UF = 0.0;
 	double RF, RS, EF, ES, SBL, SFL, SSL, SEL;
-	double CB = sqrt(GAM * PB / RB);
-	double CE = sqrt(GAM * PE / RE);
-	double EB = CB * CB / SGAM;
-	double EE = CE * CE / SGAM;
+	const double IRB = 1.0 / RB, IRE = 1.0 / RE;
+	double CB = sqrt(GAM * PB * IRB);
+	double CE = sqrt(GAM * PE * IRE);
+	double EB = CB * CB * K.ISGAM;
+	double EE = CE * CE * K.ISGAM;
+	const double IPB = 1.0 / PB, IPE = 1.0 / PE;
 	double RCB = RB * CB;
 	double RCE = RE * CE;
 	double DU = UB - UE;
@@ -170,7 +191,7 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 		for (;;) {
 			if (P < eps) P = eps;
 			double F1, FS1, F2, FS2;
-			double PPB = P / PB;
+			double PPB = P * IPB;
 			if (PB > P) {
 				double ZFB = CB * pow_ogam(PPB, K);
 				F1 = DGAM * (ZFB - CB);
@@ -178,10 +199,11 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 			} else {
 				double PKB = PGAM * PPB + OGAM;
 				double ZNB = RCB * sqrt(PKB);
-				F1 = (P - PB) / ZNB;
-				FS1 = (QGAM * PPB + FGAM) / (RGAM * ZNB * PKB);
+				double IZ = 1.0 / (ZNB * PKB);
+				F1 = (P - PB) * PKB * IZ;
+				FS1 = (QGAM * PPB + FGAM) * IZ * K.IRGAM;
 			}
-			double PPE = P / PE;
+			double PPE = P * IPE;
 			if (PE > P) {
 				double ZFE = CE * pow_ogam(PPE, K);
 				F2 = DGAM * (ZFE - CE);
@@ -189,8 +211,9 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 			} else {
 				double PKE = PGAM * PPE + OGAM;
 				double ZNE = RCE * sqrt(PKE);
-				F2 = (P - PE) / ZNE;
-				FS2 = (QGAM * PPE + FGAM) / (RGAM * ZNE * PKE);
+				double IZ = 1.0 / (ZNE * PKE);
+				F2 = (P - PE) * PKE * IZ;
+				FS2 = (QGAM * PPE + FGAM) * IZ * K.IRGAM;
 			}
 			double res = DU - F1 - F2;
 			P = P + res / (FS1 + FS2);
@@ -204,36 +227,40 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 		// the reference forms both rarefaction heads before branching; each is
 		// only read on its rarefaction side, so it is formed there
 		if (PB > P) {
-			double ZFB = CB * pow_ogam(P / PB, K);
-			EF = ZFB * ZFB / SGAM;
+			double ZFB = CB * pow_ogam(P * IPB, K);
+			EF = ZFB * ZFB * K.ISGAM;
 			UF = UB + DGAM * (CB - ZFB);
 			RF = P / (AGAM * EF);
 			SBL = UB - CB;
 			SFL = UF - ZFB;
 		} else {
-			double D = UB - sqrt((TGAM * P + HGAM * PB) / RB);
+			double D = UB - sqrt((TGAM * P + HGAM * PB) * IRB);
 			double UBD = UB - D;
 			double RUBD = RB * UBD;
-			RF = RUBD * RUBD / (PB - P + RUBD * UBD);
-			UF = D + RUBD / RF;
-			EF = P / (AGAM * RF);
+			double Q = PB - P + RUBD * UBD;
+			double IR = 1.0 / RUBD;
+			RF = RUBD * RUBD / Q;
+			UF = D + Q * IR;
+			EF = P * Q * IR * IR * K.IAGAM;
 			SBL = D;
 			SFL = D;
 		}
 		if (PE > P) {
-			double ZFE = CE * pow_ogam(P / PE, K);
-			ES = ZFE * ZFE / SGAM;
+			double ZFE = CE * pow_ogam(P * IPE, K);
+			ES = ZFE * ZFE * K.ISGAM;
 			US = UE - DGAM * (CE - ZFE);
 			RS = P / (AGAM * ES);
 			SSL = US + ZFE;
 			SEL = UE + CE;
 		} else {
-			double D = UE + sqrt((TGAM * P + HGAM * PE) / RE);
+			double D = UE + sqrt((TGAM * P + HGAM * PE) * IRE);
 			double UED = UE - D;
 			double RUED = RE * UED;
-			RS = RUED * RUED / (PE - P + RUED * UED);
-			US = D + RUED / RS;
-			ES = P / (AGAM * RS);
+			double Q = PE - P + RUED * UED;
+			double IR = 1.0 / RUED;
+			RS = RUED * RUED / Q;
+			US = D + Q * IR;
+			ES = P * Q * IR * IR * K.IAGAM;
 			SEL = D;
 			SSL = D;
 		}

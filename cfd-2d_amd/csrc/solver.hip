@@ -309,12 +309,14 @@ __device__ __forceinline__ void recon(Prim& q, const Bufs& b, int c, double dx, 
 
 __device__ __forceinline__ void grad_T(const Bufs& b, int c, const Prim& q, const MatV& m, double& tx, double& ty)
 {
+	// T = p / (r Rg), Rg = gR / M: grad T = (grad p - (p/r) grad r) / (r Rg)
 	double g[4];
 	ld4(b.G, 2 * (size_t)c, g);  // rx ry px py
-	double Rg = GAS_R / m.M;
-	double pr = q.p / q.r;
-	tx = (g[2] - pr * g[0]) / (q.r * Rg);
-	ty = (g[3] - pr * g[1]) / (q.r * Rg);
+	double ir = 1.0 / q.r;
+	double pr = q.p * ir;
+	double s = ir * m.M * (1.0 / GAS_R);
+	tx = (g[2] - pr * g[0]) * s;
+	ty = (g[3] - pr * g[1]) * s;
 }
 
 // Edge flux pass, FVM_TVD::run edge loop + reconstruct + calcFlux
