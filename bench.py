@@ -18,6 +18,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 for _p in (os.path.join(REPO, "cfd-2d_amd"), os.path.join(REPO, "oracle")):
     if _p not in sys.path:
@@ -170,6 +172,47 @@ def adjoint_vs_forward(steps, ckpt, ny=289):
     return out
 
 
+def batched_adjoint(n_cases, steps, seed=11):
+    """Batched adjoint: n_cases bump channels (5k-100k cells, log-uniform;
+    Mach 0.3-2.0, angle -5..5 deg, bump height 0.02-0.1; LAX/GG/RK2, no
+    limiter), J_k = y pressure force on case k's bump after `steps` steps.
+    One cfd2d_adjoint call for all cases (automatic interval, arena from an
+    untimed first call) against the batched forward run of the same steps;
+    the returned gradients include dJ_k/dMach_k and dJ_k/dangle_k."""
+    from cfd2d import batch
+    from cfd2d import case as cases
+    from cfd2d import meshgen, sensitivity
+    specs = meshgen.bump_batch(n_cases, 5000, 100000, seed=seed, mach_range=(0.3, 2.0))
+    cs = [cases.bump_channel(s["ny"], height=s["height"], mach=s["mach"], angle_deg=s["angle_deg"], seed=s["seed"],
+                             flux="LAX", gradient="GREEN_GAUSS", limiter="NONE", rk_stages=2) for s in specs]
+    b = batch.BatchSolver(cs)
+    mask = b.edge_mask("bump")
+    U0 = b.ctx.state()
+    b.run(2)
+    b.ctx.set_state(U0)
+    b.ctx.sync()
+    t0 = time.perf_counter()
+    b.run(steps)
+    t_fwd = time.perf_counter() - t0
+    b.ctx.set_state(U0)
+    t0 = time.perf_counter()
+    b.ctx.adjoint(steps, mask, 0)
+    t_first = time.perf_counter() - t0
+    b.ctx.set_state(U0)
+    b.ctx.sync()
+    t0 = time.perf_counter()
+    res = b.adjoint(steps, mask, 0)
+    t_adj = time.perf_counter() - t0
+    b.close()
+    finite = all(np.isfinite(gU).all() and np.isfinite(gb).all() for _, gU, gb in res)
+    grads = [sensitivity.mach_angle_gradient(c, gU, gb) for c, (_, gU, gb) in zip(cs, res)]
+    cells = sum(c.mesh.n_cells for c in cs)
+    return dict(cases=n_cases, cells=cells, steps=steps, forward_s=round(t_fwd, 4), adjoint_s=round(t_adj, 4),
+                first_call_s=round(t_first, 4), ratio=round(t_adj / t_fwd, 3),
+                adjoint_cell_steps_per_s=cells * steps / t_adj, finite=bool(finite),
+                dJ_dMach_case0=grads[0][0], dJ_dangle_case0=grads[0][1])
+
+
 def load_traffic(workload):
     """HBM bytes per launch of each kernel from the committed rocprofv3 PMC summary."""
     path = os.path.join(REPO, "profiles", "traffic.json")
@@ -196,6 +239,9 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--adj-ckpt", type=int, default=20)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--badj-cases", type=int, default=64)
+    ap.add_argument("--badj-steps", type=int, default=300)
+    ap.add_argument("--no-badj", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -259,6 +305,9 @@ def main():
     if not args.no_adjoint:
         out["adjoint"] = adjoint_vs_forward(args.adj_steps, args.adj_ckpt)
         log("adjoint: %s" % out["adjoint"])
+    if not args.no_badj:
+        out["batched_adjoint"] = batched_adjoint(args.badj_cases, args.badj_steps)
+        log("batched adjoint: %s" % out["batched_adjoint"])
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case_fn, args.cpu_steps)
         ref = reference_cpu()

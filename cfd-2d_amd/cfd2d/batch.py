@@ -31,10 +31,12 @@ class BatchSolver:
         self.cell_off, self.edge_off = self.mesh.case_offsets()
         # boundary objects of all cases, groups renumbered per part
         bcs, gbc, cmat, U = [], [], [], []
+        self.bc_off = [0]
         for c, part in zip(self.cases, parts):
             base = len(bcs)
             own = [b for b in c.config.boundaries if b.type != "BOUND_PERIODIC"]
             bcs.extend(own)
+            self.bc_off.append(len(bcs))
             g = c.group_bc()
             gbc.extend([base + v if v >= 0 else -1 for v in g])
             cmat.append(c.cell_material())
@@ -57,6 +59,42 @@ class BatchSolver:
     def states(self):
         U = self.ctx.state()
         return [U[:, self.cell_off[k]:self.cell_off[k + 1]] for k in range(len(self.cases))]
+
+    def edge_mask(self, group):
+        """int32 mask of the edges in the named edge group of every case (the
+        concatenated mesh names case k's groups "k:name")."""
+        names = self.mesh.group_names(0)
+        ids = [i for i, n in enumerate(names) if n.split(":", 1)[-1] == group]
+        eg = self.mesh.export()["edge_group"]
+        return np.isin(eg, ids).astype(np.int32)
+
+    def adjoint(self, nsteps, edge_mask, ckpt_every=0):
+        """One reverse sweep for every case together: the cases are
+        independent, so the adjoint of the summed objective gives each case's
+        own gradients.  Returns per case (J_k, dJ_k/dU0 (4, nc_k),
+        dJ_k/dpar (n_bc_k, 4)); the context ends at the final state."""
+        _, gU, gb = self.ctx.adjoint(nsteps, edge_mask, ckpt_every)
+        Jc = self.wall_force_y(edge_mask)
+        return [(Jc[k], gU[:, self.cell_off[k]:self.cell_off[k + 1]], gb[self.bc_off[k]:self.bc_off[k + 1]])
+                for k in range(len(self.cases))]
+
+    def wall_force_y(self, edge_mask):
+        """Per case: sum over its masked edges of p(c1) * l * n_y on the current
+        state (one download of the state for all cases)."""
+        if not hasattr(self, "_edge_geom"):
+            ex = self.mesh.export()
+            self._edge_geom = (ex["edge_int"][:, 2].copy(), ex["edge_geom"][:, 2] * ex["edge_geom"][:, 1])
+        c1, lny = self._edge_geom
+        U = self.ctx.state()
+        gam = self.cases[0].config.materials[0].gamma
+        r = U[0]
+        u, v, E = U[1] / r, U[2] / r, U[3] / r
+        p = r * (E - 0.5 * (u * u + v * v)) * (gam - 1)
+        out = []
+        for k in range(len(self.cases)):
+            e = np.nonzero(edge_mask[self.edge_off[k]:self.edge_off[k + 1]])[0] + self.edge_off[k]
+            out.append(float(np.sum(p[c1[e]] * lny[e])))
+        return out
 
     def close(self):
         self.ctx.close()

@@ -230,11 +230,13 @@ def bump_channel(ny=20, height=0.05, jitter=0.1, seed=3, cell_name="fluid"):
     return m
 
 
-def bump_batch(n_cases=256, min_cells=2000, max_cells=60000, seed=7):
+def bump_batch(n_cases=256, min_cells=2000, max_cells=60000, seed=7, mach_range=(0.2, 3.0)):
     """Sizes, bump heights and inflow (Mach, angle) for a batch of bump-channel cases.
 
     Cell counts are log-uniform in [min_cells, max_cells]; each case's grid is
-    the (3*ny x ny) channel with 6*ny^2 triangles closest to its draw.
+    the (3*ny x ny) channel with 6*ny^2 triangles closest to its draw; Mach
+    uniform in mach_range, angle uniform in [-5, 5] degrees, bump height
+    uniform in [0.02, 0.1].
     """
     idx = np.arange(n_cases)
     u0 = 0.5 * (uniform(seed, 10, idx) + 1.0)
@@ -244,7 +246,7 @@ def bump_batch(n_cases=256, min_cells=2000, max_cells=60000, seed=7):
     cells = np.exp(math.log(min_cells) + u0 * (math.log(max_cells) - math.log(min_cells)))
     ny = np.maximum(np.rint(np.sqrt(cells / 6.0)).astype(int), 4)
     height = 0.02 + 0.08 * u1
-    mach = 0.2 + 2.8 * u2
+    mach = mach_range[0] + (mach_range[1] - mach_range[0]) * u2
     angle = 5.0 * u3
     return [dict(ny=int(ny[c]), height=float(height[c]), mach=float(mach[c]),
                  angle_deg=float(angle[c]), seed=int(seed * 100003 + c)) for c in range(n_cases)]

@@ -148,3 +148,63 @@ def test_adjoint_rejects_unsupported(over):
         g.adjoint(0, np.zeros(g.ne, np.int32))
     assert ei.value.code == 1
     g.close()
+
+
+def test_mach_and_angle_gradients_match_finite_differences():
+    """dJ/dMach and dJ/dangle through the inlet parameters and the
+    free-stream initial state (cfd2d.sensitivity) against central differences
+    of cases rebuilt at Mach +- h and angle +- h, time step held fixed."""
+    from cfd2d import sensitivity
+
+    def make(mach, ang):
+        return cases.bump_channel(16, height=0.06, mach=mach, angle_deg=ang, seed=3, flux="HLLC", gradient="GREEN_GAUSS",
+                                  limiter="NONE", rk_stages=2)
+
+    M, A, n = 0.8, 2.0, 30
+    c = make(M, A)
+    g = native.GpuSolver(c)
+    tau, _ = g.tau()
+    mask = g.edge_mask("bump") | g.edge_mask("wall_bottom")
+    J, gU, gb = g.adjoint(n, mask)
+    g.close()
+    dM, dA = sensitivity.mach_angle_gradient(c, gU, gb)
+
+    def Jat(mach, ang):
+        s = native.GpuSolver(make(mach, ang))
+        s.set_tau(tau)
+        s.run(n)
+        j = s.wall_force(mask)[1]
+        s.close()
+        return j
+
+    fdM = (Jat(M + 1e-6, A) - Jat(M - 1e-6, A)) / 2e-6
+    fdA = (Jat(M, A + 1e-5) - Jat(M, A - 1e-5)) / 2e-5
+    print("dJ/dMach %.12e fd %.12e; dJ/dangle %.12e fd %.12e" % (dM, fdM, dA, fdA))
+    assert abs(Jat(M, A) - J) <= 1e-13 * abs(J)
+    assert abs(dM - fdM) <= 1e-6 * abs(fdM)
+    assert abs(dA - fdA) <= 1e-5 * abs(fdA)
+
+
+@pytest.mark.parametrize("flux,gradient,rk", [("LAX", "GREEN_GAUSS", 2), ("HLLC", "LSQ", 3)])
+def test_batched_adjoint_equals_individual(flux, gradient, rk):
+    """One reverse sweep over concatenated cases gives each case's own
+    gradients: dJ/dU0 bit for bit (per-cell work is identical), J and the
+    boundary-parameter sums to round-off (the per-object reduction splits the
+    boundary edges over threads differently)."""
+    from cfd2d import batch
+    specs = [(5, 0.7, 1.0, 0.04), (7, 1.3, -2.0, 0.08), (9, 0.5, 4.0, 0.06)]
+    cs = [cases.bump_channel(ny, height=h, mach=m, angle_deg=a, seed=ny, flux=flux, gradient=gradient,
+                             limiter="NONE", rk_stages=rk) for ny, m, a, h in specs]
+    n = 12
+    b = batch.BatchSolver(cs)
+    mask = b.edge_mask("bump") | b.edge_mask("wall_bottom")
+    res = b.adjoint(n, mask, ckpt_every=5)
+    b.close()
+    for c, (Jb, gUb, gbb) in zip(cs, res):
+        s = native.GpuSolver(c)
+        m = s.edge_mask("bump") | s.edge_mask("wall_bottom")
+        J, gU, gb = s.adjoint(n, m, ckpt_every=5)
+        s.close()
+        assert abs(Jb - J) <= 1e-13 * abs(J)
+        assert np.array_equal(gUb, gU)
+        assert np.max(np.abs(gbb - gb)) <= 1e-13 * np.max(np.abs(gb))
