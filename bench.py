@@ -59,6 +59,24 @@ def cpu_baseline(case_fn, steps, threads=1):
                 % (steps, s.nc, threads), seconds=round(dt, 3), setup_seconds=round(setup, 3))
 
 
+def host_cpu():
+    """Host CPU model and the cores this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return dict(model=model, usable_cores=usable, logical_cpus=os.cpu_count())
+
+
 def reference_cpu(steps=100):
     """The reference's own FVM_TVD (compiled from its sources into
     oracle/_ref) on the 8,000-cell shock tube, where it can run."""
@@ -213,6 +231,27 @@ def batched_adjoint(n_cases, steps, seed=11):
                 dJ_dMach_case0=grads[0][0], dJ_dangle_case0=grads[0][1])
 
 
+def renumbering(n, steps=10):
+    """The same periodic Navier-Stokes case (n x n quads x 2) in three cell
+    orders: the generator's row order, a seeded random shuffle (what an
+    unordered external mesh looks like), and the shuffle renumbered along a
+    Hilbert curve (cfd2d/renumber.py).  ms per step of each."""
+    from cfd2d import case as cases
+    from cfd2d import native, renumber
+    base = cases.periodic_pulse(n, seed=2)
+    shuffled, _ = renumber.renumber_case(base, "random", seed=1)
+    hilbert, _ = renumber.renumber_case(shuffled, "hilbert")
+    out = dict(cells=base.mesh.n_cells, steps=steps)
+    for name, c in (("generator", base), ("random", shuffled), ("hilbert", hilbert)):
+        s = native.GpuSolver(c)
+        s.run(2)
+        t0 = time.perf_counter()
+        s.run(steps)
+        out[name + "_ms_per_step"] = round((time.perf_counter() - t0) * 1e3 / steps, 4)
+        s.close()
+    return out
+
+
 def load_traffic(workload):
     """HBM bytes per launch of each kernel from the committed rocprofv3 PMC summary."""
     path = os.path.join(REPO, "profiles", "traffic.json")
@@ -230,7 +269,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1024, help="quads per side of the periodic square")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay each step as a recorded HIP graph")
     ap.add_argument("--batch-cases", type=int, default=256)
@@ -242,6 +281,7 @@ def main():
     ap.add_argument("--badj-cases", type=int, default=64)
     ap.add_argument("--badj-steps", type=int, default=300)
     ap.add_argument("--no-badj", action="store_true")
+    ap.add_argument("--no-renumber", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -308,8 +348,17 @@ def main():
     if not args.no_badj:
         out["batched_adjoint"] = batched_adjoint(args.badj_cases, args.badj_steps)
         log("batched adjoint: %s" % out["batched_adjoint"])
+    if not args.no_renumber:
+        out["renumbering"] = renumbering(args.n)
+        log("renumbering: %s" % out["renumbering"])
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case_fn, args.cpu_steps)
+        host = host_cpu()
+        # the same restatement with OpenMP on the cores this process may use
+        # (capped at 16, the host share of one GPU)
+        nthr = max(1, min(16, host["usable_cores"] or 1))
+        out["cpu_baseline_parallel"] = cpu_baseline(case_fn, max(args.cpu_steps, 4), threads=nthr)
+        out["host_cpu"] = host
         ref = reference_cpu()
         if ref:
             out["cpu_reference"] = ref
