@@ -74,7 +74,7 @@ class BatchSolver:
         own gradients.  Returns per case (J_k, dJ_k/dU0 (4, nc_k),
         dJ_k/dpar (n_bc_k, 4)); the context ends at the final state."""
         _, gU, gb = self.ctx.adjoint(nsteps, edge_mask, ckpt_every)
-        Jc = self.wall_force_y(edge_mask)
+        Jc = self.ctx.adjoint_objectives(len(self.cases))
         return [(Jc[k], gU[:, self.cell_off[k]:self.cell_off[k + 1]], gb[self.bc_off[k]:self.bc_off[k + 1]])
                 for k in range(len(self.cases))]
 

@@ -79,6 +79,7 @@ SIGNATURES = {
     "cfd2d_wall_force": (C.c_int, [C.c_void_p, _ip, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "cfd2d_adjoint": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip, C.POINTER(C.c_double), C.c_void_p,
                                 C.c_void_p]),
+    "cfd2d_adjoint_objectives": (C.c_int, [C.c_void_p, _dp]),
     "cfd2d_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "cfd2d_timing_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "cfd2d_ctx_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -313,6 +314,12 @@ class Context:
         check(lib().cfd2d_adjoint(self._h, int(nsteps), int(ckpt_every), np.ascontiguousarray(edge_mask, np.int32),
                                   C.byref(J), gU.ctypes.data_as(C.c_void_p), gb.ctypes.data_as(C.c_void_p)))
         return J.value, gU.reshape(4, self.nc), gb.reshape(-1, 4)[:self.n_bc]
+
+    def adjoint_objectives(self, n_cases):
+        """Per-case objective of the last adjoint call (concatenated meshes)."""
+        out = np.zeros(n_cases)
+        check(lib().cfd2d_adjoint_objectives(self._h, out))
+        return out
 
     def timing(self, enable=True):
         check(lib().cfd2d_timing(self._h, 1 if enable else 0))

@@ -114,8 +114,9 @@ __device__ __forceinline__ void ghost_vjp(int type, const double* par, double nx
 	for (int q = 0; q < 4; q++) dpar[q] = 0.0;
 }
 
-// reverse of flux_lax: Lb, Rb (r, p, u, v) = fb times the flux Jacobian.  Written for few divisions: 1/r per side, and
-// dc/dp = c/(2p), dc/dr = -c/(2r) on the side that sets the wave speed.
+// reverse of flux_lax: Lb, Rb (r, p, u, v) = fb times the flux Jacobian.
+// Written for few divisions: 1/r per side, and dc/dp = c/(2p),
+// dc/dr = -c/(2r) on the side that sets the wave speed.
 __device__ __forceinline__ void lax_vjp(const Prim& L, const Prim& R, double nx, double ny, double G, double igm1,
                                         const double* fb, double* Lb, double* Rb)
 {
@@ -519,6 +520,16 @@ __global__ __launch_bounds__(BLOCK) void k_bc_reduce(Bufs b, AdjBufs a, int nbc)
 	}
 }
 
+// [n][4] records -> [4][n] arrays
+__global__ __launch_bounds__(BLOCK) void k_records_to_arrays(const double* rec, double* arr, int n)
+{
+	int i = blockIdx.x * BLOCK + threadIdx.x;
+	if (i >= n) return;
+	double v[4];
+	ld4(rec, i, v);
+	for (int q = 0; q < 4; q++) arr[(size_t)q * n + i] = v[q];
+}
+
 __global__ __launch_bounds__(BLOCK) void k_fill(double* x, size_t count, double v)
 {
 	size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
@@ -779,15 +790,21 @@ int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, do
 	HIPCHK(hipStreamSynchronize(c->st));
 	if ((rc = check_launch("adjoint forward pass"))) return rc;
 	if (lim) return fail(CFD2D_ERR_CONFIG, "adjoint: a cell left the bounds (LIM flag) during the steps");
-	if (J) {
-		// ascending edge order, as cfd2d_wall_force sums it
+	{
+		// ascending edge order, as cfd2d_wall_force sums it; also per case of
+		// a concatenated mesh (kept for cfd2d_adjoint_objectives)
 		double Jv = 0.0;
+		c->adj_Jcase.assign(c->ncase, 0.0);
 		for (int e = 0; e < ne; e++) {
 			if (!mask[e]) continue;
-			size_t k = std::lower_bound(cells.begin(), cells.end(), c->hc1[e]) - cells.begin();
-			Jv += pk[k] * c->hel[e] * c->hny[e];
+			int i = c->hc1[e];
+			size_t k = std::lower_bound(cells.begin(), cells.end(), i) - cells.begin();
+			double t = pk[k] * c->hel[e] * c->hny[e];
+			Jv += t;
+			int cs = (int)(std::upper_bound(c->case_off.begin(), c->case_off.end(), i) - c->case_off.begin()) - 1;
+			c->adj_Jcase[cs] += t;
 		}
-		*J = Jv;
+		if (J) *J = Jv;
 	}
 	// reverse sweep, segment by segment (the last one is already recorded)
 	const double wa2[2] = {0.0, 0.5}, wb2[2] = {1.0, 0.5};
@@ -813,8 +830,13 @@ int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, do
 	}
 	if (c->nbc > 0 && a.nbedges > 0)
 		hipLaunchKernelGGL(k_bc_reduce, dim3(1), dim3(BLOCK), 0, c->st, c->b, a, c->nbc);
-	std::vector<double> rec(dJdU0 ? n4 : 0);
-	if (dJdU0) HIPCHK(hipMemcpyAsync(rec.data(), a.Wn, bytes, hipMemcpyDeviceToHost, c->st));
+	if (dJdU0) {
+		// per-cell records to the API's [4][nc], transposed on the device
+		// (into Ws, free now) and copied once
+		hipLaunchKernelGGL(k_records_to_arrays, dim3(grid_of(nc)), dim3(BLOCK), 0, c->st, (const double*)a.Wn, a.Ws,
+		                   nc);
+		HIPCHK(hipMemcpyAsync(dJdU0, a.Ws, bytes, hipMemcpyDeviceToHost, c->st));
+	}
 	if (dJdbc && c->nbc > 0) {
 		if (a.nbedges > 0)
 			HIPCHK(hipMemcpyAsync(dJdbc, a.bcbar, 4 * (size_t)c->nbc * sizeof(double), hipMemcpyDeviceToHost, c->st));
@@ -825,9 +847,6 @@ int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, do
 	HIPCHK(hipMemcpyAsync(c->b.U, A->Ufin, bytes, hipMemcpyDeviceToDevice, c->st));
 	launch_prim<MM>(c);
 	HIPCHK(hipStreamSynchronize(c->st));
-	if (dJdU0)  // per-cell records to the API's [4][nc]
-		for (int i = 0; i < nc; i++)
-			for (int q = 0; q < 4; q++) dJdU0[(size_t)q * nc + i] = rec[4 * (size_t)i + q];
 	return check_launch("adjoint reverse sweep");
 }
 
@@ -846,4 +865,12 @@ extern "C" int cfd2d_adjoint(cfd2d_ctx* c, int nsteps, int ckpt_every, const int
 	               : adjoint_run<false>(c, nsteps, ckpt_every, edge_mask, J, dJdU0, dJdbc);
 	c->timing = timing;
 	return rc;
+}
+
+extern "C" int cfd2d_adjoint_objectives(cfd2d_ctx* c, double* J_case)
+{
+	if (!c || !J_case) return fail(CFD2D_ERR_ARG, "cfd2d_adjoint_objectives: bad arguments");
+	if ((int)c->adj_Jcase.size() != c->ncase) return fail(CFD2D_ERR_ARG, "cfd2d_adjoint_objectives: no adjoint run yet");
+	std::copy(c->adj_Jcase.begin(), c->adj_Jcase.end(), J_case);
+	return CFD2D_OK;
 }

@@ -622,6 +622,7 @@ struct cfd2d_ctx {
 	// calls and grown on demand
 	void* adj_ws = nullptr;
 	size_t adj_ws_bytes = 0;
+	std::vector<double> adj_Jcase;  // objective per case of the last adjoint call
 };
 
 namespace {

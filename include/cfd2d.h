@@ -162,6 +162,10 @@ int  cfd2d_ctx_stream(cfd2d_ctx* c, void** hip_stream);
  * context is left at the final state, as after cfd2d_step(c, nsteps). */
 int  cfd2d_adjoint(cfd2d_ctx* c, int nsteps, int ckpt_every, const int* edge_mask, double* J, double* dJ_dU0,
                    double* dJ_dbc);
+/* The objective of the last cfd2d_adjoint call split by case of a
+ * concatenated mesh (double[n_cases]; each case's own masked edges, summed
+ * in ascending edge order).  Extension, no reference counterpart. */
+int  cfd2d_adjoint_objectives(cfd2d_ctx* c, double* J_case);
 
 #ifdef __cplusplus
 }
