@@ -358,36 +358,92 @@ void FVM_TVD::run()
 		if (rc != CFD2D_OK) throw Exception(cfd2d_last_error(), rc);
 		step += k;
 		t = tt;
-		if (FILE_SAVE_STEP > 0 && step % FILE_SAVE_STEP == 0) save((int)step);
+		if (FILE_SAVE_STEP > 0 && step % FILE_SAVE_STEP == 0) saveAsync((int)step);
 		if (PRINT_STEP > 0 && step % PRINT_STEP == 0) log("step: %ld\t\ttime step: %.16f\n", step, t);
 	}
+	joinWriter();  // every file is complete when run() returns
 }
 
 void FVM_TVD::done()
 {
+	if (writer.joinable()) writer.join();  // an error it held was reported by run()
 	if (ctx) cfd2d_ctx_destroy(ctx);
 	if (msh) cfd2d_mesh_destroy(msh);
 	ctx = nullptr;
 	msh = nullptr;
 }
 
+// One output step copied to the host: the state and the per-cell time step.
+struct FVM_TVD::Snapshot {
+	int step = 0;
+	std::vector<double> U, ctau;
+};
+
+std::shared_ptr<FVM_TVD::Snapshot> FVM_TVD::snapshot(int step)
+{
+	if (outCells.empty()) {  // the mesh part of every file, read once
+		int nn = 0, ne = 0;
+		check(cfd2d_mesh_counts(msh, &nn, nullptr, &ne));
+		outCells.assign(3 * (size_t)nc, 0);
+		check(cfd2d_mesh_export(msh, nullptr, outCells.data(), nullptr, nullptr, nullptr, nullptr, nullptr));
+		outXY.assign(2 * (size_t)nn, 0.0);
+		check(cfd2d_mesh_nodes(msh, outXY.data()));
+		outNodes = nn;
+	}
+	auto s = std::make_shared<Snapshot>();
+	s->step = step;
+	s->U.resize(4 * (size_t)nc);
+	s->ctau.resize(nc);
+	check(cfd2d_get_state(ctx, s->U.data()));
+	check(cfd2d_get_tau(ctx, nullptr, s->ctau.data()));
+	return s;
+}
+
+void FVM_TVD::save(int step)
+{
+	joinWriter();
+	writeVtk(*snapshot(step));
+}
+
+// The state is copied at once (one device-to-host copy); formatting and
+// writing the text file, which takes far longer, runs on a writer thread
+// while the device continues.  One file is in flight at a time.
+void FVM_TVD::saveAsync(int step)
+{
+	joinWriter();
+	std::shared_ptr<Snapshot> s = snapshot(step);
+	writer = std::thread([this, s]() {
+		try {
+			writeVtk(*s);
+		} catch (...) {
+			writerError = std::current_exception();
+		}
+	});
+}
+
+void FVM_TVD::joinWriter()
+{
+	if (writer.joinable()) writer.join();
+	if (writerError) {
+		std::exception_ptr e = writerError;
+		writerError = nullptr;
+		std::rethrow_exception(e);
+	}
+}
+
 // FVM_TVD::save (src/methods/fvm_tvd.cpp:501-600): legacy-ASCII VTK with the
 // same fields, formats and line structure (including "MachNumber" =
 // (u^2+v^2)/c and the single-line data blocks), so existing post-processing
 // reads these files unchanged.
-void FVM_TVD::save(int step)
+void FVM_TVD::writeVtk(const Snapshot& snap) const
 {
-	int nn = 0, ne = 0;
-	check(cfd2d_mesh_counts(msh, &nn, nullptr, &ne));
-	std::vector<double> U(4 * (size_t)nc), ctau(nc), xy;
-	std::vector<int> cn(3 * (size_t)nc);
-	check(cfd2d_get_state(ctx, U.data()));
-	check(cfd2d_get_tau(ctx, nullptr, ctau.data()));
-	check(cfd2d_mesh_export(msh, nullptr, cn.data(), nullptr, nullptr, nullptr, nullptr, nullptr));
-	xy.assign(2 * (size_t)nn, 0.0);
-	check(cfd2d_mesh_nodes(msh, xy.data()));
+	const int nn = outNodes;
+	const std::vector<double>& U = snap.U;
+	const std::vector<double>& ctau = snap.ctau;
+	const std::vector<double>& xy = outXY;
+	const std::vector<int>& cn = outCells;
 	char fName[50];
-	snprintf(fName, sizeof(fName), "res_%010d.vtk", step);
+	snprintf(fName, sizeof(fName), "res_%010d.vtk", snap.step);
 	FILE* fp = fopen(fName, "w");
 	if (!fp) throw Exception(std::string("File '") + fName + "' opening error.", Exception::FILE_OPENING_ERROR);
 	fprintf(fp, "# vtk DataFile Version 2.0\n");

@@ -5,7 +5,10 @@
 #pragma once
 
 #include <cstdio>
+#include <exception>
+#include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/cfd2d.h"
@@ -84,8 +87,13 @@ public:
 	void save(int step);
 
 private:
+	struct Snapshot;
 	void parse(const char* xmlFileName);
 	void build();
+	std::shared_ptr<Snapshot> snapshot(int step);
+	void writeVtk(const Snapshot& s) const;
+	void saveAsync(int step);
+	void joinWriter();
 
 	double TMAX = 0, TAU = 0, CFL = 0;
 	int STEP_MAX = 0, FILE_SAVE_STEP = 1, PRINT_STEP = 1;
@@ -100,6 +108,12 @@ private:
 	cfd2d_ctx* ctx = nullptr;
 	int nc = 0;
 	std::vector<int> cellMat;
+	// output: mesh part of the VTK files (read once) and the writer thread
+	std::vector<double> outXY;
+	std::vector<int> outCells;
+	int outNodes = 0;
+	std::thread writer;
+	std::exception_ptr writerError;
 };
 
 namespace Solver {

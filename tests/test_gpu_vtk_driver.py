@@ -1,0 +1,49 @@
+"""The drop-in driver (cfd2d_run: ./task.xml in, task.log and res_*.vtk out)
+end to end on the GPU: output every FILE_OUTPUT_STEP steps (written by the
+writer thread while the device continues), and the density of the last file
+equals the oracle's state after the same steps."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle
+from cfd2d import case as cases
+from cfd2d import meshgen, unv
+
+pytestmark = pytest.mark.gpu
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _block(text, head, n):
+    k = text.index(head) + len(head)
+    return np.array(text[k:].split()[:n], dtype=np.float64)
+
+
+def test_driver_writes_vtk_matching_the_oracle():
+    exe = os.path.join(REPO, "cfd-2d_amd", "cfd2d", "cfd2d_run")
+    nx, ny, steps, every = 40, 6, 20, 5
+    with tempfile.TemporaryDirectory() as d:
+        unv.write_unv(meshgen.rect_mesh(nx, ny, 1.0, 0.2, seed=1), os.path.join(d, "mesh.unv"))
+        regions = [dict(name="cells_left", Vx=0.0, Vy=0.0, T=1.0, P=1.0),
+                   dict(name="cells_right", Vx=0.0, Vy=0.0, T=1.0, P=0.5)]
+        bnds = [dict(name="edge_bottom", type="BOUND_WALL_SLIP"), dict(name="edge_top", type="BOUND_WALL_SLIP"),
+                dict(name="edge_left", type="BOUND_OUTLET"), dict(name="edge_right", type="BOUND_OUTLET")]
+        with open(os.path.join(d, "task.xml"), "w") as f:
+            f.write(unv.task_xml("mesh.unv", regions=regions, boundaries=bnds, step_max=steps,
+                                 file_output_step=every, log_output_step=every))
+        r = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        names = sorted(f for f in os.listdir(d) if f.startswith("res_"))
+        assert names == ["res_%010d.vtk" % s for s in range(0, steps + 1, every)]
+        text = open(os.path.join(d, names[-1])).read()
+        assert "step: 20" in open(os.path.join(d, "task.log")).read()
+    nc = 2 * nx * ny
+    rho = _block(text, "SCALARS Density float 1\nLOOKUP_TABLE default\n", nc)
+    s = oracle.OracleSolver(cases.shock_tube(nx, ny, seed=1, mild=True))
+    s.run(steps)
+    ref = s.state()[0]
+    assert np.max(np.abs(rho - ref)) <= 1e-11 * np.max(np.abs(ref))
