@@ -348,6 +348,43 @@ __global__ __launch_bounds__(BLOCK) void k_edge_flux(Bufs b, Phys ph)
 		p2 = p1;
 		R = ghost_of<MM>(b, ph, e, c1, L, p1);
 	}
+	// Viscous terms first (only sx, sy and the energy term stay live through
+	// the Riemann solve).  Face gradient = mean of the two cell gradients (the
+	// cell's own on a boundary); face velocity = mean of the reconstructed
+	// states.  The pair's lanes share the cell work: lane 0 loads c1's
+	// velocity gradient and forms its grad T, lane 1 does c2's (c1's on a
+	// boundary), and they swap them; the sums are the same as one lane
+	// forming both.
+	double sx = 0.0, sy = 0.0, ev = 0.0;
+	if (VISC) {
+		MatV m = mat_of<MM>(b, ph, c1);
+		bool two = gp == 1 && c2 >= 0;
+		int cs = two ? c2 : c1;
+		double own[6];
+		ld4(b.G, 2 * (size_t)cs + 1, own);  // ux uy vx vy
+		grad_T(b, cs, two ? p2 : p1, mat_of<MM>(b, ph, cs), own[4], own[5]);
+		double oth[6];
+		for (int q = 0; q < 6; q++) oth[q] = __shfl_xor(own[q], 1);
+		const double* g1 = gp ? oth : own;  // c1's values
+		const double* g2 = gp ? own : oth;  // c2's values (c1's on a boundary)
+		double ux = g1[0], uy = g1[1], vx = g1[2], vy = g1[3], tx = g1[4], ty = g1[5];
+		if (c2 >= 0) {
+			ux = 0.5 * (ux + g2[0]);
+			uy = 0.5 * (uy + g2[1]);
+			vx = 0.5 * (vx + g2[2]);
+			vy = 0.5 * (vy + g2[3]);
+			tx = 0.5 * (tx + g2[4]);
+			ty = 0.5 * (ty + g2[5]);
+		}
+		double div = ux + vy;
+		double txx = m.mu * (2.0 * ux - (2.0 / 3.0) * div);
+		double tyy = m.mu * (2.0 * vy - (2.0 / 3.0) * div);
+		double txy = m.mu * (uy + vx);
+		sx = txx * nx + txy * ny;
+		sy = txy * nx + tyy * ny;
+		double uf = 0.5 * (L.u + R.u), vf = 0.5 * (L.v + R.v);
+		ev = uf * sx + vf * sy + m.kc * (tx * nx + ty * ny);
+	}
 	double f[4];
 	int hit = 0;
 	if (FLUX == CFD2D_FLUX_LAX) flux_lax(f, L, R, nx, ny, ph.gam_flux);
@@ -356,35 +393,9 @@ __global__ __launch_bounds__(BLOCK) void k_edge_flux(Bufs b, Phys ph)
 	else flux_godunov(f, L, R, nx, ny, ph.rk, ph.max_newton, hit);
 	if (hit) atomicAdd(&b.err[0], 1ull);
 	if (VISC) {
-		// face gradient = mean of the two cell gradients (the cell's own on a
-		// boundary); face velocity = mean of the reconstructed states
-		MatV m = mat_of<MM>(b, ph, c1);
-		double gv[4];
-		ld4(b.G, 2 * (size_t)c1 + 1, gv);  // ux uy vx vy
-		double ux = gv[0], uy = gv[1], vx = gv[2], vy = gv[3];
-		double tx, ty;
-		grad_T(b, c1, p1, m, tx, ty);
-		if (c2 >= 0) {
-			MatV m2 = mat_of<MM>(b, ph, c2);
-			double tx2, ty2;
-			grad_T(b, c2, p2, m2, tx2, ty2);
-			ld4(b.G, 2 * (size_t)c2 + 1, gv);
-			ux = 0.5 * (ux + gv[0]);
-			uy = 0.5 * (uy + gv[1]);
-			vx = 0.5 * (vx + gv[2]);
-			vy = 0.5 * (vy + gv[3]);
-			tx = 0.5 * (tx + tx2);
-			ty = 0.5 * (ty + ty2);
-		}
-		double div = ux + vy;
-		double txx = m.mu * (2.0 * ux - (2.0 / 3.0) * div);
-		double tyy = m.mu * (2.0 * vy - (2.0 / 3.0) * div);
-		double txy = m.mu * (uy + vx);
-		double sx = txx * nx + txy * ny, sy = txy * nx + tyy * ny;
-		double uf = 0.5 * (L.u + R.u), vf = 0.5 * (L.v + R.v);
 		f[1] -= sx;
 		f[2] -= sy;
-		f[3] -= uf * sx + vf * sy + m.kc * (tx * nx + ty * ny);
+		f[3] -= ev;
 	}
 	double o[4];
 	for (int q = 0; q < 4; q++) o[q] = __shfl_xor(f[q], 1);
