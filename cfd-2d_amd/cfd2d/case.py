@@ -123,15 +123,16 @@ def inflow_state(mach, angle_deg, gamma=1.4, rho=1.0, p=None):
     return mach * c * math.cos(ang), mach * c * math.sin(ang), p / rho, p
 
 
-def bump_channel(ny=20, height=0.05, mach=0.8, angle_deg=0.0, seed=3, **cfg):
+def bump_channel(ny=20, height=0.05, mach=0.8, angle_deg=0.0, seed=3, outlet="BOUND_PRESSURE_OUTLET", **cfg):
     """Channel with a circular-arc bump: inlet (Vx, Vy, T, P) from Mach and angle,
-    pressure outlet at the isentropic static pressure, slip walls elsewhere;
+    pressure outlet at the free-stream static pressure (or outlet="BOUND_OUTLET",
+    the extrapolating outlet the implicit method knows), slip walls elsewhere;
     the channel starts at the free-stream state."""
     m = meshgen.bump_channel(ny, height, seed=seed)
     mat = Material()
     u, v, T, p = inflow_state(mach, angle_deg, mat.gamma)
     bnds = [Boundary("inlet", "BOUND_INLET", Vx=u, Vy=v, T=T, P=p),
-            Boundary("outlet", "BOUND_PRESSURE_OUTLET", P=p),
+            Boundary("outlet", outlet, P=p),
             Boundary("wall_bottom", "BOUND_WALL_SLIP"), Boundary("bump", "BOUND_WALL_SLIP"),
             Boundary("wall_top", "BOUND_WALL_SLIP")]
     defaults = dict(flux="GODUNOV", gradient="GREEN_GAUSS", limiter="MINMOD", rk_stages=2, cfl=0.5)

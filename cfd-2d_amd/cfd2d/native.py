@@ -18,7 +18,7 @@ _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 
 K_GRADIENT, K_EDGE_FLUX, K_UPDATE, K_REMEDIATE, K_TAU, K_PRIM = range(6)
-KERNEL_NAMES = ["gradient", "edge_flux", "update", "remediate", "tau", "prim"]
+KERNEL_NAMES = ["gradient", "edge_flux", "update", "remediate", "tau", "prim", "implicit"]
 
 
 class Params(C.Structure):
@@ -80,6 +80,8 @@ SIGNATURES = {
     "cfd2d_adjoint": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip, C.POINTER(C.c_double), C.c_void_p,
                                 C.c_void_p]),
     "cfd2d_adjoint_objectives": (C.c_int, [C.c_void_p, _dp]),
+    "cfd2d_implicit_assemble": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p]),
     "cfd2d_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "cfd2d_timing_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "cfd2d_ctx_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -320,6 +322,22 @@ class Context:
         out = np.zeros(n_cases)
         check(lib().cfd2d_adjoint_objectives(self._h, out))
         return out
+
+    def implicit_assemble(self, steady=False, cfl=0.5, tau=1.0e-3):
+        """The implicit method's linear system for the current state as block
+        rows: (blocks (nc, 4, 4, 4) with slot 0 the diagonal and slots 1..3 the
+        faces in ascending edge order, cols (nc, 4) with -1 on boundary
+        faces, rhs (nc, 4), ctau (nc,))."""
+        nc = self.nc
+        blocks = np.zeros(64 * nc)
+        cols = np.zeros(4 * nc, np.int32)
+        rhs = np.zeros(4 * nc)
+        ctau = np.zeros(nc)
+        vp = C.c_void_p
+        check(lib().cfd2d_implicit_assemble(self._h, int(bool(steady)), float(cfl), float(tau),
+                                            blocks.ctypes.data_as(vp), cols.ctypes.data_as(vp),
+                                            rhs.ctypes.data_as(vp), ctau.ctypes.data_as(vp)))
+        return blocks.reshape(nc, 4, 4, 4), cols.reshape(nc, 4), rhs.reshape(nc, 4), ctau
 
     def timing(self, enable=True):
         check(lib().cfd2d_timing(self._h, 1 if enable else 0))

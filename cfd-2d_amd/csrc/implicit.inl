@@ -1,0 +1,216 @@
+// Assembly of the implicit method's linear system on the device (the
+// reference's FVM_TVD_IMPLICIT::run matrix fill, src/methods/
+// fvm_tvd_implicit.cpp:534-646); the solve stays on the host.  Included at
+// the end of solver.hip.
+//
+// Per step the reference forms, for the increment dU,
+//   [S/tau I + sum_faces A+(n_out) l] dU_i + sum_faces A-(n_out) l dU_j = RHS_i
+// where A+- = 0.5 R diag(lambda+-) L at the Roe average of the two cells'
+// centre states (ghost on a boundary), n_out the normal out of cell i, and
+// RHS_i the explicit residual of the implicit method's own flux (2 Gauss
+// points, linear reconstruction on interior faces, centre values on
+// boundary faces).  Steady runs take tau_i = 2 CFL S_i / (sum l (|u|+c) of
+// the Roe states + 1e-100).
+//
+// Layout of the result, one block row per cell (BSR with at most 4 blocks a
+// row): blocks[nc][4][16] row-major 4x4 blocks, slot 0 the diagonal, slots
+// 1..3 the faces in ascending edge order; cols[nc][4] the block columns
+// (slot 0 = the cell, -1 for a boundary face); rhs[nc][4]; ctau[nc].
+//
+// Three passes: gradients (Green-Gauss with the implicit ghost), the edge
+// flux (two lanes per edge), and one cell-centric pass gathering the RHS in
+// ascending edge order and forming the cell's Jacobian blocks face by face
+// (each cell recomputes the Roe average of its faces: no per-edge matrix
+// buffer, no atomics).
+
+namespace {
+
+template <bool MM>
+__device__ __forceinline__ Prim ghost_impl_of(const Bufs& b, const Phys& ph, int e, int c1, const Prim& inside)
+{
+	int bc = -b.ec2[e] - 1;
+	return ghost_implicit(b.bctype[bc], &b.bcpar[4 * bc], b.enx[e], b.eny[e], inside, mat_of<MM>(b, ph, c1));
+}
+
+// FVM_TVD_IMPLICIT::calcGrad (fvm_tvd_implicit.cpp:186-262): Green-Gauss of
+// the cell-centre primitives, the implicit ghost on a boundary
+template <bool MM>
+__global__ __launch_bounds__(BLOCK) void k_impl_gradient(Bufs b, Phys ph)
+{
+	__shared__ double lds[BLOCK * 9];
+	int c0 = xcd_block() * BLOCK;
+	int i = c0 + threadIdx.x;
+	double g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	if (i < b.nc) {
+		size_t n = b.nc;
+		Prim pi = load_prim(b, i);
+		for (int k = 0; k < 3; k++) {
+			int f = b.cface[k * n + i];
+			int e = f >> 1, side = f & 1;
+			int j = b.cnb[k * n + i];
+			Prim q = j >= 0 ? load_prim(b, j) : ghost_impl_of<MM>(b, ph, e, i, pi);
+			double nx = b.enx[e], ny = b.eny[e], l = b.el[e];
+			double s[4] = {pi.r + q.r, pi.p + q.p, pi.u + q.u, pi.v + q.v};
+			for (int c = 0; c < 4; c++) {
+				double tx = s[c] / 2 * nx * l;
+				double ty = s[c] / 2 * ny * l;
+				if (side == 0) {
+					g[2 * c] += tx;
+					g[2 * c + 1] += ty;
+				} else {
+					g[2 * c] -= tx;
+					g[2 * c + 1] -= ty;
+				}
+			}
+		}
+		double si = b.S[i];
+		for (int q = 0; q < 8; q++) g[q] = g[q] / si;
+	}
+	store_records<8>(b.G, c0, min(BLOCK, b.nc - c0), g, lds);
+}
+
+// The RHS flux of each edge (fvm_tvd_implicit.cpp:612-627): sum over the two
+// Gauss points, reconstructed on interior faces (reconstruct with a point,
+// 444-476), centre state and ghost on boundary faces; two lanes per edge,
+// F = (f1 + f2) l/2
+template <bool MM, int FLUX>
+__global__ __launch_bounds__(BLOCK) void k_impl_edge(Bufs b, Phys ph)
+{
+	int t = xcd_block() * BLOCK + threadIdx.x;
+	int e = t >> 1, gp = t & 1;
+	if (e >= b.ne) return;  // both lanes of a pair leave together
+	size_t ne = b.ne;
+	int c1 = b.ec1[e], c2 = b.ec2[e];
+	double nx = b.enx[e], ny = b.eny[e];
+	Prim L = load_prim(b, c1), R;
+	if (c2 >= 0) {
+		recon(L, b, c1, b.edl[(size_t)(2 * gp) * ne + e], b.edl[(size_t)(2 * gp + 1) * ne + e]);
+		R = load_prim(b, c2);
+		recon(R, b, c2, b.edl[(size_t)(4 + 2 * gp) * ne + e], b.edl[(size_t)(5 + 2 * gp) * ne + e]);
+	} else {
+		R = ghost_impl_of<MM>(b, ph, e, c1, L);
+	}
+	double f[4];
+	int hit = 0;
+	if (FLUX == CFD2D_FLUX_LAX) flux_lax(f, L, R, nx, ny, ph.gam_flux);
+	else flux_godunov(f, L, R, nx, ny, ph.rk, ph.max_newton, hit);
+	if (hit) atomicAdd(&b.err[0], 1ull);
+	double o[4];
+	for (int q = 0; q < 4; q++) o[q] = __shfl_xor(f[q], 1);
+	if (gp == 0) {
+		double hl = b.el[e] * 0.5;
+		for (int q = 0; q < 4; q++) b.F[q * ne + e] = (f[q] + o[q]) * hl;
+	}
+}
+
+struct ImplOut {
+	double* blocks;  // [nc][4][16]
+	int* cols;       // [nc][4]
+	double* rhs;     // [nc][4]
+	double* ctau;    // [nc]
+	double cfl, tau;
+	int steady;
+};
+
+// one block row per cell: RHS gathered in ascending edge order (the
+// reference's scatter order), then the face blocks in the same order
+template <bool MM>
+__global__ __launch_bounds__(BLOCK) void k_impl_assemble(Bufs b, Phys ph, ImplOut o)
+{
+	int i = xcd_block() * BLOCK + threadIdx.x;
+	if (i >= b.nc) return;
+	size_t n = b.nc;
+	double r[4];
+	gather_residual(b, i, r);
+	Prim pi = load_prim(b, i);
+	double diag[16];
+	for (int q = 0; q < 16; q++) diag[q] = 0.0;
+	double lam = 0.0;
+	double* blk = o.blocks + (size_t)i * 64;
+	int* col = o.cols + (size_t)i * 4;
+	col[0] = i;
+	for (int k = 0; k < 3; k++) {
+		int f = b.cface[k * n + i];
+		int e = f >> 1, side = f & 1;
+		int j = b.cnb[k * n + i];
+		// the edge's (c1, c2) centre states, as reconstruct(iEdge, L, R) gives them
+		Prim q = j >= 0 ? load_prim(b, j) : ghost_impl_of<MM>(b, ph, e, i, pi);
+		RoeAvg a = side ? roe_average(q, pi, ph.gam_flux) : roe_average(pi, q, ph.gam_flux);
+		double l = b.el[e];
+		if (o.steady) lam += (sqrt(a.u * a.u + a.v * a.v) + a.c) * l;
+		double sn = side ? -1.0 : 1.0;
+		double Ap[16], Am[16];
+		split_jacobians(a, sn * b.enx[e], sn * b.eny[e], ph.gam_flux, l, Ap, Am);
+		for (int q2 = 0; q2 < 16; q2++) diag[q2] += Ap[q2];
+		double* off = blk + 16 * (k + 1);
+		if (j >= 0) {
+			for (int q2 = 0; q2 < 16; q2++) off[q2] = Am[q2];
+			col[k + 1] = j;
+		} else {
+			for (int q2 = 0; q2 < 16; q2++) off[q2] = 0.0;
+			col[k + 1] = -1;
+		}
+	}
+	double S = b.S[i];
+	double ct = o.steady ? 2.0 * o.cfl * S / (lam + 1.0e-100) : o.tau;
+	for (int q = 0; q < 4; q++) diag[5 * q] += S / ct;
+	for (int q = 0; q < 16; q++) blk[q] = diag[q];
+	for (int q = 0; q < 4; q++) o.rhs[4 * (size_t)i + q] = r[q];
+	o.ctau[i] = ct;
+}
+
+}  // namespace
+
+extern "C" int cfd2d_implicit_assemble(cfd2d_ctx* c, int steady, double cfl, double tau, double* blocks, int* cols,
+                                       double* rhs, double* ctau)
+{
+	if (!c || (!steady && !(tau > 0.0)) || (steady && !(cfl > 0.0)))
+		return fail(CFD2D_ERR_ARG, "cfd2d_implicit_assemble: bad arguments");
+	if (c->p.flux != CFD2D_FLUX_GODUNOV && c->p.flux != CFD2D_FLUX_LAX)
+		return fail(CFD2D_ERR_CONFIG, "implicit assembly: FLUX must be GODUNOV or LAX");
+	std::vector<int> bt(c->nbc > 0 ? c->nbc : 1);
+	if (c->nbc > 0) HIPCHK(hipMemcpy(bt.data(), c->b.bctype, c->nbc * sizeof(int), hipMemcpyDeviceToHost));
+	for (int k = 0; k < c->nbc; k++)
+		if (bt[k] == CFD2D_BND_PRESSURE_OUTLET)
+			return fail(CFD2D_ERR_CONFIG, "implicit assembly: no pressure-outlet boundary in the implicit method");
+	int nc = c->nc;
+	size_t need = 64 * (size_t)nc * sizeof(double) + 4 * (size_t)nc * sizeof(int) + 5 * (size_t)nc * sizeof(double) + 1024;
+	if (need > c->impl_ws_bytes) {
+		if (c->impl_ws) (void)hipFree(c->impl_ws);
+		c->impl_ws = nullptr;
+		c->impl_ws_bytes = 0;
+		HIPCHK(hipMalloc(&c->impl_ws, need));
+		c->impl_ws_bytes = need;
+	}
+	char* base = (char*)c->impl_ws;
+	ImplOut o;
+	o.blocks = (double*)base;
+	o.rhs = o.blocks + 64 * (size_t)nc;
+	o.ctau = o.rhs + 4 * (size_t)nc;
+	o.cols = (int*)(o.ctau + nc);
+	o.cfl = cfl;
+	o.tau = tau;
+	o.steady = steady;
+	rec_start(c, CFD2D_K_IMPLICIT);
+	if (c->mm) hipLaunchKernelGGL((k_impl_gradient<true>), dim3(grid_of(nc)), dim3(BLOCK), 0, c->st, c->b, c->ph);
+	else hipLaunchKernelGGL((k_impl_gradient<false>), dim3(grid_of(nc)), dim3(BLOCK), 0, c->st, c->b, c->ph);
+	dim3 ge(grid_of(2 * c->ne));
+	if (c->p.flux == CFD2D_FLUX_LAX) {
+		if (c->mm) hipLaunchKernelGGL((k_impl_edge<true, CFD2D_FLUX_LAX>), ge, dim3(BLOCK), 0, c->st, c->b, c->ph);
+		else hipLaunchKernelGGL((k_impl_edge<false, CFD2D_FLUX_LAX>), ge, dim3(BLOCK), 0, c->st, c->b, c->ph);
+	} else {
+		if (c->mm) hipLaunchKernelGGL((k_impl_edge<true, CFD2D_FLUX_GODUNOV>), ge, dim3(BLOCK), 0, c->st, c->b, c->ph);
+		else hipLaunchKernelGGL((k_impl_edge<false, CFD2D_FLUX_GODUNOV>), ge, dim3(BLOCK), 0, c->st, c->b, c->ph);
+	}
+	if (c->mm) hipLaunchKernelGGL((k_impl_assemble<true>), dim3(grid_of(nc)), dim3(BLOCK), 0, c->st, c->b, c->ph, o);
+	else hipLaunchKernelGGL((k_impl_assemble<false>), dim3(grid_of(nc)), dim3(BLOCK), 0, c->st, c->b, c->ph, o);
+	rec_stop(c);
+	int rc = check_launch("implicit assembly");
+	if (rc) return rc;
+	if (blocks) HIPCHK(hipMemcpyAsync(blocks, o.blocks, 64 * (size_t)nc * sizeof(double), hipMemcpyDeviceToHost, c->st));
+	if (cols) HIPCHK(hipMemcpyAsync(cols, o.cols, 4 * (size_t)nc * sizeof(int), hipMemcpyDeviceToHost, c->st));
+	if (rhs) HIPCHK(hipMemcpyAsync(rhs, o.rhs, 4 * (size_t)nc * sizeof(double), hipMemcpyDeviceToHost, c->st));
+	if (ctau) HIPCHK(hipMemcpyAsync(ctau, o.ctau, (size_t)nc * sizeof(double), hipMemcpyDeviceToHost, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	return CFD2D_OK;
+}

@@ -327,6 +327,93 @@ __device__ __forceinline__ void flux_lax(double* f, const Prim& L, const Prim& R
 	f[3] = 0.5 * ((rer + R.p) * unr + (rel + L.p) * unl - alpha * (rer - rel));
 }
 
+// The implicit method's ghost, FVM_TVD_IMPLICIT::boundaryCond
+// (src/methods/fvm_tvd_implicit.cpp:963-1008): inlet takes T, p, u, v from
+// the parameters (held here in the explicit order u, v, T, p) and rho =
+// p M / (T gR) (URS(2)); outlet copies the inside state; walls copy it with
+// the normal velocity mirrored.  Unlike the explicit ghost, rho is not
+// recomputed from the cell-centre temperature.
+__device__ __forceinline__ Prim ghost_implicit(int type, const double* par, double nx, double ny, const Prim& pL,
+                                               const MatV& m)
+{
+	Prim g = pL;
+	if (type == BND_INLET) {
+		g.u = par[0];
+		g.v = par[1];
+		g.p = par[3];
+		g.r = g.p * m.M / (par[2] * GAS_R);
+	} else if (type != BND_OUTLET) {
+		double Un = pL.u * nx + pL.v * ny;
+		g.u = pL.u - nx * Un * 2.0;
+		g.v = pL.v - ny * Un * 2.0;
+	}
+	return g;
+}
+
+// roe_orig (src/global.cpp:407-441) without the third velocity: Roe averages
+// of u, v and the total enthalpy; p from H, then c = sqrt(GAM p / r) and
+// H = E + p / r as FVM_TVD_IMPLICIT::calcRoeAverage forms them
+// (fvm_tvd_implicit.cpp:408-429)
+struct RoeAvg {
+	double r, u, v, p, c, H;
+};
+
+__device__ __forceinline__ RoeAvg roe_average(const Prim& L, const Prim& R, double GAM)
+{
+	double AGAM = GAM - 1.0;
+	double sb = sqrt(L.r), se = sqrt(R.r);
+	double s_ = 1.0 / (sb + se);
+	RoeAvg a;
+	a.r = sb * se;
+	a.u = (sb * L.u + se * R.u) * s_;
+	a.v = (sb * L.v + se * R.v) * s_;
+	double EB = L.p / (L.r * AGAM), EE = R.p / (R.r * AGAM);
+	double HB = EB + (L.u * L.u + L.v * L.v) * 0.5 + L.p / L.r;
+	double HE = EE + (R.u * R.u + R.v * R.v) * 0.5 + R.p / R.r;
+	double HI = (sb * HB + se * HE) * s_;
+	a.p = (HI - (a.u * a.u + a.v * a.v) * 0.5) * a.r * (GAM - 1.0) / GAM;
+	double e = a.p / (a.r * AGAM);
+	a.c = sqrt(GAM * a.p / a.r);
+	a.H = e + 0.5 * (a.u * a.u + a.v * a.v) + a.p / a.r;
+	return a;
+}
+
+// A+ and A- of FVM_TVD_IMPLICIT (eigenValues, rightEigenVector,
+// leftEigenVector, calcAP, calcAM: fvm_tvd_implicit.cpp:335-406):
+// A+- = 0.5 R diag(max/min(lambda, 0)) L at the Roe state for normal n
+// (the reference's factor 0.5 included), each times scale
+__device__ __forceinline__ void split_jacobians(const RoeAvg& a, double nx, double ny, double GAM, double scale,
+                                                double* Ap, double* Am)
+{
+	double c = a.c, u = a.u, v = a.v, H = a.H;
+	double qn = u * nx + v * ny, q2 = u * u + v * v;
+	double lx = -ny, ly = nx, ql = u * lx + v * ly;
+	double g1 = GAM - 1.0, c2 = c * c;
+	double lam[4] = {qn - c, qn, qn + c, qn};
+	double Rm[4][4] = {{1.0, 1.0, 1.0, 0.0},
+	                   {u - c * nx, u, u + c * nx, lx},
+	                   {v - c * ny, v, v + c * ny, ly},
+	                   {H - qn * c, q2 / 2, H + qn * c, ql}};
+	double Lm[4][4] = {{0.5 * (0.5 * q2 * g1 / c2 + qn / c), -0.5 * (g1 * u / c2 + nx / c), -0.5 * (g1 * v / c2 + ny / c),
+	                    0.5 * g1 / c2},
+	                   {1 - 0.5 * q2 * g1 / c2, g1 * u / c2, g1 * v / c2, -g1 / c2},
+	                   {0.5 * (0.5 * q2 * g1 / c2 - qn / c), -0.5 * (g1 * u / c2 - nx / c), -0.5 * (g1 * v / c2 - ny / c),
+	                    0.5 * g1 / c2},
+	                   {-ql, lx, ly, 0}};
+	// (R diag(lam+-)) L, summed over k in the reference's order
+	for (int i = 0; i < 4; i++)
+		for (int j = 0; j < 4; j++) {
+			double sp = 0.0, sm = 0.0;
+			for (int k = 0; k < 4; k++) {
+				double lp = lam[k] > 0 ? lam[k] : 0, lmn = lam[k] < 0 ? lam[k] : 0;
+				sp += (Rm[i][k] * lp) * Lm[k][j];
+				sm += (Rm[i][k] * lmn) * Lm[k][j];
+			}
+			Ap[4 * i + j] = sp * 0.5 * scale;
+			Am[4 * i + j] = sm * 0.5 * scale;
+		}
+}
+
 // Extension: HLLC (Toro, Riemann Solvers, 3rd ed., 10.4-10.6), Davis wave speeds
 __device__ __forceinline__ void flux_hllc(double* f, const Prim& L, const Prim& R, double nx, double ny, double GAM)
 {

@@ -634,6 +634,9 @@ struct cfd2d_ctx {
 	void* adj_ws = nullptr;
 	size_t adj_ws_bytes = 0;
 	std::vector<double> adj_Jcase;  // objective per case of the last adjoint call
+	// implicit assembly output (blocks, rhs, ctau, cols), grown on demand
+	void* impl_ws = nullptr;
+	size_t impl_ws_bytes = 0;
 };
 
 namespace {
@@ -865,6 +868,7 @@ void destroy_ctx(cfd2d_ctx* c)
 	for (auto e : c->ev1) (void)hipEventDestroy(e);
 	for (void* p : c->allocs) (void)hipFree(p);
 	if (c->adj_ws) (void)hipFree(c->adj_ws);
+	if (c->impl_ws) (void)hipFree(c->impl_ws);
 	if (c->st) (void)hipStreamDestroy(c->st);
 	delete c;
 }
@@ -1311,3 +1315,4 @@ int cfd2d_ctx_stream(cfd2d_ctx* c, void** s)
 }  // extern "C"
 
 #include "adjoint.inl"
+#include "implicit.inl"

@@ -50,7 +50,7 @@ enum { CFD2D_MAXMAT = 8 };
 /* kernels reported by cfd2d_timing_read */
 enum {
 	CFD2D_K_GRADIENT = 0, CFD2D_K_EDGE_FLUX = 1, CFD2D_K_UPDATE = 2, CFD2D_K_REMEDIATE = 3,
-	CFD2D_K_TAU = 4, CFD2D_K_PRIM = 5, CFD2D_K_COUNT = 6
+	CFD2D_K_TAU = 4, CFD2D_K_PRIM = 5, CFD2D_K_IMPLICIT = 6, CFD2D_K_COUNT = 7
 };
 
 /*
@@ -166,6 +166,30 @@ int  cfd2d_adjoint(cfd2d_ctx* c, int nsteps, int ckpt_every, const int* edge_mas
  * concatenated mesh (double[n_cases]; each case's own masked edges, summed
  * in ascending edge order).  Extension, no reference counterpart. */
 int  cfd2d_adjoint_objectives(cfd2d_ctx* c, double* J_case);
+
+/* The implicit method's linear system for the context's current state,
+ * the matrix fill of FVM_TVD_IMPLICIT::run (src/methods/fvm_tvd_implicit.cpp:
+ * 534-663: calcGrad 186-262, reconstruct 431-476, calcRoeAverage 408-429,
+ * eigenValues/right/leftEigenVector/calcAP/calcAM 335-406, calcFlux
+ * 917-961, boundaryCond 963-1008), assembled on the device; the solve
+ * (MatrixSolver, HYPRE GMRES) stays with the caller.
+ * One block row per cell, at most 4 blocks (BSR):
+ *   blocks double[n_cells][4][16]  row-major 4x4 blocks; slot 0 = diagonal
+ *                                  (sum of A+ l over the faces + S/ctau I),
+ *                                  slots 1..3 = A- l of the faces in
+ *                                  ascending edge order (zero on a boundary)
+ *   cols   int[n_cells][4]         block column of each slot (-1: boundary)
+ *   rhs    double[n_cells][4]      right4: -sum over faces of the 2-point
+ *                                  flux times l/2 (outward from the cell)
+ *   ctau   double[n_cells]         steady: 2 cfl S / (sum l (|u|+c) of the
+ *                                  faces' Roe states + 1e-100); else tau
+ * steady != 0 uses cfl, otherwise tau.  FLUX must be GODUNOV or LAX (the
+ * implicit method's calcFlux); GAM = 1.4 in the flux and the Jacobians as in
+ * the reference.  Any output pointer may be NULL.  Boundary objects of type
+ * PRESSURE_OUTLET are rejected (the implicit method has none).  Uses the
+ * context's gradient and edge-flux buffers as scratch. */
+int  cfd2d_implicit_assemble(cfd2d_ctx* c, int steady, double cfl, double tau, double* blocks, int* cols,
+                             double* rhs, double* ctau);
 
 #ifdef __cplusplus
 }

@@ -25,6 +25,14 @@
  *     viscous flux (Couette/shear closed form), SSP-RK3 (order on a linear
  *     ODE), periodic pairing (conservation, translation invariance),
  *     spectral time step (uniform-state closed form), pressure outlet.
+ *   implicit-method matrix fill (or_implicit_assemble, or_roe_jacobians;
+ *   the reference's fvm_tvd_implicit.cpp does not build here: it includes
+ *   HYPRE headers from an empty submodule) pinned by tests/test_implicit_cpu.py:
+ *     Roe property A(U_roe)(U_R - U_L) = F(U_R) - F(U_L), A+ + A- = A/2 at
+ *     the Roe state against the analytic Euler Jacobian, spectra of A+ and
+ *     A-, supersonic upwinding, zero column sums on a periodic mesh,
+ *     RHS = the explicit residual (golden-pinned) on a periodic mesh,
+ *     free-stream RHS = 0, ghost closed forms, steady tau closed form.
  */
 #include "oracle.h"
 
@@ -1283,6 +1291,330 @@ double or_wall_force(const or_solver* s, const int* edge_mask, double* fx)
 	}
 	if (fx) *fx = Fx;
 	return Fy;
+}
+
+/* ------------------------------------------------------------ implicit */
+
+/*
+ * FVM_TVD_IMPLICIT::boundaryCond (src/methods/fvm_tvd_implicit.cpp:963-1008):
+ * inlet takes T, p, u, v from the boundary and rho from URS(2), then URS(1);
+ * outlet copies the inside state; the wall copies it and mirrors the normal
+ * velocity (no URS afterwards, unlike the explicit method's ghost).
+ */
+static void ghost_impl(const or_solver* s, int e, const prm* pL, prm* pR)
+{
+	const or_mesh* m = s->m;
+	int b = s->edge_bc[e];
+	int mat = s->cell_mat[m->c1[e]];
+	const double* par = s->c.bc_par[b];
+	if (s->c.bc_type[b] == OR_BND_INLET) {
+		memset(pR, 0, sizeof(*pR));
+		pR->T = par[2];
+		pR->p = par[3];
+		pR->u = par[0];
+		pR->v = par[1];
+		urs(s, mat, pR, 2);
+		urs(s, mat, pR, 1);
+	} else if (s->c.bc_type[b] == OR_BND_OUTLET) {
+		*pR = *pL;
+	} else {
+		*pR = *pL;
+		double Un = pL->u * m->nx[e] + pL->v * m->ny[e];
+		double Vx = m->nx[e] * Un * 2.0;
+		double Vy = m->ny[e] * Un * 2.0;
+		pR->u = pL->u - Vx;
+		pR->v = pL->v - Vy;
+	}
+}
+
+/* FVM_TVD_IMPLICIT::reconstruct(iFace, pL, pR), fvm_tvd_implicit.cpp:431-442 */
+static void impl_centre(const or_solver* s, int e, prm* pL, prm* pR)
+{
+	const or_mesh* m = s->m;
+	*pL = s->P[m->c1[e]];
+	if (m->c2[e] >= 0) *pR = s->P[m->c2[e]];
+	else ghost_impl(s, e, pL, pR);
+}
+
+/* FVM_TVD_IMPLICIT::calcGrad, fvm_tvd_implicit.cpp:186-262 (edge scatter);
+ * g[8][nc] as (rx ry px py ux uy vx vy) */
+static void impl_grad(const or_solver* s, double* g)
+{
+	const or_mesh* m = s->m;
+	int nc = m->nc;
+	memset(g, 0, 8 * (size_t)nc * sizeof(double));
+	for (int e = 0; e < m->ne; e++) {
+		int c1 = m->c1[e], c2 = m->c2[e];
+		prm pL, pR;
+		impl_centre(s, e, &pL, &pR);
+		double nx = m->nx[e], ny = m->ny[e], l = m->l[e];
+		double t[8];
+		t[0] = (pL.r + pR.r) / 2 * nx * l;
+		t[1] = (pL.r + pR.r) / 2 * ny * l;
+		t[2] = (pL.p + pR.p) / 2 * nx * l;
+		t[3] = (pL.p + pR.p) / 2 * ny * l;
+		t[4] = (pL.u + pR.u) / 2 * nx * l;
+		t[5] = (pL.u + pR.u) / 2 * ny * l;
+		t[6] = (pL.v + pR.v) / 2 * nx * l;
+		t[7] = (pL.v + pR.v) / 2 * ny * l;
+		for (int q = 0; q < 8; q++) g[(size_t)q * nc + c1] += t[q];
+		if (c2 > -1)
+			for (int q = 0; q < 8; q++) g[(size_t)q * nc + c2] -= t[q];
+	}
+	for (int i = 0; i < nc; i++) {
+		double si = m->S[i];
+		for (int q = 0; q < 8; q++) g[(size_t)q * nc + i] /= si;
+	}
+}
+
+/* FVM_TVD_IMPLICIT::reconstruct(iFace, pL, pR, p), fvm_tvd_implicit.cpp:444-476:
+ * linear from both centroids on an inner face, centre state and ghost on a
+ * boundary face */
+static void impl_recon(const or_solver* s, const double* g, int e, int gp, prm* pL, prm* pR)
+{
+	const or_mesh* m = s->m;
+	int nc = m->nc;
+	int c1 = m->c1[e], c2 = m->c2[e];
+	*pL = s->P[c1];
+	if (c2 < 0) {
+		ghost_impl(s, e, pL, pR);
+		return;
+	}
+	*pR = s->P[c2];
+	double dx, dy;
+	gp_offset(m, e, c1, gp, &dx, &dy);
+	pL->r += g[c1] * dx + g[(size_t)nc + c1] * dy;
+	pL->p += g[2 * (size_t)nc + c1] * dx + g[3 * (size_t)nc + c1] * dy;
+	pL->u += g[4 * (size_t)nc + c1] * dx + g[5 * (size_t)nc + c1] * dy;
+	pL->v += g[6 * (size_t)nc + c1] * dx + g[7 * (size_t)nc + c1] * dy;
+	gp_offset(m, e, c2, gp, &dx, &dy);
+	pR->r += g[c2] * dx + g[(size_t)nc + c2] * dy;
+	pR->p += g[2 * (size_t)nc + c2] * dx + g[3 * (size_t)nc + c2] * dy;
+	pR->u += g[4 * (size_t)nc + c2] * dx + g[5 * (size_t)nc + c2] * dy;
+	pR->v += g[6 * (size_t)nc + c2] * dx + g[7 * (size_t)nc + c2] * dy;
+}
+
+/* roe_orig, src/global.cpp:407-441 */
+static void roe_orig(double* RI, double* EI, double* PI, double* UI, double* VI, double* WI,
+                     double RB, double PB, double UB, double VB, double WB,
+                     double RE, double PE, double UE, double VE, double WE, double GAM)
+{
+	double AGAM = GAM - 1.0;
+	double fSB = sqrt(RB), fSE = sqrt(RE);
+	double fS_ = 1.0 / (fSB + fSE);
+	*RI = fSB * fSE;
+	*UI = (fSB * UB + fSE * UE) * fS_;
+	*VI = (fSB * VB + fSE * VE) * fS_;
+	*WI = (fSB * WB + fSE * WE) * fS_;
+	double EB = PB / (RB * AGAM), EE = PE / (RE * AGAM);
+	double HB = EB + (UB * UB + VB * VB + WB * WB) * 0.5 + PB / RB;
+	double HE = EE + (UE * UE + VE * VE + WE * WE) * 0.5 + PE / RE;
+	double HI = (fSB * HB + fSE * HE) * fS_;
+	*PI = (HI - (*UI * *UI + *VI * *VI + *WI * *WI) * 0.5) * *RI * (GAM - 1.0) / GAM;
+	*EI = *PI / (*RI * AGAM);
+}
+
+/* FVM_TVD_IMPLICIT::calcRoeAverage, fvm_tvd_implicit.cpp:408-429 */
+static void roe_average(prm* a, const prm* pL, const prm* pR, double GAM)
+{
+	double WI;
+	roe_orig(&a->r, &a->e, &a->p, &a->u, &a->v, &WI, pL->r, pL->p, pL->u, pL->v, 0.0, pR->r, pR->p, pR->u, pR->v,
+	         0.0, GAM);
+	a->cz = sqrt(GAM * a->p / a->r);
+	a->E = a->e + 0.5 * (a->u * a->u + a->v * a->v);
+}
+
+/* eigenValues, rightEigenVector, leftEigenVector, fvm_tvd_implicit.cpp:335-369 */
+static void eigen_system(double lam[4][4], double R[4][4], double L[4][4], const prm* a, double nx, double ny,
+                         double H, double GAM)
+{
+	double c = a->cz, u = a->u, v = a->v;
+	double qn = u * nx + v * ny, q2 = u * u + v * v;
+	double lx = -ny, ly = nx, ql = u * lx + v * ly;
+	double g1 = GAM - 1.0, c2 = c * c;
+	memset(lam, 0, 16 * sizeof(double));
+	lam[0][0] = qn - c;
+	lam[1][1] = qn;
+	lam[2][2] = qn + c;
+	lam[3][3] = qn;
+	R[0][0] = 1.0;        R[0][1] = 1.0;     R[0][2] = 1.0;        R[0][3] = 0.0;
+	R[1][0] = u - c * nx; R[1][1] = u;       R[1][2] = u + c * nx; R[1][3] = lx;
+	R[2][0] = v - c * ny; R[2][1] = v;       R[2][2] = v + c * ny; R[2][3] = ly;
+	R[3][0] = H - qn * c; R[3][1] = q2 / 2;  R[3][2] = H + qn * c; R[3][3] = ql;
+	L[0][0] = 0.5 * (0.5 * q2 * g1 / c2 + qn / c);
+	L[0][1] = -0.5 * (g1 * u / c2 + nx / c);
+	L[0][2] = -0.5 * (g1 * v / c2 + ny / c);
+	L[0][3] = 0.5 * g1 / c2;
+	L[1][0] = 1 - 0.5 * q2 * g1 / c2;
+	L[1][1] = g1 * u / c2;
+	L[1][2] = g1 * v / c2;
+	L[1][3] = -g1 / c2;
+	L[2][0] = 0.5 * (0.5 * q2 * g1 / c2 - qn / c);
+	L[2][1] = -0.5 * (g1 * u / c2 - nx / c);
+	L[2][2] = -0.5 * (g1 * v / c2 - ny / c);
+	L[2][3] = 0.5 * g1 / c2;
+	L[3][0] = -ql;
+	L[3][1] = lx;
+	L[3][2] = ly;
+	L[3][3] = 0;
+}
+
+/* multMtx4, fvm_tvd_implicit.cpp:264-277 */
+static void mult4(double d[4][4], double a[4][4], double b[4][4])
+{
+	for (int i = 0; i < 4; i++)
+		for (int j = 0; j < 4; j++) {
+			double sum = 0;
+			for (int k = 0; k < 4; k++) sum += a[i][k] * b[k][j];
+			d[i][j] = sum;
+		}
+}
+
+/* calcAP / calcAM, fvm_tvd_implicit.cpp:371-406: 0.5 R diag(lambda+-) L */
+static void split_part(double d[4][4], double R[4][4], double lam[4][4], double L[4][4], int plus)
+{
+	double t[4][4], D[4][4];
+	for (int i = 0; i < 4; i++)
+		for (int j = 0; j < 4; j++)
+			D[i][j] = plus ? (lam[i][j] > 0 ? lam[i][j] : 0) : (lam[i][j] < 0 ? lam[i][j] : 0);
+	mult4(t, R, D);
+	mult4(d, t, L);
+	for (int i = 0; i < 4; i++)
+		for (int j = 0; j < 4; j++) d[i][j] *= 0.5;
+}
+
+/* A+ l and A- l of one face for normal n at the Roe state a */
+static void split_face(double Ap[4][4], double Am[4][4], const prm* a, double nx, double ny, double l, double GAM)
+{
+	double lam[4][4], R[4][4], L[4][4];
+	double H = a->E + a->p / a->r;
+	eigen_system(lam, R, L, a, nx, ny, H, GAM);
+	split_part(Ap, R, lam, L, 1);
+	split_part(Am, R, lam, L, 0);
+	for (int i = 0; i < 4; i++)
+		for (int j = 0; j < 4; j++) {
+			Am[i][j] *= l;
+			Ap[i][j] *= l;
+		}
+}
+
+/* the block slot of edge e in cell i's row: 1 + its rank among the cell's edges */
+static int slot_of(const or_mesh* m, int i, int e)
+{
+	for (int k = 0; k < 3; k++)
+		if (m->cedges[3 * (size_t)i + k] == e) return k + 1;
+	return -1;
+}
+
+static void add_block(double* blocks, int i, int slot, double A[4][4])
+{
+	double* b = &blocks[(size_t)i * 64 + 16 * (size_t)slot];
+	for (int r = 0; r < 4; r++)
+		for (int c = 0; c < 4; c++) b[4 * r + c] += A[r][c];
+}
+
+/*
+ * The matrix fill of FVM_TVD_IMPLICIT::run (src/methods/fvm_tvd_implicit.cpp:
+ * 534-663) for the current state: per edge in index order the Roe average of
+ * the centre states, A+(n) l into (c1,c1), A-(n) l into (c1,c2), A+(-n) l into
+ * (c2,c2), A-(-n) l into (c2,c1); the 2-Gauss-point flux times l/2 out of c1
+ * and into c2; the steady spectral sum; then S/ctau on the diagonal.
+ * Output as block rows: blocks[nc][4][16] (slot 0 diagonal, slot k+1 the
+ * k-th face in ascending edge order), cols[nc][4], rhs[nc][4], ctau[nc].
+ * GAM is the configured flux gamma (the reference's __GAM = 1.4).
+ */
+int or_implicit_assemble(or_solver* s, int steady, double cfl, double tau, double* blocks, int* cols, double* rhs,
+                         double* ctau)
+{
+	const or_mesh* m = s->m;
+	int nc = m->nc, ne = m->ne;
+	double GAM = s->c.gam_flux;
+	for (int b = 0; b < s->c.nbc; b++)
+		if (s->c.bc_type[b] == OR_BND_PRESSURE_OUTLET) return 1;
+	if (s->c.flux != OR_FLUX_GODUNOV && s->c.flux != OR_FLUX_LAX) return 2;
+	prim_pass(s, s->U);
+	double* g = (double*)malloc(8 * (size_t)nc * sizeof(double));
+	double* lam_sum = (double*)calloc(nc, sizeof(double));
+	memset(blocks, 0, 64 * (size_t)nc * sizeof(double));
+	memset(rhs, 0, 4 * (size_t)nc * sizeof(double));
+	for (int i = 0; i < nc; i++) {
+		cols[4 * (size_t)i] = i;
+		for (int k = 0; k < 3; k++) {
+			int e = m->cedges[3 * (size_t)i + k];
+			int j = m->c1[e] == i ? m->c2[e] : m->c1[e];
+			cols[4 * (size_t)i + 1 + k] = j;
+		}
+	}
+	impl_grad(s, g);
+	long hits = 0;
+	for (int e = 0; e < ne; e++) {
+		int c1 = m->c1[e], c2 = m->c2[e];
+		double l = m->l[e], nx = m->nx[e], ny = m->ny[e];
+		prm cL, cR, avg;
+		impl_centre(s, e, &cL, &cR);
+		roe_average(&avg, &cL, &cR, GAM);
+		if (steady) {
+			double lambda = sqrt(avg.u * avg.u + avg.v * avg.v) + avg.cz;
+			lambda *= l;
+			lam_sum[c1] += lambda;
+			if (c2 >= 0) lam_sum[c2] += lambda;
+		}
+		double Ap[4][4], Am[4][4];
+		split_face(Ap, Am, &avg, nx, ny, l, GAM);
+		add_block(blocks, c1, 0, Ap);
+		if (c2 >= 0) {
+			add_block(blocks, c1, slot_of(m, c1, e), Am);
+			split_face(Ap, Am, &avg, -nx, -ny, l, GAM);
+			add_block(blocks, c2, 0, Ap);
+			add_block(blocks, c2, slot_of(m, c2, e), Am);
+		}
+		double fr[4] = {0.0, 0.0, 0.0, 0.0};
+		for (int gp = 0; gp < 2; gp++) {
+			prm pL, pR;
+			impl_recon(s, g, e, gp, &pL, &pR);
+			double f[4];
+			int hit = 0;
+			if (s->c.flux == OR_FLUX_LAX) flux_lax(f, &pL, &pR, nx, ny, GAM);
+			else flux_godunov(f, &pL, &pR, nx, ny, GAM, s->c.max_newton, &hit);
+			hits += hit;
+			for (int q = 0; q < 4; q++) fr[q] += f[q];
+		}
+		for (int q = 0; q < 4; q++) rhs[4 * (size_t)c1 + q] -= 0.5 * l * fr[q];
+		if (c2 >= 0)
+			for (int q = 0; q < 4; q++) rhs[4 * (size_t)c2 + q] += 0.5 * l * fr[q];
+	}
+	for (int i = 0; i < nc; i++) {
+		double ct = steady ? 2.0 * cfl * m->S[i] / (lam_sum[i] + 1.0e-100) : tau;
+		if (ctau) ctau[i] = ct;
+		for (int q = 0; q < 4; q++) blocks[(size_t)i * 64 + 5 * q] += m->S[i] / ct;
+	}
+	s->cap_hits += hits;
+	free(g);
+	free(lam_sum);
+	return 0;
+}
+
+/* the face Jacobians alone: rows (rL,pL,uL,vL,rR,pR,uR,vR,nx,ny) ->
+ * (Roe r,u,v,p,c,H, A+ [16], A- [16]) with l = 1 */
+void or_roe_jacobians(int n, const double* in, double* out, double gam)
+{
+	for (int i = 0; i < n; i++) {
+		const double* r = &in[10 * (size_t)i];
+		double* o = &out[38 * (size_t)i];
+		prm pL, pR, a;
+		memset(&pL, 0, sizeof(pL));
+		memset(&pR, 0, sizeof(pR));
+		pL.r = r[0]; pL.p = r[1]; pL.u = r[2]; pL.v = r[3];
+		pR.r = r[4]; pR.p = r[5]; pR.u = r[6]; pR.v = r[7];
+		roe_average(&a, &pL, &pR, gam);
+		double Ap[4][4], Am[4][4];
+		split_face(Ap, Am, &a, r[8], r[9], 1.0, gam);
+		o[0] = a.r; o[1] = a.u; o[2] = a.v; o[3] = a.p; o[4] = a.cz; o[5] = a.E + a.p / a.r;
+		for (int k = 0; k < 16; k++) {
+			o[6 + k] = Ap[k / 4][k % 4];
+			o[22 + k] = Am[k / 4][k % 4];
+		}
+	}
 }
 
 /* ------------------------------------------------------------ API */

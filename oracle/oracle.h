@@ -72,6 +72,13 @@ void or_rim(int n, const double* in, double* out, int max_iter, int* iters);
 /* a numerical flux alone: rows (rL,pL,uL,vL,rR,pR,uR,vR,nx,ny) -> (fr,fu,fv,fe) */
 void or_flux(int kind, int n, const double* in, double* out, double gam);
 
+/* the implicit method's block rows for the current state (0, or 1 = pressure
+ * outlet present, 2 = flux not GODUNOV/LAX) */
+int  or_implicit_assemble(or_solver* s, int steady, double cfl, double tau, double* blocks, int* cols,
+                          double* rhs, double* ctau);
+/* face Jacobians: rows (rL,pL,uL,vL,rR,pR,uR,vR,nx,ny) -> (Roe r,u,v,p,c,H, A+[16], A-[16]) */
+void or_roe_jacobians(int n, const double* in, double* out, double gam);
+
 void or_set_threads(int n);
 
 #ifdef __cplusplus

@@ -68,6 +68,9 @@ def lib():
         L.or_rim.argtypes = [C.c_int, _dp, _dp, C.c_int, _ip]
         L.or_flux.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double]
         L.or_set_threads.argtypes = [C.c_int]
+        L.or_implicit_assemble.restype = C.c_int
+        L.or_implicit_assemble.argtypes = [vp, C.c_int, C.c_double, C.c_double, _dp, _ip, _dp, _dp]
+        L.or_roe_jacobians.argtypes = [C.c_int, _dp, _dp, C.c_double]
         _lib = L
     return _lib
 
@@ -187,6 +190,18 @@ class OracleSolver:
         fy = lib().or_wall_force(self._h, np.ascontiguousarray(edge_mask, np.int32), C.byref(fx))
         return fx.value, fy
 
+    def implicit_assemble(self, steady=False, cfl=0.5, tau=1.0e-3):
+        """The implicit method's block rows: (blocks[nc,4,4,4], cols[nc,4], rhs[nc,4], ctau[nc])."""
+        nc = self.nc
+        blocks = np.zeros(64 * nc)
+        cols = np.zeros(4 * nc, np.int32)
+        rhs = np.zeros(4 * nc)
+        ctau = np.zeros(nc)
+        rc = lib().or_implicit_assemble(self._h, int(bool(steady)), float(cfl), float(tau), blocks, cols, rhs, ctau)
+        if rc:
+            raise ValueError("implicit assembly: %s" % ("pressure outlet present" if rc == 1 else "flux must be GODUNOV or LAX"))
+        return blocks.reshape(nc, 4, 4, 4), cols.reshape(nc, 4), rhs.reshape(nc, 4), ctau
+
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
             _lib.or_solver_free(self._h)
@@ -207,3 +222,12 @@ def flux(kind, rows, gam=1.4):
     out = np.zeros((rows.shape[0], 4))
     lib().or_flux(kind, rows.shape[0], rows.ravel(), out.ravel(), gam)
     return out
+
+
+def roe_jacobians(rows, gam=1.4):
+    """Rows (rL,pL,uL,vL,rR,pR,uR,vR,nx,ny) -> (roe[n,6] = r,u,v,p,c,H; A+[n,4,4]; A-[n,4,4]) for l = 1."""
+    rows = np.ascontiguousarray(rows, dtype=np.float64)
+    n = rows.shape[0]
+    out = np.zeros((n, 38))
+    lib().or_roe_jacobians(n, rows.ravel(), out.ravel(), gam)
+    return out[:, :6], out[:, 6:22].reshape(n, 4, 4), out[:, 22:38].reshape(n, 4, 4)
