@@ -45,7 +45,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(case_fn, steps, threads=1):
+def cpu_baseline(case_fn, steps, threads=1, implicit=False):
     import oracle
     c = case_fn()
     t0 = time.perf_counter()
@@ -54,9 +54,15 @@ def cpu_baseline(case_fn, steps, threads=1):
     t0 = time.perf_counter()
     s.run(steps)
     dt = time.perf_counter() - t0
-    return dict(value=s.nc * steps / dt, unit="cell-updates/s", cores=threads, kind="port",
-                sample="%d step(s) of the same %d-cell workload on %d host thread(s) (C restatement, gcc -O2)"
-                % (steps, s.nc, threads), seconds=round(dt, 3), setup_seconds=round(setup, 3))
+    out = dict(value=s.nc * steps / dt, unit="cell-updates/s", cores=threads, kind="port",
+               sample="%d step(s) of the same %d-cell workload on %d host thread(s) (C restatement, gcc -O2)"
+               % (steps, s.nc, threads), seconds=round(dt, 3), setup_seconds=round(setup, 3))
+    if implicit:
+        # the implicit method's matrix fill, one serial edge-loop pass as the reference makes it
+        t0 = time.perf_counter()
+        s.implicit_assemble(steady=True, cfl=0.8)
+        out["implicit_assembly_s"] = round(time.perf_counter() - t0, 3)
+    return out
 
 
 def host_cpu():
@@ -252,6 +258,41 @@ def renumbering(n, steps=10):
     return out
 
 
+def implicit_assembly(s, reps=20):
+    """The implicit method's matrix fill (cfd2d_implicit_assemble) on the
+    context of the main workload: device-only assembly timed per kernel with
+    HIP events over reps calls, then one call that also copies the block rows
+    back to host memory (what a CPU solver is fed)."""
+    nc, ne = s.nc, s.ne
+    s.implicit_assemble(steady=True, cfl=0.8, outputs=False)
+    s.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        s.implicit_assemble(steady=True, cfl=0.8, outputs=False)
+    wall = (time.perf_counter() - t0) / reps
+    tim = s.timing_read()
+    s.timing(False)
+    alg = dict(impl_gradient=164 * nc, impl_edge=128 * ne + 96 * nc, impl_assemble=632 * nc + 56 * ne)
+    per = {}
+    total = 0.0
+    for k, b in alg.items():
+        ms, n = tim[k]
+        avg = ms / max(n, 1)
+        total += avg
+        per[k] = dict(avg_ms=round(avg, 5), launches=n, alg_bytes=b, gbs=round(b / (avg * 1e-3) / 1e9, 1))
+    bufs = s.implicit_buffers()  # page-locked host arrays, as a CPU solver would hold them
+    s.implicit_assemble(steady=True, cfl=0.8, out=bufs)
+    t0 = time.perf_counter()
+    B, C, R, T = s.implicit_assemble(steady=True, cfl=0.8, out=bufs)
+    with_copy = time.perf_counter() - t0
+    out_bytes = B.nbytes + C.nbytes + R.nbytes + T.nbytes
+    finite = bool(np.isfinite(B).all() and np.isfinite(R).all())
+    del B, C, R, T, bufs
+    return dict(cells=nc, reps=reps, kernels=per, device_ms=round(total, 4), wall_ms_device_only=round(wall * 1e3, 4),
+                cells_per_s=nc / (total * 1e-3), with_copy_ms=round(with_copy * 1e3, 2), output_bytes=out_bytes,
+                d2h_gbs=round(out_bytes / (with_copy - total * 1e-3) / 1e9, 1), finite=finite)
+
+
 def load_traffic(workload):
     """HBM bytes per launch of each kernel from the committed rocprofv3 PMC summary."""
     path = os.path.join(REPO, "profiles", "traffic.json")
@@ -282,6 +323,7 @@ def main():
     ap.add_argument("--badj-steps", type=int, default=300)
     ap.add_argument("--no-badj", action="store_true")
     ap.add_argument("--no-renumber", action="store_true")
+    ap.add_argument("--no-implicit", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -310,6 +352,7 @@ def main():
     tim = s.timing_read()
     s.timing(False)
     hits, nonfinite = s.status()
+    impl = None if args.no_implicit else implicit_assembly(s)
     s.close()
     value = s.nc * args.steps / dt
     per = {}
@@ -339,6 +382,9 @@ def main():
         "kernels": per,
         "newton_cap_hits": hits, "nonfinite": nonfinite,
     }
+    if impl:
+        out["implicit_assembly"] = impl
+        log("implicit assembly: %s" % impl)
     if not args.no_batch:
         out["batched"] = batched_vs_serial(args.batch_cases, args.batch_steps)
         log("batched: %s" % out["batched"])
@@ -352,7 +398,7 @@ def main():
         out["renumbering"] = renumbering(args.n)
         log("renumbering: %s" % out["renumbering"])
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(case_fn, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(case_fn, args.cpu_steps, implicit=not args.no_implicit)
         host = host_cpu()
         # the same restatement with OpenMP on the cores this process may use
         # (capped at 16, the host share of one GPU)

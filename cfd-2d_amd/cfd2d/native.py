@@ -18,7 +18,8 @@ _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 
 K_GRADIENT, K_EDGE_FLUX, K_UPDATE, K_REMEDIATE, K_TAU, K_PRIM = range(6)
-KERNEL_NAMES = ["gradient", "edge_flux", "update", "remediate", "tau", "prim", "implicit"]
+KERNEL_NAMES = ["gradient", "edge_flux", "update", "remediate", "tau", "prim", "impl_gradient", "impl_edge",
+                "impl_assemble"]
 
 
 class Params(C.Structure):
@@ -82,6 +83,8 @@ SIGNATURES = {
     "cfd2d_adjoint_objectives": (C.c_int, [C.c_void_p, _dp]),
     "cfd2d_implicit_assemble": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p]),
+    "cfd2d_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
+    "cfd2d_host_free": (C.c_int, [C.c_void_p]),
     "cfd2d_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "cfd2d_timing_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "cfd2d_ctx_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -323,21 +326,36 @@ class Context:
         check(lib().cfd2d_adjoint_objectives(self._h, out))
         return out
 
-    def implicit_assemble(self, steady=False, cfl=0.5, tau=1.0e-3):
+    def implicit_assemble(self, steady=False, cfl=0.5, tau=1.0e-3, outputs=True, out=None):
         """The implicit method's linear system for the current state as block
         rows: (blocks (nc, 4, 4, 4) with slot 0 the diagonal and slots 1..3 the
         faces in ascending edge order, cols (nc, 4) with -1 on boundary
-        faces, rhs (nc, 4), ctau (nc,))."""
+        faces, rhs (nc, 4), ctau (nc,)).  outputs=False assembles on the
+        device only (nothing copied back) and returns None; out = the four
+        arrays to fill (e.g. from implicit_buffers: page-locked)."""
         nc = self.nc
-        blocks = np.zeros(64 * nc)
-        cols = np.zeros(4 * nc, np.int32)
-        rhs = np.zeros(4 * nc)
-        ctau = np.zeros(nc)
+        if not outputs:
+            check(lib().cfd2d_implicit_assemble(self._h, int(bool(steady)), float(cfl), float(tau), None, None, None,
+                                                None))
+            return None
+        if out is None:
+            out = (np.zeros((nc, 4, 4, 4)), np.zeros((nc, 4), np.int32), np.zeros((nc, 4)), np.zeros(nc))
+        blocks, cols, rhs, ctau = out
+        for a, shape, dt in ((blocks, (nc, 4, 4, 4), np.float64), (cols, (nc, 4), np.int32), (rhs, (nc, 4), np.float64),
+                             (ctau, (nc,), np.float64)):
+            if a.shape != shape or a.dtype != dt or not a.flags.c_contiguous:
+                raise ValueError("implicit_assemble: output must be C-contiguous %s %s" % (shape, np.dtype(dt).name))
         vp = C.c_void_p
         check(lib().cfd2d_implicit_assemble(self._h, int(bool(steady)), float(cfl), float(tau),
                                             blocks.ctypes.data_as(vp), cols.ctypes.data_as(vp),
                                             rhs.ctypes.data_as(vp), ctau.ctypes.data_as(vp)))
-        return blocks.reshape(nc, 4, 4, 4), cols.reshape(nc, 4), rhs.reshape(nc, 4), ctau
+        return blocks, cols, rhs, ctau
+
+    def implicit_buffers(self):
+        """Page-locked output arrays for implicit_assemble(out=...)."""
+        nc = self.nc
+        return (pinned_array((nc, 4, 4, 4), np.float64), pinned_array((nc, 4), np.int32),
+                pinned_array((nc, 4), np.float64), pinned_array((nc,), np.float64))
 
     def timing(self, enable=True):
         check(lib().cfd2d_timing(self._h, 1 if enable else 0))
@@ -357,6 +375,28 @@ class Context:
 
     def __del__(self):
         self.close()
+
+
+class _Pinned:
+    """Page-locked host allocation (cfd2d_host_alloc), freed with its last array."""
+
+    def __init__(self, nbytes):
+        self.ptr = C.c_void_p()
+        check(lib().cfd2d_host_alloc(nbytes, C.byref(self.ptr)))
+        self.__array_interface__ = {"shape": (nbytes,), "typestr": "|u1", "data": (self.ptr.value, False),
+                                    "version": 3}
+
+    def __del__(self):
+        if self.ptr and _lib is not None:
+            _lib.cfd2d_host_free(self.ptr)
+            self.ptr = None
+
+
+def pinned_array(shape, dtype):
+    dt = np.dtype(dtype)
+    n = int(np.prod(shape)) * dt.itemsize
+    buf = _Pinned(max(n, 1))
+    return np.asarray(buf)[:n].view(dt).reshape(shape)
 
 
 class GpuSolver:

@@ -191,10 +191,12 @@ extern "C" int cfd2d_implicit_assemble(cfd2d_ctx* c, int steady, double cfl, dou
 	o.cfl = cfl;
 	o.tau = tau;
 	o.steady = steady;
-	rec_start(c, CFD2D_K_IMPLICIT);
+	rec_start(c, CFD2D_K_IMPL_GRADIENT);
 	if (c->mm) hipLaunchKernelGGL((k_impl_gradient<true>), dim3(grid_of(nc)), dim3(BLOCK), 0, c->st, c->b, c->ph);
 	else hipLaunchKernelGGL((k_impl_gradient<false>), dim3(grid_of(nc)), dim3(BLOCK), 0, c->st, c->b, c->ph);
+	rec_stop(c);
 	dim3 ge(grid_of(2 * c->ne));
+	rec_start(c, CFD2D_K_IMPL_EDGE);
 	if (c->p.flux == CFD2D_FLUX_LAX) {
 		if (c->mm) hipLaunchKernelGGL((k_impl_edge<true, CFD2D_FLUX_LAX>), ge, dim3(BLOCK), 0, c->st, c->b, c->ph);
 		else hipLaunchKernelGGL((k_impl_edge<false, CFD2D_FLUX_LAX>), ge, dim3(BLOCK), 0, c->st, c->b, c->ph);
@@ -202,6 +204,8 @@ extern "C" int cfd2d_implicit_assemble(cfd2d_ctx* c, int steady, double cfl, dou
 		if (c->mm) hipLaunchKernelGGL((k_impl_edge<true, CFD2D_FLUX_GODUNOV>), ge, dim3(BLOCK), 0, c->st, c->b, c->ph);
 		else hipLaunchKernelGGL((k_impl_edge<false, CFD2D_FLUX_GODUNOV>), ge, dim3(BLOCK), 0, c->st, c->b, c->ph);
 	}
+	rec_stop(c);
+	rec_start(c, CFD2D_K_IMPL_ASSEMBLE);
 	if (c->mm) hipLaunchKernelGGL((k_impl_assemble<true>), dim3(grid_of(nc)), dim3(BLOCK), 0, c->st, c->b, c->ph, o);
 	else hipLaunchKernelGGL((k_impl_assemble<false>), dim3(grid_of(nc)), dim3(BLOCK), 0, c->st, c->b, c->ph, o);
 	rec_stop(c);
@@ -211,6 +215,19 @@ extern "C" int cfd2d_implicit_assemble(cfd2d_ctx* c, int steady, double cfl, dou
 	if (cols) HIPCHK(hipMemcpyAsync(cols, o.cols, 4 * (size_t)nc * sizeof(int), hipMemcpyDeviceToHost, c->st));
 	if (rhs) HIPCHK(hipMemcpyAsync(rhs, o.rhs, 4 * (size_t)nc * sizeof(double), hipMemcpyDeviceToHost, c->st));
 	if (ctau) HIPCHK(hipMemcpyAsync(ctau, o.ctau, (size_t)nc * sizeof(double), hipMemcpyDeviceToHost, c->st));
-	HIPCHK(hipStreamSynchronize(c->st));
+	return cfd2d_sync(c);  // waits, and reports a Newton cap reached in the flux
+}
+
+extern "C" int cfd2d_host_alloc(size_t bytes, void** p)
+{
+	if (!p) return fail(CFD2D_ERR_ARG, "cfd2d_host_alloc: null");
+	*p = nullptr;
+	HIPCHK(hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault));
+	return CFD2D_OK;
+}
+
+extern "C" int cfd2d_host_free(void* p)
+{
+	if (p) HIPCHK(hipHostFree(p));
 	return CFD2D_OK;
 }

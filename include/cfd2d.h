@@ -50,7 +50,8 @@ enum { CFD2D_MAXMAT = 8 };
 /* kernels reported by cfd2d_timing_read */
 enum {
 	CFD2D_K_GRADIENT = 0, CFD2D_K_EDGE_FLUX = 1, CFD2D_K_UPDATE = 2, CFD2D_K_REMEDIATE = 3,
-	CFD2D_K_TAU = 4, CFD2D_K_PRIM = 5, CFD2D_K_IMPLICIT = 6, CFD2D_K_COUNT = 7
+	CFD2D_K_TAU = 4, CFD2D_K_PRIM = 5, CFD2D_K_IMPL_GRADIENT = 6, CFD2D_K_IMPL_EDGE = 7,
+	CFD2D_K_IMPL_ASSEMBLE = 8, CFD2D_K_COUNT = 9
 };
 
 /*
@@ -190,6 +191,11 @@ int  cfd2d_adjoint_objectives(cfd2d_ctx* c, double* J_case);
  * context's gradient and edge-flux buffers as scratch. */
 int  cfd2d_implicit_assemble(cfd2d_ctx* c, int steady, double cfl, double tau, double* blocks, int* cols,
                              double* rhs, double* ctau);
+/* Page-locked host memory (hipHostMalloc): output buffers of
+ * cfd2d_implicit_assemble placed here are copied at the link's full rate.
+ * No reference counterpart (the reference's matrix lives in host memory). */
+int  cfd2d_host_alloc(size_t bytes, void** p);
+int  cfd2d_host_free(void* p);
 
 #ifdef __cplusplus
 }
