@@ -112,51 +112,77 @@ struct ImplOut {
 	int steady;
 };
 
-// one block row per cell: RHS gathered in ascending edge order (the
-// reference's scatter order), then the face blocks in the same order
+// As store_records, for records of W doubles that lie `stride` doubles apart
+// (here one 4x4 block of each block row): eight lanes write one record's
+// 128 bytes.  The trailing barrier lets the caller reuse lds at once.
+template <int W>
+__device__ __forceinline__ void store_strided(double* base, size_t stride, int c0, int nvalid, const double* v,
+                                              double* lds)
+{
+	constexpr int PAD = W + 1;
+	int t = threadIdx.x;
+	for (int q = 0; q < W; q++) lds[t * PAD + q] = v[q];
+	__syncthreads();
+	int nd = nvalid * W;
+	for (int d = 2 * t; d < nd; d += 2 * BLOCK) {
+		int c = d / W, q = d % W;  // W even: d and d + 1 lie in one record
+		double* o = static_cast<double*>(__builtin_assume_aligned(base + (size_t)(c0 + c) * stride + q, 16));
+		o[0] = lds[c * PAD + q];
+		o[1] = lds[c * PAD + q + 1];
+	}
+	__syncthreads();
+}
+
+// One block row per cell, one lane per cell: the faces' blocks in ascending
+// edge order (the reference's order for the diagonal sum), each staged
+// through LDS so that a store instruction writes whole 128-byte blocks; the
+// RHS gathered in ascending edge order like the explicit residual.  Lanes
+// past the last cell compute on the block's first cell and store nothing.
 template <bool MM>
 __global__ __launch_bounds__(BLOCK) void k_impl_assemble(Bufs b, Phys ph, ImplOut o)
 {
-	int i = xcd_block() * BLOCK + threadIdx.x;
-	if (i >= b.nc) return;
+	__shared__ double lds[BLOCK * 17];
+	int c0 = xcd_block() * BLOCK;
+	int nvalid = min(BLOCK, b.nc - c0);
+	int i = c0 + (threadIdx.x < nvalid ? threadIdx.x : 0);
 	size_t n = b.nc;
-	double r[4];
-	gather_residual(b, i, r);
 	Prim pi = load_prim(b, i);
+	RoeSide si = roe_side(pi.r);
 	double diag[16];
 	for (int q = 0; q < 16; q++) diag[q] = 0.0;
 	double lam = 0.0;
-	double* blk = o.blocks + (size_t)i * 64;
-	int* col = o.cols + (size_t)i * 4;
-	col[0] = i;
+	int cl[4];
+	cl[0] = i;
 	for (int k = 0; k < 3; k++) {
 		int f = b.cface[k * n + i];
 		int e = f >> 1, side = f & 1;
 		int j = b.cnb[k * n + i];
 		// the edge's (c1, c2) centre states, as reconstruct(iEdge, L, R) gives them
 		Prim q = j >= 0 ? load_prim(b, j) : ghost_impl_of<MM>(b, ph, e, i, pi);
-		RoeAvg a = side ? roe_average(q, pi, ph.gam_flux) : roe_average(pi, q, ph.gam_flux);
+		RoeSide sq = roe_side(q.r);
+		RoeAvg a = side ? roe_average(q, sq, pi, si, ph.gam_flux) : roe_average(pi, si, q, sq, ph.gam_flux);
 		double l = b.el[e];
 		if (o.steady) lam += (sqrt(a.u * a.u + a.v * a.v) + a.c) * l;
 		double sn = side ? -1.0 : 1.0;
-		double Ap[16], Am[16];
-		split_jacobians(a, sn * b.enx[e], sn * b.eny[e], ph.gam_flux, l, Ap, Am);
-		for (int q2 = 0; q2 < 16; q2++) diag[q2] += Ap[q2];
-		double* off = blk + 16 * (k + 1);
-		if (j >= 0) {
-			for (int q2 = 0; q2 < 16; q2++) off[q2] = Am[q2];
-			col[k + 1] = j;
-		} else {
-			for (int q2 = 0; q2 < 16; q2++) off[q2] = 0.0;
-			col[k + 1] = -1;
-		}
+		double am[16];
+		split_face(a, sn * b.enx[e], sn * b.eny[e], ph.gam_flux, l, diag, am);
+		if (j < 0)
+			for (int q2 = 0; q2 < 16; q2++) am[q2] = 0.0;
+		cl[k + 1] = j;
+		store_strided<16>(o.blocks + 16 * (k + 1), 64, c0, nvalid, am, lds);
 	}
 	double S = b.S[i];
 	double ct = o.steady ? 2.0 * o.cfl * S / (lam + 1.0e-100) : o.tau;
 	for (int q = 0; q < 4; q++) diag[5 * q] += S / ct;
-	for (int q = 0; q < 16; q++) blk[q] = diag[q];
-	for (int q = 0; q < 4; q++) o.rhs[4 * (size_t)i + q] = r[q];
-	o.ctau[i] = ct;
+	store_strided<16>(o.blocks, 64, c0, nvalid, diag, lds);
+	if (threadIdx.x < nvalid) {
+		double r[4];
+		gather_residual(b, i, r);
+		st4(o.rhs, i, r);
+		int4* cp = reinterpret_cast<int4*>(o.cols) + i;
+		*cp = make_int4(cl[0], cl[1], cl[2], cl[3]);
+		o.ctau[i] = ct;
+	}
 }
 
 }  // namespace
@@ -184,10 +210,10 @@ extern "C" int cfd2d_implicit_assemble(cfd2d_ctx* c, int steady, double cfl, dou
 	}
 	char* base = (char*)c->impl_ws;
 	ImplOut o;
-	o.blocks = (double*)base;
+	o.blocks = (double*)base;  // 512 B per cell, then rhs 32 B, cols 16 B, ctau 8 B: every array aligned
 	o.rhs = o.blocks + 64 * (size_t)nc;
-	o.ctau = o.rhs + 4 * (size_t)nc;
-	o.cols = (int*)(o.ctau + nc);
+	o.cols = (int*)(o.rhs + 4 * (size_t)nc);
+	o.ctau = (double*)(o.cols + 4 * (size_t)nc);
 	o.cfl = cfl;
 	o.tau = tau;
 	o.steady = steady;

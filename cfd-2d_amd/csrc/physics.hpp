@@ -350,68 +350,92 @@ __device__ __forceinline__ Prim ghost_implicit(int type, const double* par, doub
 	return g;
 }
 
-// roe_orig (src/global.cpp:407-441) without the third velocity: Roe averages
-// of u, v and the total enthalpy; p from H, then c = sqrt(GAM p / r) and
-// H = E + p / r as FVM_TVD_IMPLICIT::calcRoeAverage forms them
-// (fvm_tvd_implicit.cpp:408-429)
-struct RoeAvg {
-	double r, u, v, p, c, H;
+// roe_orig (src/global.cpp:407-441) without the third velocity, with c and
+// H as FVM_TVD_IMPLICIT::calcRoeAverage forms them (fvm_tvd_implicit.cpp:
+// 408-429): Roe averages of u, v and the total enthalpy, p from H, c =
+// sqrt(GAM p / r), H = e + q^2/2 + p/r.  Each side comes with sqrt(rho) and
+// its reciprocal (a cell's are formed once and shared by its faces) and the
+// divisions of the reference's expressions become products with
+// reciprocals: the results differ from them by rounding only.
+struct RoeSide {
+	double s, is;  // sqrt(rho), 1/sqrt(rho)
 };
 
-__device__ __forceinline__ RoeAvg roe_average(const Prim& L, const Prim& R, double GAM)
+__device__ __forceinline__ RoeSide roe_side(double r)
+{
+	RoeSide x;
+	x.s = sqrt(r);
+	x.is = 1.0 / x.s;
+	return x;
+}
+
+struct RoeAvg {
+	double u, v, c, ic, H;  // ic = 1/c
+};
+
+__device__ __forceinline__ RoeAvg roe_average(const Prim& L, RoeSide sl, const Prim& R, RoeSide sr, double GAM)
 {
 	double AGAM = GAM - 1.0;
-	double sb = sqrt(L.r), se = sqrt(R.r);
-	double s_ = 1.0 / (sb + se);
+	double kH = GAM / AGAM;  // e + p/r = (p/r) GAM/(GAM-1)
+	double s_ = 1.0 / (sl.s + sr.s);
 	RoeAvg a;
-	a.r = sb * se;
-	a.u = (sb * L.u + se * R.u) * s_;
-	a.v = (sb * L.v + se * R.v) * s_;
-	double EB = L.p / (L.r * AGAM), EE = R.p / (R.r * AGAM);
-	double HB = EB + (L.u * L.u + L.v * L.v) * 0.5 + L.p / L.r;
-	double HE = EE + (R.u * R.u + R.v * R.v) * 0.5 + R.p / R.r;
-	double HI = (sb * HB + se * HE) * s_;
-	a.p = (HI - (a.u * a.u + a.v * a.v) * 0.5) * a.r * (GAM - 1.0) / GAM;
-	double e = a.p / (a.r * AGAM);
-	a.c = sqrt(GAM * a.p / a.r);
-	a.H = e + 0.5 * (a.u * a.u + a.v * a.v) + a.p / a.r;
+	a.u = (sl.s * L.u + sr.s * R.u) * s_;
+	a.v = (sl.s * L.v + sr.s * R.v) * s_;
+	double HB = L.p * (sl.is * sl.is) * kH + (L.u * L.u + L.v * L.v) * 0.5;
+	double HE = R.p * (sr.is * sr.is) * kH + (R.u * R.u + R.v * R.v) * 0.5;
+	double HI = (sl.s * HB + sr.s * HE) * s_;
+	double q2 = a.u * a.u + a.v * a.v;
+	double r = sl.s * sr.s;
+	double p = (HI - q2 * 0.5) * r * (AGAM / GAM);
+	double por = p * (sl.is * sr.is);  // p / r
+	a.c = sqrt(GAM * por);
+	a.ic = 1.0 / a.c;
+	a.H = por * kH + 0.5 * q2;
 	return a;
 }
 
 // A+ and A- of FVM_TVD_IMPLICIT (eigenValues, rightEigenVector,
 // leftEigenVector, calcAP, calcAM: fvm_tvd_implicit.cpp:335-406):
-// A+- = 0.5 R diag(max/min(lambda, 0)) L at the Roe state for normal n
-// (the reference's factor 0.5 included), each times scale
-__device__ __forceinline__ void split_jacobians(const RoeAvg& a, double nx, double ny, double GAM, double scale,
-                                                double* Ap, double* Am)
+// A+- = 0.5 R diag(max/min(lambda, 0)) L at the Roe state for normal n, the
+// reference's factor 0.5 included, each times l.  A+ is added to diag, A- is
+// written to am (both row-major 4x4); the sums over k of (R[i][k]
+// lambda_k) L[k][j] run in the reference's order (multMtx4,
+// fvm_tvd_implicit.cpp:264-277).
+__device__ __forceinline__ void split_face(const RoeAvg& a, double nx, double ny, double GAM, double l, double* diag,
+                                           double* am)
 {
-	double c = a.c, u = a.u, v = a.v, H = a.H;
+	double u = a.u, v = a.v, c = a.c, H = a.H;
 	double qn = u * nx + v * ny, q2 = u * u + v * v;
 	double lx = -ny, ly = nx, ql = u * lx + v * ly;
-	double g1 = GAM - 1.0, c2 = c * c;
+	double gc = (GAM - 1.0) * (a.ic * a.ic);  // g1 / c^2
+	double h = 0.5 * q2 * gc, qc = qn * a.ic, gu = u * gc, xc = nx * a.ic, gv = v * gc, yc = ny * a.ic;
+	double L[4][4] = {{0.5 * (h + qc), -0.5 * (gu + xc), -0.5 * (gv + yc), 0.5 * gc},
+	                  {1 - h, gu, gv, -gc},
+	                  {0.5 * (h - qc), -0.5 * (gu - xc), -0.5 * (gv - yc), 0.5 * gc},
+	                  {-ql, lx, ly, 0}};
 	double lam[4] = {qn - c, qn, qn + c, qn};
-	double Rm[4][4] = {{1.0, 1.0, 1.0, 0.0},
-	                   {u - c * nx, u, u + c * nx, lx},
-	                   {v - c * ny, v, v + c * ny, ly},
-	                   {H - qn * c, q2 / 2, H + qn * c, ql}};
-	double Lm[4][4] = {{0.5 * (0.5 * q2 * g1 / c2 + qn / c), -0.5 * (g1 * u / c2 + nx / c), -0.5 * (g1 * v / c2 + ny / c),
-	                    0.5 * g1 / c2},
-	                   {1 - 0.5 * q2 * g1 / c2, g1 * u / c2, g1 * v / c2, -g1 / c2},
-	                   {0.5 * (0.5 * q2 * g1 / c2 - qn / c), -0.5 * (g1 * u / c2 - nx / c), -0.5 * (g1 * v / c2 - ny / c),
-	                    0.5 * g1 / c2},
-	                   {-ql, lx, ly, 0}};
-	// (R diag(lam+-)) L, summed over k in the reference's order
-	for (int i = 0; i < 4; i++)
+	double R[4][4] = {{1.0, 1.0, 1.0, 0.0},
+	                  {u - c * nx, u, u + c * nx, lx},
+	                  {v - c * ny, v, v + c * ny, ly},
+	                  {H - qn * c, q2 / 2, H + qn * c, ql}};
+	double hl = 0.5 * l;
+#pragma unroll
+	for (int i = 0; i < 4; i++) {
+		double wp[4], wm[4];
+		for (int k = 0; k < 4; k++) {
+			wp[k] = R[i][k] * (lam[k] > 0 ? lam[k] : 0);
+			wm[k] = R[i][k] * (lam[k] < 0 ? lam[k] : 0);
+		}
 		for (int j = 0; j < 4; j++) {
 			double sp = 0.0, sm = 0.0;
 			for (int k = 0; k < 4; k++) {
-				double lp = lam[k] > 0 ? lam[k] : 0, lmn = lam[k] < 0 ? lam[k] : 0;
-				sp += (Rm[i][k] * lp) * Lm[k][j];
-				sm += (Rm[i][k] * lmn) * Lm[k][j];
+				sp += wp[k] * L[k][j];
+				sm += wm[k] * L[k][j];
 			}
-			Ap[4 * i + j] = sp * 0.5 * scale;
-			Am[4 * i + j] = sm * 0.5 * scale;
+			diag[4 * i + j] += sp * hl;
+			am[4 * i + j] = sm * hl;
 		}
+	}
 }
 
 // Extension: HLLC (Toro, Riemann Solvers, 3rd ed., 10.4-10.6), Davis wave speeds
