@@ -258,7 +258,7 @@ def renumbering(n, steps=10):
     return out
 
 
-def implicit_assembly(s, reps=20):
+def implicit_assembly(s, reps=20, traffic=None):
     """The implicit method's matrix fill (cfd2d_implicit_assemble) on the
     context of the main workload: device-only assembly timed per kernel with
     HIP events over reps calls, then one call that also copies the block rows
@@ -288,8 +288,13 @@ def implicit_assembly(s, reps=20):
     out_bytes = B.nbytes + C.nbytes + R.nbytes + T.nbytes
     finite = bool(np.isfinite(B).all() and np.isfinite(R).all())
     del B, C, R, T, bufs
-    return dict(cells=nc, reps=reps, kernels=per, device_ms=round(total, 4), wall_ms_device_only=round(wall * 1e3, 4),
-                cells_per_s=nc / (total * 1e-3), with_copy_ms=round(with_copy * 1e3, 2), output_bytes=out_bytes,
+    a = per["impl_assemble"]
+    roof = dict(bound="hbm", kernel="impl_assemble", achieved=a["gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(a["gbs"] / HBM_PEAK_GBS, 4), traffic=(traffic or {}).get("impl_assemble"),
+                alg_bytes_per_launch=a["alg_bytes"], avg_ms=a["avg_ms"])
+    return dict(cells=nc, reps=reps, kernels=per, roofline=roof, device_ms=round(total, 4),
+                wall_ms_device_only=round(wall * 1e3, 4), cells_per_s=nc / (total * 1e-3),
+                with_copy_ms=round(with_copy * 1e3, 2), output_bytes=out_bytes,
                 d2h_gbs=round(out_bytes / (with_copy - total * 1e-3) / 1e9, 1), finite=finite)
 
 
@@ -352,7 +357,7 @@ def main():
     tim = s.timing_read()
     s.timing(False)
     hits, nonfinite = s.status()
-    impl = None if args.no_implicit else implicit_assembly(s)
+    impl = None if args.no_implicit else implicit_assembly(s, traffic=load_traffic("periodic_ns_%d" % args.n))
     s.close()
     value = s.nc * args.steps / dt
     per = {}
