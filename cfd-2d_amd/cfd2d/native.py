@@ -86,6 +86,7 @@ SIGNATURES = {
     "cfd2d_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
     "cfd2d_host_free": (C.c_int, [C.c_void_p]),
     "cfd2d_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "cfd2d_timing_kernels": (C.c_int, [C.c_void_p, C.c_uint]),
     "cfd2d_timing_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "cfd2d_ctx_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cfd2d_ctx_stream": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
@@ -357,8 +358,12 @@ class Context:
         return (pinned_array((nc, 4, 4, 4), np.float64), pinned_array((nc, 4), np.int32),
                 pinned_array((nc, 4), np.float64), pinned_array((nc,), np.float64))
 
-    def timing(self, enable=True):
+    def timing(self, enable=True, kernels=None):
+        """Per-kernel HIP-event timing on or off (counters reset); kernels =
+        names from KERNEL_NAMES to time (default all)."""
         check(lib().cfd2d_timing(self._h, 1 if enable else 0))
+        mask = 0xFFFFFFFF if kernels is None else sum(1 << KERNEL_NAMES.index(k) for k in kernels)
+        check(lib().cfd2d_timing_kernels(self._h, mask))
 
     def timing_read(self):
         out = {}

@@ -615,6 +615,8 @@ struct cfd2d_ctx {
 	std::vector<double> case_tau;
 	// timing
 	bool timing = false;
+	unsigned timing_mask = ~0u;  // kernel ids (bit CFD2D_K_*) that get events
+	bool ev_open = false;        // rec_start recorded a start event
 	std::vector<hipEvent_t> ev0, ev1;
 	std::vector<int> ev_kernel;
 	int ev_used = 0;
@@ -679,7 +681,8 @@ int grid_of(int n) { return (n + BLOCK - 1) / BLOCK; }
 
 void rec_start(cfd2d_ctx* c, int k)
 {
-	if (!c->timing || c->capturing || c->ev_used >= EV_POOL) return;
+	c->ev_open = false;
+	if (!c->timing || c->capturing || c->ev_used >= EV_POOL || !((c->timing_mask >> k) & 1u)) return;
 	if ((int)c->ev0.size() <= c->ev_used) {
 		hipEvent_t a, z;
 		if (hipEventCreate(&a) != hipSuccess) return;
@@ -693,10 +696,13 @@ void rec_start(cfd2d_ctx* c, int k)
 	}
 	c->ev_kernel[c->ev_used] = k;
 	(void)hipEventRecord(c->ev0[c->ev_used], c->st);
+	c->ev_open = true;
 }
 
 void rec_stop(cfd2d_ctx* c)
 {
+	if (!c->ev_open) return;
+	c->ev_open = false;
 	if (!c->timing || c->capturing || c->ev_used >= EV_POOL || c->ev_used >= (int)c->ev0.size()) return;
 	(void)hipEventRecord(c->ev1[c->ev_used], c->st);
 	c->ev_used++;
@@ -1283,6 +1289,13 @@ int cfd2d_timing(cfd2d_ctx* c, int enable)
 		c->tot_ms[k] = 0.0;
 		c->launches[k] = 0;
 	}
+	return CFD2D_OK;
+}
+
+int cfd2d_timing_kernels(cfd2d_ctx* c, unsigned mask)
+{
+	if (!c) return fail(CFD2D_ERR_ARG, "null");
+	c->timing_mask = mask;
 	return CFD2D_OK;
 }
 

@@ -146,6 +146,9 @@ int  cfd2d_status(cfd2d_ctx* c, long long* newton_cap_hits, int* nonfinite);
 int  cfd2d_wall_force(cfd2d_ctx* c, const int* edge_mask, double* fx, double* fy);
 /* per-kernel HIP-event timing on the context stream */
 int  cfd2d_timing(cfd2d_ctx* c, int enable);
+/* which kernels get events while timing is on: bit k = CFD2D_K_* id
+ * (default all); fewer events disturb the stream less */
+int  cfd2d_timing_kernels(cfd2d_ctx* c, unsigned mask);
 int  cfd2d_timing_read(cfd2d_ctx* c, int kernel, double* total_ms, long long* launches);
 int  cfd2d_ctx_info(cfd2d_ctx* c, int* n_cells, int* n_edges, int* n_cases);
 int  cfd2d_ctx_stream(cfd2d_ctx* c, void** hip_stream);
