@@ -300,6 +300,16 @@ def implicit_assembly(s, reps=20, traffic=None):
                 d2h_gbs=round(out_bytes / (with_copy - total * 1e-3) / 1e9, 1), finite=finite)
 
 
+def load_sq(workload):
+    """VALU instructions per wave and FP64 issue fraction per kernel from the
+    committed rocprofv3 SQ-counter summary."""
+    try:
+        with open(os.path.join(REPO, "profiles", "sq_counters.json")) as f:
+            return json.load(f).get(workload) or {}
+    except (OSError, ValueError):
+        return {}
+
+
 def load_traffic(workload):
     """HBM bytes per launch of each kernel from the committed rocprofv3 PMC summary."""
     path = os.path.join(REPO, "profiles", "traffic.json")
@@ -384,6 +394,11 @@ def main():
     roof = dict(bound="hbm", kernel=dom, achieved=d["gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(d["gbs"] / HBM_PEAK_GBS, 4) if d["gbs"] else None,
                 traffic=(traffic or {}).get(dom), alg_bytes_per_launch=d["alg_bytes"], avg_ms=d["avg_ms"])
+    sq = load_sq("periodic_ns_%d" % args.n).get(dom)
+    if sq:
+        # what actually limits the kernel: FP64 instruction issue (profiles/periodic_ns_1024_sq_counters.md)
+        roof["valu_insts_per_wave"] = sq["valu_insts_per_wave"]
+        roof["valu_issue_frac_fp64"] = sq["valu_issue_frac_fp64"]
     step_bytes = 1968 * s.nc  # per cell-update: 3 stages x (464 + 128 e), e = 1.5
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
