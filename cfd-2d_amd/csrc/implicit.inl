@@ -45,26 +45,18 @@ __global__ __launch_bounds__(BLOCK) void k_impl_gradient(Bufs b, Phys ph)
 		size_t n = b.nc;
 		Prim pi = load_prim(b, i);
 		for (int k = 0; k < 3; k++) {
-			int f = b.cface[k * n + i];
-			int e = f >> 1, side = f & 1;
 			int j = b.cnb[k * n + i];
-			Prim q = j >= 0 ? load_prim(b, j) : ghost_impl_of<MM>(b, ph, e, i, pi);
-			double nx = b.enx[e], ny = b.eny[e], l = b.el[e];
+			Prim q = j >= 0 ? load_prim(b, j) : ghost_impl_of<MM>(b, ph, b.cface[k * n + i] >> 1, i, pi);
+			// n l out of cell i (the reference adds for c1 and subtracts for c2)
+			double fx = b.cfn[(size_t)(2 * k) * n + i], fy = b.cfn[(size_t)(2 * k + 1) * n + i];
 			double s[4] = {pi.r + q.r, pi.p + q.p, pi.u + q.u, pi.v + q.v};
 			for (int c = 0; c < 4; c++) {
-				double tx = s[c] / 2 * nx * l;
-				double ty = s[c] / 2 * ny * l;
-				if (side == 0) {
-					g[2 * c] += tx;
-					g[2 * c + 1] += ty;
-				} else {
-					g[2 * c] -= tx;
-					g[2 * c + 1] -= ty;
-				}
+				g[2 * c] += s[c] / 2 * fx;
+				g[2 * c + 1] += s[c] / 2 * fy;
 			}
 		}
-		double si = b.S[i];
-		for (int q = 0; q < 8; q++) g[q] = g[q] / si;
+		double isi = 1.0 / b.S[i];
+		for (int q = 0; q < 8; q++) g[q] *= isi;
 	}
 	store_records<8>(b.G, c0, min(BLOCK, b.nc - c0), g, lds);
 }

@@ -11,7 +11,8 @@
 //          8-byte gathers; dtS[nc] = tau_i / S_i,
 //          S[nc], faces cface[3][nc] = edge*2 + side in ascending edge order,
 //          cnb[3][nc] neighbour across each face (-1 on a boundary),
-//          lsqw[6][nc] least-squares weights, flags[nc]
+//          lsqw[6][nc] least-squares weights, cfn[6][nc] the faces'
+//          outward n l (Green-Gauss weights), flags[nc]
 //   edges  ec1/ec2 (ec2 = -(bc+1) on a boundary), enx, eny, el,
 //          edl[8][ne] Gauss-point offsets (c1: gp1 x,y gp2 x,y; c2: same),
 //          F[4][ne] = (flux at gp1 + flux at gp2) * l/2, written once
@@ -43,7 +44,7 @@ constexpr int NXCD = 8;
 
 struct Bufs {
 	int nc, ne, nblocks, nmat;
-	double *U, *U0, *P, *Po, *G, *dtS, *S, *lsqw, *loc, *R;
+	double *U, *U0, *P, *Po, *G, *dtS, *S, *lsqw, *cfn, *loc, *R;
 	int *cface, *cnb, *cmat;
 	unsigned* flags;
 	int* blockflag;
@@ -274,23 +275,16 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(Bufs b, Phys ph)
 			}
 		} else {
 			for (int k = 0; k < 3; k++) {
-				int e = face[k] >> 1, side = face[k] & 1;
-				double nx = b.enx[e], ny = b.eny[e], l = b.el[e];
+				// n l out of cell i (the reference adds for c1 and subtracts for c2)
+				double fx = b.cfn[(size_t)(2 * k) * n + i], fy = b.cfn[(size_t)(2 * k + 1) * n + i];
 				double s[4] = {pi.r + nb[k].r, pi.p + nb[k].p, pi.u + nb[k].u, pi.v + nb[k].v};
 				for (int q = 0; q < 4; q++) {
-					double tx = s[q] / 2 * nx * l;
-					double ty = s[q] / 2 * ny * l;
-					if (side == 0) {
-						g[2 * q] += tx;
-						g[2 * q + 1] += ty;
-					} else {
-						g[2 * q] -= tx;
-						g[2 * q + 1] -= ty;
-					}
+					g[2 * q] += s[q] / 2 * fx;
+					g[2 * q + 1] += s[q] / 2 * fy;
 				}
 			}
-			double si = b.S[i];
-			for (int q = 0; q < 8; q++) g[q] = g[q] / si;
+			double isi = 1.0 / b.S[i];
+			for (int q = 0; q < 8; q++) g[q] *= isi;
 		}
 		if (LIM) minmod_limit(b, i, face, pi, nb, g);
 	}
@@ -975,12 +969,16 @@ int cfd2d_ctx_create(const cfd2d_mesh* mesh, const cfd2d_params* p, int n_bc, co
 
 	int nc = m.nc, ne = m.ne;
 	std::vector<int> cface(3 * (size_t)nc), cnb(3 * (size_t)nc);
+	std::vector<double> cfn(6 * (size_t)nc);
 	for (int i = 0; i < nc; i++)
 		for (int k = 0; k < 3; k++) {
 			int e = m.cedges[3 * (size_t)i + k];
 			int side = (m.c1[e] == i) ? 0 : 1;
 			cface[(size_t)k * nc + i] = 2 * e + side;
 			cnb[(size_t)k * nc + i] = side == 0 ? m.c2[e] : m.c1[e];
+			double sg = side == 0 ? 1.0 : -1.0;
+			cfn[(size_t)(2 * k) * nc + i] = sg * (m.nx[e] * m.l[e]);
+			cfn[(size_t)(2 * k + 1) * nc + i] = sg * (m.ny[e] * m.l[e]);
 		}
 	std::vector<double> edl(8 * (size_t)ne, 0.0);
 	for (int e = 0; e < ne; e++) {
@@ -1011,7 +1009,7 @@ int cfd2d_ctx_create(const cfd2d_mesh* mesh, const cfd2d_params* p, int n_bc, co
 	if ((rc = dalloc(c, &b.U, n4)) || (rc = dalloc(c, &b.U0, n4)) || (rc = dalloc(c, &b.P, n4)) ||
 	    (rc = dalloc(c, &b.G, 2 * n4)) || (rc = dalloc(c, &b.dtS, nc)) || (rc = upload(c, &b.S, m.S)) ||
 	    (rc = dalloc(c, &b.loc, nc)) || (rc = dalloc(c, &b.R, n4)) || (rc = upload(c, &b.cface, cface)) ||
-	    (rc = upload(c, &b.cnb, cnb)) || (rc = upload(c, &b.cmat, cmat)) || (rc = dalloc(c, &b.flags, nc)) ||
+	    (rc = upload(c, &b.cnb, cnb)) || (rc = upload(c, &b.cfn, cfn)) || (rc = upload(c, &b.cmat, cmat)) || (rc = dalloc(c, &b.flags, nc)) ||
 	    (rc = dalloc(c, &b.blockflag, b.nblocks)) || (rc = dalloc(c, &b.anyflag, 2)) || (rc = upload(c, &b.ec1, m.c1)) ||
 	    (rc = upload(c, &b.ec2, ec2)) || (rc = upload(c, &b.enx, m.nx)) || (rc = upload(c, &b.eny, m.ny)) ||
 	    (rc = upload(c, &b.el, m.l)) || (rc = upload(c, &b.edl, edl)) || (rc = dalloc(c, &b.F, 4 * (size_t)ne)) ||
