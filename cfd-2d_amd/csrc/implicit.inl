@@ -89,10 +89,7 @@ __global__ __launch_bounds__(BLOCK) void k_impl_edge(Bufs b, Phys ph)
 	if (hit) atomicAdd(&b.err[0], 1ull);
 	double o[4];
 	for (int q = 0; q < 4; q++) o[q] = __shfl_xor(f[q], 1);
-	if (gp == 0) {
-		double hl = b.el[e] * 0.5;
-		for (int q = 0; q < 4; q++) b.F[q * ne + e] = (f[q] + o[q]) * hl;
-	}
+	store_edge_flux(b.F, e, gp, f, o, b.el[e] * 0.5);
 }
 
 struct ImplOut {

@@ -15,7 +15,8 @@
 //          outward n l (Green-Gauss weights), flags[nc]
 //   edges  ec1/ec2 (ec2 = -(bc+1) on a boundary), enx, eny, el,
 //          edl[8][ne] Gauss-point offsets (c1: gp1 x,y gp2 x,y; c2: same),
-//          F[4][ne] = (flux at gp1 + flux at gp2) * l/2, written once
+//          F[ne][4] = (flux at gp1 + flux at gp2) * l/2, one record per edge
+//          written once
 // Each RK stage is three passes: gradient (cell gather), edge flux (one
 // thread per edge, result written once to F), update (cell gather of its
 // three faces in ascending edge order, then the RK combination).  No
@@ -313,6 +314,20 @@ __device__ __forceinline__ void grad_T(const Bufs& b, int c, const Prim& q, cons
 	ty = (g[3] - pr * g[1]) * s;
 }
 
+// The lane pair's (f_gp1 + f_gp2) l/2 as the edge's 32-byte flux record: the
+// even lane adds its partner's flux to its own (the reference's order), the
+// odd lane the other way round (the same sums), and lane gp writes
+// components 2gp and 2gp+1, so the 64 lanes of a wave store 1 KB
+// contiguously
+__device__ __forceinline__ void store_edge_flux(double* F, int e, int gp, const double* f, const double* o, double hl)
+{
+	double a0 = gp ? f[2] + o[2] : f[0] + o[0];
+	double a1 = gp ? f[3] + o[3] : f[1] + o[1];
+	double* p = static_cast<double*>(__builtin_assume_aligned(F + 4 * (size_t)e + 2 * gp, 16));
+	p[0] = a0 * hl;
+	p[1] = a1 * hl;
+}
+
 // Edge flux pass, FVM_TVD::run edge loop + reconstruct + calcFlux
 // (src/methods/fvm_tvd.cpp:329-352, 646-691, 602-622).  Two adjacent lanes
 // serve one edge, one Gauss point each: each reconstructs (r, p, u, v)
@@ -393,25 +408,24 @@ __global__ __launch_bounds__(BLOCK) void k_edge_flux(Bufs b, Phys ph)
 	}
 	double o[4];
 	for (int q = 0; q < 4; q++) o[q] = __shfl_xor(f[q], 1);
-	if (gp == 0) {
-		double hl = b.el[e] * 0.5;
-		for (int q = 0; q < 4; q++) b.F[q * ne + e] = (f[q] + o[q]) * hl;
-	}
+	store_edge_flux(b.F, e, gp, f, o, b.el[e] * 0.5);
 }
 
 // residual of cell i: -F on c1's side, +F on c2's, ascending edge order
-// (the scatter of fvm_tvd.cpp:353-363 seen from the cell)
+// (the scatter of fvm_tvd.cpp:353-363 seen from the cell); an edge's flux is
+// one 32-byte record, two 16-byte loads
 __device__ __forceinline__ void gather_residual(const Bufs& b, int i, double* r)
 {
-	size_t n = b.nc, ne = b.ne;
+	size_t n = b.nc;
 	r[0] = r[1] = r[2] = r[3] = 0.0;
 	for (int k = 0; k < 3; k++) {
 		int f = b.cface[k * n + i];
-		int e = f >> 1;
+		double v[4];
+		ld4(b.F, f >> 1, v);
 		if (f & 1)
-			for (int q = 0; q < 4; q++) r[q] += b.F[q * ne + e];
+			for (int q = 0; q < 4; q++) r[q] += v[q];
 		else
-			for (int q = 0; q < 4; q++) r[q] -= b.F[q * ne + e];
+			for (int q = 0; q < 4; q++) r[q] -= v[q];
 	}
 }
 
@@ -1192,11 +1206,8 @@ int cfd2d_edge_fluxes(cfd2d_ctx* c, double* f)
 	do_edge(c);
 	int rc = check_launch("k_edge_flux");
 	if (rc) return rc;
-	std::vector<double> F(4 * (size_t)c->ne);
-	HIPCHK(hipMemcpyAsync(F.data(), c->b.F, F.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+	HIPCHK(hipMemcpyAsync(f, c->b.F, 4 * (size_t)c->ne * sizeof(double), hipMemcpyDeviceToHost, c->st));
 	HIPCHK(hipStreamSynchronize(c->st));
-	for (int e = 0; e < c->ne; e++)
-		for (int q = 0; q < 4; q++) f[4 * (size_t)e + q] = F[(size_t)q * c->ne + e];
 	return CFD2D_OK;
 }
 
