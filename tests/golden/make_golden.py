@@ -13,7 +13,13 @@ Cases (all reference-exact: Green-Gauss, exact Godunov, Heun RK2):
   steady   30x4  rectangle, STEADY=1 (local time steps), inlet
   bounds   50x5  rectangle, tight |u| limit -> LIM flags + remediation
   rect8k   200x20 rectangle (8000 cells), mild shock tube, 1000 steps
+  twomat   50x5  rectangle, inlet/outlet/slip walls, two materials (the
+           right half a gas with R = 0.5 and gamma = 4/3): the per-cell
+           material of the EOS, the ghosts and the time step
   rim      rim_orig on Toro's five tests and 4000 seeded random pairs
+
+`python3 make_golden.py twomat` regenerates only the named cases and keeps
+the others' manifest entries.
 
 The full Sod state (rho,p = 1,1 | 0.125,0.1) is not a case: with the
 reference's unlimited reconstruction it produces negative reconstructed
@@ -67,6 +73,12 @@ def case_specs():
              full=True),
         dict(name="rect8k", mesh=dict(nx=200, ny=20, lx=1.0, ly=0.2, seed=1), regions=regions_mild,
              boundaries=walls_outlets, steady=0, cfl=0.5, steps=[1, 10, 100, 1000], full=False),
+        dict(name="twomat", mesh=dict(nx=50, ny=5, lx=1.0, ly=0.2, seed=12),
+             materials=[dict(name="air", M=8.314472, Cp=3.5, K=0.0, ML=0.0),
+                        dict(name="heavy", M=16.628944, Cp=2.0, K=0.0, ML=0.0)],
+             regions=[dict(name="cells_left", material=0, Vx=0.3, Vy=0.0, T=1.0, P=1.0),
+                      dict(name="cells_right", material=1, Vx=0.2, Vy=0.0, T=1.4, P=0.8)],
+             boundaries=inlet, steady=0, cfl=0.5, steps=[1, 10, 100], full=True),
     ]
 
 
@@ -75,8 +87,8 @@ def build_case_inputs(spec, workdir):
     mesh_path = os.path.join(workdir, "mesh.unv")
     unv.write_unv(m, mesh_path)
     text = unv.task_xml("mesh.unv", regions=spec["regions"], boundaries=spec["boundaries"],
-                        steady=spec["steady"], cfl=spec["cfl"], step_max=max(spec["steps"]),
-                        limits=spec.get("limits"))
+                        materials=spec.get("materials"), steady=spec["steady"], cfl=spec["cfl"],
+                        step_max=max(spec["steps"]), limits=spec.get("limits"))
     with open(os.path.join(workdir, "task.xml"), "w") as f:
         f.write(text)
     with open(mesh_path, "rb") as f:
@@ -119,11 +131,14 @@ def run_case(spec, outdir, timeout=600):
             arrs["u_step%d" % s] = _load(wd, "u_step%d.f64" % s, np.float64).reshape(4, nc)
             arrs["flags_step%d" % s] = _load(wd, "flags_step%d.u32" % s, np.uint32)
         np.savez_compressed(os.path.join(outdir, spec["name"] + ".npz"), **arrs)
-        return dict(name=spec["name"], mesh=spec["mesh"], regions=spec["regions"],
+        out = dict(name=spec["name"], mesh=spec["mesh"], regions=spec["regions"],
                     boundaries=spec["boundaries"], steady=spec["steady"], cfl=spec["cfl"],
                     steps=spec["steps"], limits=spec.get("limits"), unv_sha256=sha,
                     geometry_sha256=geo_sha, counts=[nn, nc, ne], full=spec["full"],
                     task_sha256=hashlib.sha256(text.encode()).hexdigest())
+        if spec.get("materials"):
+            out["materials"] = spec["materials"]
+        return out
 
 
 TORO = [  # (rhoL, uL, pL, rhoR, uR, pR) -- Toro, Riemann Solvers, Table 4.1
@@ -166,12 +181,23 @@ def run_rim(outdir):
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("reference harness missing: run `make -C oracle ref` where the reference tree exists")
+    only = set(sys.argv[1:])
     manifest = dict(cases=[], generator="tests/golden/make_golden.py",
                     harness="oracle/_ref/ref_harness (reference sources, g++ -O2 -ffp-contract=off)")
+    old = {}
+    if only:
+        with open(os.path.join(HERE, "manifest.json")) as f:
+            prev = json.load(f)
+        old = {c["name"]: c for c in prev["cases"]}
+        manifest["rim"] = prev["rim"]
     for spec in case_specs():
+        if only and spec["name"] not in only:
+            manifest["cases"].append(old[spec["name"]])
+            continue
         print("case", spec["name"], flush=True)
         manifest["cases"].append(run_case(spec, HERE))
-    manifest["rim"] = run_rim(HERE)
+    if not only or "rim" in only:
+        manifest["rim"] = run_rim(HERE)
     with open(os.path.join(HERE, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1)
     print("done")
