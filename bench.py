@@ -30,14 +30,17 @@ DOMINANT = "edge_flux"     # the kernel timed with HIP events inside the timed r
 KERNEL_PASS_STEPS = 10     # untimed steps that time every other kernel
 METRIC = "FP64 cell-updates/s (2.1M-tri NS) + % HBM peak; batched/serial and adjoint/fwd ratios"
 
-# algorithmic bytes per launch (state arrays touched once per pass, neighbour
-# reuse perfect): gradient 156 B/cell with LSQ (164 with Green-Gauss), edge
-# flux 128 B/edge + 96 B/cell, update 212 B/cell
-def alg_bytes(kernel, nc, ne, lsq=True):
+# algorithmic bytes per launch (arrays touched once per pass, neighbour reuse
+# perfect): gradient 156 B/cell with LSQ (164 with Green-Gauss); edge flux
+# 128 B/edge (c1, c2, n, l, Gauss-point offsets read, the flux record written)
+# plus, reading stored gradients, 96 B/cell (P and G records) or, forming
+# them itself as unlimited steps do, 104 B/cell (P record, cface, cnb and the
+# 48 B of LSQ weights); update 212 B/cell
+def alg_bytes(kernel, nc, ne, lsq=True, fused=True):
     if kernel == "gradient":
         return (156 if lsq else 164) * nc
     if kernel == "edge_flux":
-        return 128 * ne + 96 * nc
+        return 128 * ne + ((104 if lsq else 112) if fused else 96) * nc
     if kernel == "update":
         return 212 * nc
     return 0
@@ -399,7 +402,10 @@ def main():
         # what actually limits the kernel: FP64 instruction issue (profiles/periodic_ns_1024_sq_counters.md)
         roof["valu_insts_per_wave"] = sq["valu_insts_per_wave"]
         roof["valu_issue_frac_fp64"] = sq["valu_issue_frac_fp64"]
-    step_bytes = 1968 * s.nc  # per cell-update: 3 stages x (464 + 128 e), e = 1.5
+    # per cell-update by the canonical three-pass stage (gradient, edge, update):
+    # 3 stages x (464 + 128 e), e = 1.5.  Unlimited steps here run two passes
+    # (the edge pass forms the gradients) and move 3 x (316 + 128 e) = 1 524 B.
+    step_bytes = 1968 * s.nc
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True,

@@ -87,6 +87,7 @@ SIGNATURES = {
     "cfd2d_host_free": (C.c_int, [C.c_void_p]),
     "cfd2d_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "cfd2d_timing_kernels": (C.c_int, [C.c_void_p, C.c_uint]),
+    "cfd2d_fuse_gradient": (C.c_int, [C.c_void_p, C.c_int]),
     "cfd2d_timing_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "cfd2d_ctx_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cfd2d_ctx_stream": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
@@ -357,6 +358,11 @@ class Context:
         nc = self.nc
         return (pinned_array((nc, 4, 4, 4), np.float64), pinned_array((nc, 4), np.int32),
                 pinned_array((nc, 4), np.float64), pinned_array((nc,), np.float64))
+
+    def fuse_gradient(self, enable=True):
+        """Unlimited steps: the edge pass forms the gradients (True, default)
+        or reads a separate gradient pass's (False); the same results."""
+        check(lib().cfd2d_fuse_gradient(self._h, 1 if enable else 0))
 
     def timing(self, enable=True, kernels=None):
         """Per-kernel HIP-event timing on or off (counters reset); kernels =

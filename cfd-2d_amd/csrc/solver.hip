@@ -246,7 +246,45 @@ __device__ __forceinline__ void minmod_limit(const Bufs& b, int i, const int* fa
 // Green-Gauss gradients, FVM_TVD::calcGrad (src/methods/fvm_tvd.cpp:242-301)
 // as a gather: the faces of cell i in ascending edge order, each adding
 // (phi_c1 + phi_c2)/2 * n * l with + on c1's side and - on c2's, then / S.
-// LSQ (extension): grad = sum_k w_k (phi_k - phi_i).
+// LSQ (extension): grad = sum_k w_k (phi_k - phi_i).  pi, face and nb (the
+// neighbour states, ghosts on a boundary) are returned for the limiter.
+template <bool MM, bool LSQ>
+__device__ __forceinline__ void cell_gradient(const Bufs& b, const Phys& ph, int i, Prim& pi, int* face, Prim* nb,
+                                              double* g)
+{
+	size_t n = b.nc;
+	pi = load_prim(b, i);
+	for (int q = 0; q < 8; q++) g[q] = 0.0;
+	for (int k = 0; k < 3; k++) {
+		face[k] = b.cface[k * n + i];
+		int j = b.cnb[k * n + i];
+		if (j >= 0) nb[k] = load_prim(b, j);
+		else nb[k] = ghost_of<MM>(b, ph, face[k] >> 1, i, pi, pi);
+	}
+	if (LSQ) {
+		for (int k = 0; k < 3; k++) {
+			double wx = b.lsqw[(size_t)(2 * k) * n + i], wy = b.lsqw[(size_t)(2 * k + 1) * n + i];
+			double dphi[4] = {nb[k].r - pi.r, nb[k].p - pi.p, nb[k].u - pi.u, nb[k].v - pi.v};
+			for (int q = 0; q < 4; q++) {
+				g[2 * q] += wx * dphi[q];
+				g[2 * q + 1] += wy * dphi[q];
+			}
+		}
+	} else {
+		for (int k = 0; k < 3; k++) {
+			// n l out of cell i (the reference adds for c1 and subtracts for c2)
+			double fx = b.cfn[(size_t)(2 * k) * n + i], fy = b.cfn[(size_t)(2 * k + 1) * n + i];
+			double s[4] = {pi.r + nb[k].r, pi.p + nb[k].p, pi.u + nb[k].u, pi.v + nb[k].v};
+			for (int q = 0; q < 4; q++) {
+				g[2 * q] += s[q] / 2 * fx;
+				g[2 * q + 1] += s[q] / 2 * fy;
+			}
+		}
+		double isi = 1.0 / b.S[i];
+		for (int q = 0; q < 8; q++) g[q] *= isi;
+	}
+}
+
 template <bool MM, bool LSQ, bool LIM>
 __global__ __launch_bounds__(BLOCK) void k_gradient(Bufs b, Phys ph)
 {
@@ -255,63 +293,47 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(Bufs b, Phys ph)
 	int i = c0 + threadIdx.x;
 	double g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 	if (i < b.nc) {
-		size_t n = b.nc;
-		Prim pi = load_prim(b, i);
+		Prim pi;
 		int face[3];
 		Prim nb[3];
-		for (int k = 0; k < 3; k++) {
-			face[k] = b.cface[k * n + i];
-			int j = b.cnb[k * n + i];
-			if (j >= 0) nb[k] = load_prim(b, j);
-			else nb[k] = ghost_of<MM>(b, ph, face[k] >> 1, i, pi, pi);
-		}
-		if (LSQ) {
-			for (int k = 0; k < 3; k++) {
-				double wx = b.lsqw[(size_t)(2 * k) * n + i], wy = b.lsqw[(size_t)(2 * k + 1) * n + i];
-				double dphi[4] = {nb[k].r - pi.r, nb[k].p - pi.p, nb[k].u - pi.u, nb[k].v - pi.v};
-				for (int q = 0; q < 4; q++) {
-					g[2 * q] += wx * dphi[q];
-					g[2 * q + 1] += wy * dphi[q];
-				}
-			}
-		} else {
-			for (int k = 0; k < 3; k++) {
-				// n l out of cell i (the reference adds for c1 and subtracts for c2)
-				double fx = b.cfn[(size_t)(2 * k) * n + i], fy = b.cfn[(size_t)(2 * k + 1) * n + i];
-				double s[4] = {pi.r + nb[k].r, pi.p + nb[k].p, pi.u + nb[k].u, pi.v + nb[k].v};
-				for (int q = 0; q < 4; q++) {
-					g[2 * q] += s[q] / 2 * fx;
-					g[2 * q + 1] += s[q] / 2 * fy;
-				}
-			}
-			double isi = 1.0 / b.S[i];
-			for (int q = 0; q < 8; q++) g[q] *= isi;
-		}
+		cell_gradient<MM, LSQ>(b, ph, i, pi, face, nb, g);
 		if (LIM) minmod_limit(b, i, face, pi, nb, g);
 	}
 	store_records<8>(b.G, c0, min(BLOCK, b.nc - c0), g, lds);
 }
 
-__device__ __forceinline__ void recon(Prim& q, const Bufs& b, int c, double dx, double dy)
+// linear reconstruction at offset (dx, dy) with the gradient record g
+__device__ __forceinline__ void recon_g(Prim& q, const double* g, double dx, double dy)
 {
-	double g[8];
-	ld8(b.G, c, g);
 	q.r += g[0] * dx + g[1] * dy;
 	q.p += g[2] * dx + g[3] * dy;
 	q.u += g[4] * dx + g[5] * dy;
 	q.v += g[6] * dx + g[7] * dy;
 }
 
-__device__ __forceinline__ void grad_T(const Bufs& b, int c, const Prim& q, const MatV& m, double& tx, double& ty)
+__device__ __forceinline__ void recon(Prim& q, const Bufs& b, int c, double dx, double dy)
+{
+	double g[8];
+	ld8(b.G, c, g);
+	recon_g(q, g, dx, dy);
+}
+
+// grad T from the gradient record's (rx ry px py) g
+__device__ __forceinline__ void grad_T_g(const double* g, const Prim& q, const MatV& m, double& tx, double& ty)
 {
 	// T = p / (r Rg), Rg = gR / M: grad T = (grad p - (p/r) grad r) / (r Rg)
-	double g[4];
-	ld4(b.G, 2 * (size_t)c, g);  // rx ry px py
 	double ir = 1.0 / q.r;
 	double pr = q.p * ir;
 	double s = ir * m.M * (1.0 / GAS_R);
 	tx = (g[2] - pr * g[0]) * s;
 	ty = (g[3] - pr * g[1]) * s;
+}
+
+__device__ __forceinline__ void grad_T(const Bufs& b, int c, const Prim& q, const MatV& m, double& tx, double& ty)
+{
+	double g[4];
+	ld4(b.G, 2 * (size_t)c, g);  // rx ry px py
+	grad_T_g(g, q, m, tx, ty);
 }
 
 // The lane pair's (f_gp1 + f_gp2) l/2 as the edge's 32-byte flux record: the
@@ -336,7 +358,16 @@ __device__ __forceinline__ void store_edge_flux(double* F, int e, int gp, const 
 // viscous flux (extension); the even lane adds its partner's flux to its own
 // (f1 + f2, the reference's order) and writes (f1 + f2) * l/2 once.  Halving
 // the live state per lane doubles the waves a SIMD can hold.
-template <bool MM, int FLUX, bool VISC>
+//
+// GR = 0 reads the gradients the gradient pass stored; GR = 1 (LSQ) or 2
+// (Green-Gauss) has the pair form them itself: lane 0 the gradient of c1,
+// lane 1 that of c2 (c1's on a boundary), with the gradient pass's own code,
+// and the lanes swap them.  Every cell's gradient is then formed by each of
+// its edges (three times) instead of once, but the gradient pass, its 64-byte
+// record per cell written and read back, and its launch disappear; the extra
+// arithmetic runs inside a pass that is bound by FP64 issue anyway, beside its
+// memory traffic.  Results are the same bits as with GR = 0.
+template <bool MM, int FLUX, bool VISC, int GR>
 __global__ __launch_bounds__(BLOCK) void k_edge_flux(Bufs b, Phys ph)
 {
 	int t = xcd_block() * BLOCK + threadIdx.x;
@@ -345,14 +376,30 @@ __global__ __launch_bounds__(BLOCK) void k_edge_flux(Bufs b, Phys ph)
 	size_t ne = b.ne;
 	int c1 = b.ec1[e], c2 = b.ec2[e];
 	double nx = b.enx[e], ny = b.eny[e];
+	bool two = gp == 1 && c2 >= 0;
+	int cs = two ? c2 : c1;  // the cell whose shared work this lane does
+	double gs[8], go[8];     // gradients of cs, and of the partner's cell
+	Prim ps;
+	if (GR) {
+		int face[3];
+		Prim nb[3];
+		cell_gradient<MM, GR == 1>(b, ph, cs, ps, face, nb, gs);
+		for (int q = 0; q < 8; q++) go[q] = __shfl_xor(gs[q], 1);
+	}
+	const double* g1 = gp ? go : gs;  // c1's gradients (GR != 0)
+	const double* g2 = gp ? gs : go;  // c2's (c1's on a boundary)
 	Prim p1 = load_prim(b, c1);
 	Prim L = p1, R;
-	recon(L, b, c1, b.edl[(size_t)(2 * gp) * ne + e], b.edl[(size_t)(2 * gp + 1) * ne + e]);
+	double d1x = b.edl[(size_t)(2 * gp) * ne + e], d1y = b.edl[(size_t)(2 * gp + 1) * ne + e];
+	if (GR) recon_g(L, g1, d1x, d1y);
+	else recon(L, b, c1, d1x, d1y);
 	Prim p2;
 	if (c2 >= 0) {
 		p2 = load_prim(b, c2);
 		R = p2;
-		recon(R, b, c2, b.edl[(size_t)(4 + 2 * gp) * ne + e], b.edl[(size_t)(5 + 2 * gp) * ne + e]);
+		double d2x = b.edl[(size_t)(4 + 2 * gp) * ne + e], d2y = b.edl[(size_t)(5 + 2 * gp) * ne + e];
+		if (GR) recon_g(R, g2, d2x, d2y);
+		else recon(R, b, c2, d2x, d2y);
 	} else {
 		p2 = p1;
 		R = ghost_of<MM>(b, ph, e, c1, L, p1);
@@ -360,30 +407,33 @@ __global__ __launch_bounds__(BLOCK) void k_edge_flux(Bufs b, Phys ph)
 	// Viscous terms first (only sx, sy and the energy term stay live through
 	// the Riemann solve).  Face gradient = mean of the two cell gradients (the
 	// cell's own on a boundary); face velocity = mean of the reconstructed
-	// states.  The pair's lanes share the cell work: lane 0 loads c1's
+	// states.  The pair's lanes share the cell work: lane 0 takes c1's
 	// velocity gradient and forms its grad T, lane 1 does c2's (c1's on a
 	// boundary), and they swap them; the sums are the same as one lane
 	// forming both.
 	double sx = 0.0, sy = 0.0, ev = 0.0;
 	if (VISC) {
 		MatV m = mat_of<MM>(b, ph, c1);
-		bool two = gp == 1 && c2 >= 0;
-		int cs = two ? c2 : c1;
 		double own[6];
-		ld4(b.G, 2 * (size_t)cs + 1, own);  // ux uy vx vy
-		grad_T(b, cs, two ? p2 : p1, mat_of<MM>(b, ph, cs), own[4], own[5]);
+		if (GR) {
+			for (int q = 0; q < 4; q++) own[q] = gs[4 + q];  // ux uy vx vy
+			grad_T_g(gs, ps, mat_of<MM>(b, ph, cs), own[4], own[5]);
+		} else {
+			ld4(b.G, 2 * (size_t)cs + 1, own);  // ux uy vx vy
+			grad_T(b, cs, two ? p2 : p1, mat_of<MM>(b, ph, cs), own[4], own[5]);
+		}
 		double oth[6];
 		for (int q = 0; q < 6; q++) oth[q] = __shfl_xor(own[q], 1);
-		const double* g1 = gp ? oth : own;  // c1's values
-		const double* g2 = gp ? own : oth;  // c2's values (c1's on a boundary)
-		double ux = g1[0], uy = g1[1], vx = g1[2], vy = g1[3], tx = g1[4], ty = g1[5];
+		const double* v1 = gp ? oth : own;  // c1's values
+		const double* v2 = gp ? own : oth;  // c2's values (c1's on a boundary)
+		double ux = v1[0], uy = v1[1], vx = v1[2], vy = v1[3], tx = v1[4], ty = v1[5];
 		if (c2 >= 0) {
-			ux = 0.5 * (ux + g2[0]);
-			uy = 0.5 * (uy + g2[1]);
-			vx = 0.5 * (vx + g2[2]);
-			vy = 0.5 * (vy + g2[3]);
-			tx = 0.5 * (tx + g2[4]);
-			ty = 0.5 * (ty + g2[5]);
+			ux = 0.5 * (ux + v2[0]);
+			uy = 0.5 * (uy + v2[1]);
+			vx = 0.5 * (vx + v2[2]);
+			vy = 0.5 * (vy + v2[3]);
+			tx = 0.5 * (tx + v2[4]);
+			ty = 0.5 * (ty + v2[5]);
 		}
 		double div = ux + vy;
 		double txx = m.mu * (2.0 * ux - (2.0 / 3.0) * div);
@@ -624,6 +674,7 @@ struct cfd2d_ctx {
 	// timing
 	bool timing = false;
 	unsigned timing_mask = ~0u;  // kernel ids (bit CFD2D_K_*) that get events
+	bool fuse_gradient = true;   // unlimited steps: the edge pass forms the gradients
 	bool ev_open = false;        // rec_start recorded a start event
 	std::vector<hipEvent_t> ev0, ev1;
 	std::vector<int> ev_kernel;
@@ -751,22 +802,43 @@ void launch_gradient(cfd2d_ctx* c)
 	else LAUNCH(c, CFD2D_K_GRADIENT, (k_gradient<MM, false, false>), c->nc, c->b, c->ph);
 }
 
-template <bool MM, bool VISC>
+template <bool MM, bool VISC, int GR>
 void launch_edge_v(cfd2d_ctx* c)
 {
 	switch (c->p.flux) {
-	case CFD2D_FLUX_LAX: LAUNCH(c, CFD2D_K_EDGE_FLUX, (k_edge_flux<MM, CFD2D_FLUX_LAX, VISC>), 2 * c->ne, c->b, c->ph); break;
-	case CFD2D_FLUX_HLLC: LAUNCH(c, CFD2D_K_EDGE_FLUX, (k_edge_flux<MM, CFD2D_FLUX_HLLC, VISC>), 2 * c->ne, c->b, c->ph); break;
-	case CFD2D_FLUX_ROE: LAUNCH(c, CFD2D_K_EDGE_FLUX, (k_edge_flux<MM, CFD2D_FLUX_ROE, VISC>), 2 * c->ne, c->b, c->ph); break;
-	default: LAUNCH(c, CFD2D_K_EDGE_FLUX, (k_edge_flux<MM, CFD2D_FLUX_GODUNOV, VISC>), 2 * c->ne, c->b, c->ph); break;
+	case CFD2D_FLUX_LAX: LAUNCH(c, CFD2D_K_EDGE_FLUX, (k_edge_flux<MM, CFD2D_FLUX_LAX, VISC, GR>), 2 * c->ne, c->b, c->ph); break;
+	case CFD2D_FLUX_HLLC: LAUNCH(c, CFD2D_K_EDGE_FLUX, (k_edge_flux<MM, CFD2D_FLUX_HLLC, VISC, GR>), 2 * c->ne, c->b, c->ph); break;
+	case CFD2D_FLUX_ROE: LAUNCH(c, CFD2D_K_EDGE_FLUX, (k_edge_flux<MM, CFD2D_FLUX_ROE, VISC, GR>), 2 * c->ne, c->b, c->ph); break;
+	default: LAUNCH(c, CFD2D_K_EDGE_FLUX, (k_edge_flux<MM, CFD2D_FLUX_GODUNOV, VISC, GR>), 2 * c->ne, c->b, c->ph); break;
 	}
 }
 
+// the edge pass reading the stored gradients
 template <bool MM>
 void launch_edge(cfd2d_ctx* c)
 {
-	if (c->p.viscous) launch_edge_v<MM, true>(c);
-	else launch_edge_v<MM, false>(c);
+	if (c->p.viscous) launch_edge_v<MM, true, 0>(c);
+	else launch_edge_v<MM, false, 0>(c);
+}
+
+// gradient + edge passes of one stage; without a limiter the edge pass forms
+// the gradients itself (nothing reads the stored gradients afterwards)
+template <bool MM>
+void launch_gradient_edge(cfd2d_ctx* c)
+{
+	if (c->p.limiter != CFD2D_LIMITER_NONE || !c->fuse_gradient) {
+		launch_gradient<MM>(c);
+		launch_edge<MM>(c);
+		return;
+	}
+	bool lsq = c->p.gradient == CFD2D_GRAD_LSQ;
+	if (c->p.viscous) {
+		if (lsq) launch_edge_v<MM, true, 1>(c);
+		else launch_edge_v<MM, true, 2>(c);
+	} else {
+		if (lsq) launch_edge_v<MM, false, 1>(c);
+		else launch_edge_v<MM, false, 2>(c);
+	}
 }
 
 template <bool MM>
@@ -787,15 +859,13 @@ void enqueue_step(cfd2d_ctx* c)
 	if (c->p.rk_stages == 3) {
 		const int modes[3] = {UPD_FIRST, UPD_RK3_MID, UPD_RK3_LAST};
 		for (int s = 0; s < 3; s++) {
-			launch_gradient<MM>(c);
-			launch_edge<MM>(c);
+			launch_gradient_edge<MM>(c);
 			launch_update<MM>(c, modes[s]);
 		}
 	} else {
 		const int modes[2] = {UPD_FIRST, UPD_RK2_LAST};
 		for (int s = 0; s < 2; s++) {
-			launch_gradient<MM>(c);
-			launch_edge<MM>(c);
+			launch_gradient_edge<MM>(c);
 			launch_update<MM>(c, modes[s]);
 		}
 	}
@@ -1297,6 +1367,21 @@ int cfd2d_timing(cfd2d_ctx* c, int enable)
 	for (int k = 0; k < CFD2D_K_COUNT; k++) {
 		c->tot_ms[k] = 0.0;
 		c->launches[k] = 0;
+	}
+	return CFD2D_OK;
+}
+
+int cfd2d_fuse_gradient(cfd2d_ctx* c, int enable)
+{
+	if (!c) return fail(CFD2D_ERR_ARG, "null");
+	c->fuse_gradient = enable != 0;
+	if (c->graph_exec) {  // the recorded step holds the previous choice
+		(void)hipGraphExecDestroy(c->graph_exec);
+		c->graph_exec = nullptr;
+	}
+	if (c->graph) {
+		(void)hipGraphDestroy(c->graph);
+		c->graph = nullptr;
 	}
 	return CFD2D_OK;
 }

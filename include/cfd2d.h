@@ -146,6 +146,10 @@ int  cfd2d_status(cfd2d_ctx* c, long long* newton_cap_hits, int* nonfinite);
 int  cfd2d_wall_force(cfd2d_ctx* c, const int* edge_mask, double* fx, double* fy);
 /* per-kernel HIP-event timing on the context stream */
 int  cfd2d_timing(cfd2d_ctx* c, int enable);
+/* Steps without a limiter let the edge pass form the cell gradients itself
+ * (default, enable = 1) instead of reading those of a separate gradient pass
+ * (enable = 0); both give the same bits.  No reference counterpart. */
+int  cfd2d_fuse_gradient(cfd2d_ctx* c, int enable);
 /* which kernels get events while timing is on: bit k = CFD2D_K_* id
  * (default all); fewer events disturb the stream less */
 int  cfd2d_timing_kernels(cfd2d_ctx* c, unsigned mask);

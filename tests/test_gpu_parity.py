@@ -116,6 +116,22 @@ def test_deterministic_and_graph_replay():
     assert np.array_equal(outs[0], outs[2])
 
 
+@pytest.mark.parametrize("opts", [dict(), dict(gradient="LSQ", rk_stages=3, flux="HLLC"),
+                                  dict(viscous=True, gradient="LSQ", rk_stages=3)], ids=str)
+def test_fused_gradient_edge_pass_is_bitwise(opts):
+    """The edge pass that forms the cell gradients itself gives the same bits
+    as the separate gradient pass (inlet, walls, outlet; inviscid and NS)."""
+    case = gc.case("inlet", **opts)
+    outs = []
+    for fuse in (True, False):
+        s = native.GpuSolver(case)
+        s.fuse_gradient(fuse)
+        s.run(25)
+        outs.append(s.state())
+        s.close()
+    assert np.array_equal(outs[0], outs[1])
+
+
 EXT = [  # (full Sod state?, options): unlimited variants on the mild tube, limited ones on Sod
     (False, dict(flux="LAX")), (False, dict(flux="HLLC")), (False, dict(flux="ROE")),
     (False, dict(gradient="LSQ")), (False, dict(rk_stages=3)), (False, dict(cfl_mode="SPECTRAL", cfl=0.4)),
