@@ -667,8 +667,9 @@ void record_steps(cfd2d_ctx* c, AdjWork* A, int nsteps)
 			c->b.P = A->Pst + t * n4;
 			c->b.Po = A->Pst + (t + 1) * n4;
 			c->b.G = A->Gst + t * n8;
-			launch_gradient<MM>(c);
-			launch_edge<MM>(c);
+			c->ph.store_g = 1;  // a fused edge pass stores the stage's gradients too
+			launch_gradient_edge<MM>(c);
+			c->ph.store_g = 0;
 			launch_update<MM>(c, S == 3 ? modes3[st] : modes2[st]);
 		}
 		c->b.P = A->Pst + t * n4;

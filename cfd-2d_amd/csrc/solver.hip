@@ -65,6 +65,7 @@ struct Phys {
 	double gam_flux, igm1, cfl;  // igm1 = 1 / (gam_flux - 1)
 	double ro_min, ro_max, p_min, p_max, u_max;
 	int max_newton, cfl_mode, steady;
+	int store_g;  // the fused edge pass also stores the gradient records (adjoint recording)
 };
 
 // Block index remapped so that consecutive index ranges stay on one XCD
@@ -385,6 +386,12 @@ __global__ __launch_bounds__(BLOCK) void k_edge_flux(Bufs b, Phys ph)
 		Prim nb[3];
 		cell_gradient<MM, GR == 1>(b, ph, cs, ps, face, nb, gs);
 		for (int q = 0; q < 8; q++) go[q] = __shfl_xor(gs[q], 1);
+		// when the records are wanted, each cell's is stored by the lane that
+		// formed it on the cell's first face (lowest edge index): once
+		if (ph.store_g && (face[0] >> 1) == e && !(gp == 1 && c2 < 0)) {
+			double* o = static_cast<double*>(__builtin_assume_aligned(b.G + 8 * (size_t)cs, 32));
+			for (int q = 0; q < 8; q++) o[q] = gs[q];
+		}
 	}
 	const double* g1 = gp ? go : gs;  // c1's gradients (GR != 0)
 	const double* g2 = gp ? gs : go;  // c2's (c1's on a boundary)
