@@ -17,10 +17,12 @@
 //          edl[8][ne] Gauss-point offsets (c1: gp1 x,y gp2 x,y; c2: same),
 //          F[ne][4] = (flux at gp1 + flux at gp2) * l/2, one record per edge
 //          written once
-// Each RK stage is three passes: gradient (cell gather), edge flux (one
-// thread per edge, result written once to F), update (cell gather of its
-// three faces in ascending edge order, then the RK combination).  No
-// floating-point atomics: results do not depend on scheduling.
+// Each RK stage is three passes: gradient (cell gather), edge flux (a lane
+// pair per edge, result written once to F), update (cell gather of its
+// three faces in ascending edge order, then the RK combination); without a
+// limiter the edge pass forms the two cells' gradients itself and the
+// gradient pass is not run.  No floating-point atomics: results do not
+// depend on scheduling.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
