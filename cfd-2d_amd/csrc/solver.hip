@@ -12,7 +12,8 @@
 //          S[nc], faces cface[3][nc] = edge*2 + side in ascending edge order,
 //          cnb[3][nc] neighbour across each face (-1 on a boundary),
 //          lsqw[6][nc] least-squares weights, cfn[6][nc] the faces'
-//          outward n l (Green-Gauss weights), flags[nc]
+//          outward n l (Green-Gauss weights), cgp[12][nc] the cell's offsets
+//          to its faces' Gauss points (limiter only), flags[nc]
 //   edges  ec1/ec2 (ec2 = -(bc+1) on a boundary), enx, eny, el,
 //          edl[8][ne] Gauss-point offsets (c1: gp1 x,y gp2 x,y; c2: same),
 //          F[ne][4] = (flux at gp1 + flux at gp2) * l/2, one record per edge
@@ -47,7 +48,7 @@ constexpr int NXCD = 8;
 
 struct Bufs {
 	int nc, ne, nblocks, nmat;
-	double *U, *U0, *P, *Po, *G, *dtS, *S, *lsqw, *cfn, *loc, *R;
+	double *U, *U0, *P, *Po, *G, *dtS, *S, *lsqw, *cfn, *cgp, *loc, *R;
 	int *cface, *cnb, *cmat;
 	unsigned* flags;
 	int* blockflag;
@@ -221,25 +222,29 @@ __device__ __forceinline__ void minmod_limit(const Bufs& b, int i, const int* fa
 			if (v[q] > hi[q]) hi[q] = v[q];
 		}
 	}
-	double d[6][2];
-	size_t ne = b.ne;
-	for (int k = 0; k < 3; k++) {
-		int e = face[k] >> 1, side = face[k] & 1;
-		for (int gp = 0; gp < 2; gp++) {
-			d[2 * k + gp][0] = b.edl[(size_t)(4 * side + 2 * gp) * ne + e];
-			d[2 * k + gp][1] = b.edl[(size_t)(4 * side + 2 * gp + 1) * ne + e];
-		}
+	double d[6][2];  // offsets to the Gauss points of the faces, per cell (coalesced)
+	size_t n = b.nc;
+	for (int k = 0; k < 6; k++) {
+		d[k][0] = b.cgp[(size_t)(2 * k) * n + i];
+		d[k][1] = b.cgp[(size_t)(2 * k + 1) * n + i];
 	}
+	// psi = min(1, min_k r_k), r_k = (hi - phi) / D or (lo - phi) / D by the sign
+	// of D, always >= 0: the smallest ratio is picked by cross-multiplied
+	// comparisons (a_k den < num b_k, all terms >= 0) and divided once
 	for (int q = 0; q < 4; q++) {
 		double gx = g[2 * q], gy = g[2 * q + 1];
-		double psi = 1.0;
+		double num = 1.0, den = 1.0;
 		for (int k = 0; k < 6; k++) {
 			double D = gx * d[k][0] + gy * d[k][1];
-			double r = 1.0;
-			if (D > 0.0) r = (hi[q] - phi[q]) / D;
-			else if (D < 0.0) r = (lo[q] - phi[q]) / D;
-			if (r < psi) psi = r;
+			if (D == 0.0) continue;
+			double a = D > 0.0 ? hi[q] - phi[q] : phi[q] - lo[q];
+			double bb = fabs(D);
+			if (a * den < num * bb) {
+				num = a;
+				den = bb;
+			}
 		}
+		double psi = num / den;
 		if (psi < 0.0) psi = 0.0;
 		g[2 * q] = psi * gx;
 		g[2 * q + 1] = psi * gy;
@@ -1110,6 +1115,26 @@ int cfd2d_ctx_create(const cfd2d_mesh* mesh, const cfd2d_params* p, int n_bc, co
 	    (rc = dalloc(c, &b.err, 2))) {
 		destroy_ctx(c);
 		return rc;
+	}
+	if (p->limiter != CFD2D_LIMITER_NONE) {
+		// each cell's offsets to the Gauss points of its faces, faces in
+		// ascending edge order (the limiter's order), from edl's own side
+		std::vector<double> cgp(12 * (size_t)nc);
+		for (int i = 0; i < nc; i++)
+			for (int k = 0; k < 3; k++) {
+				int f = cface[(size_t)k * nc + i];
+				int e = f >> 1, side = f & 1;
+				for (int g = 0; g < 2; g++) {
+					cgp[(size_t)(2 * (2 * k + g)) * nc + i] = edl[(size_t)(4 * side + 2 * g) * ne + e];
+					cgp[(size_t)(2 * (2 * k + g) + 1) * nc + i] = edl[(size_t)(4 * side + 2 * g + 1) * ne + e];
+				}
+			}
+		if ((rc = upload(c, &b.cgp, cgp))) {
+			destroy_ctx(c);
+			return rc;
+		}
+	} else {
+		b.cgp = nullptr;
 	}
 	if (p->gradient == CFD2D_GRAD_LSQ) {
 		std::vector<double> w;
