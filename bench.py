@@ -131,14 +131,17 @@ def batched_vs_serial(n_cases, steps, seed=7):
     cells = sum(c.mesh.n_cells for c in cs)
     b = batch.BatchSolver(cs)
     b.run(2)
+    tuned = b.autotune()  # step variant of the batch (same results whichever)
     t0 = time.perf_counter()
     b.run(steps)
     t_batch = time.perf_counter() - t0
     b.close()
     from cfd2d import native
     solvers = [native.GpuSolver(c) for c in cs]
+    picks = []
     for s in solvers:
         s.run(2)
+        picks.append(s.autotune())  # each small case tuned for itself
     t0 = time.perf_counter()
     for s in solvers:
         s.step(steps)
@@ -156,7 +159,9 @@ def batched_vs_serial(n_cases, steps, seed=7):
     rate = lambda t: cells * steps / t
     return dict(cases=n_cases, cells=cells, steps=steps, batched=rate(t_batch), serial_eager=rate(t_serial),
                 serial_graph=rate(t_graph), ratio_vs_eager=t_serial / t_batch, ratio_vs_graph=t_graph / t_batch,
-                unit="cell-updates/s")
+                unit="cell-updates/s",
+                autotune=dict(batch=tuned, serial_fused=sum(p["fuse"] for p in picks),
+                              serial_graph=sum(p["graph"] for p in picks), serial_cases=len(picks)))
 
 
 def adjoint_vs_forward(steps, ckpt, ny=289):
@@ -360,6 +365,7 @@ def main():
     s = native.GpuSolver(c)
     log("setup %.1f s: %d cells, %d edges" % (time.perf_counter() - t0, s.nc, s.ne))
     s.run(args.warmup, graph=args.graph)
+    tuned = s.autotune()  # step variant (same results whichever; timed steps below launch eagerly)
     # timed region: HIP events only around the dominant kernel (the edge flux
     # pass), so that the events themselves barely disturb the stream
     s.timing(True, kernels=[DOMINANT])
@@ -387,7 +393,7 @@ def main():
     for k, (ms, n) in tim.items():
         if n:
             avg = ms / n
-            b = alg_bytes(k, s.nc, s.ne)
+            b = alg_bytes(k, s.nc, s.ne, fused=bool(tuned["fuse"]))
             per[k] = dict(avg_ms=round(avg, 5), launches=n, alg_bytes=b,
                           gbs=round(b / (avg * 1e-3) / 1e9, 1) if b else None,
                           timed_region=(k == DOMINANT and bool(tim_timed[DOMINANT][1])))
@@ -413,7 +419,7 @@ def main():
         "data": "synthetic (seeded jittered periodic triangulation, Gaussian pulse + seeded velocity noise)",
         "config": {"workload": "periodic Navier-Stokes, %d triangles (%dx%d quads x 2), LSQ gradients, exact "
                                "Godunov flux, SSP-RK3, CFL 0.4 spectral dt, mu=1e-4, Pr=0.72" % (s.nc, args.n, args.n),
-                   "cells": s.nc, "edges": s.ne, "graph": bool(args.graph)},
+                   "cells": s.nc, "edges": s.ne, "graph": bool(args.graph), "autotune": tuned},
         "roofline": roof,
         "step_hbm_frac": round(step_bytes * args.steps / dt / 1e9 / HBM_PEAK_GBS, 4),
         "kernels": per,

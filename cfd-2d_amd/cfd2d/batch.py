@@ -49,6 +49,10 @@ class BatchSolver:
         if calc_tau:
             self.ctx.calc_tau()
 
+    def autotune(self, trial_steps=3):
+        """Picks the fastest step variant for this batch (state restored)."""
+        return self.ctx.autotune(trial_steps)
+
     def run(self, n, graph=False):
         if graph:
             self.ctx.step_graph(n)

@@ -88,6 +88,8 @@ SIGNATURES = {
     "cfd2d_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "cfd2d_timing_kernels": (C.c_int, [C.c_void_p, C.c_uint]),
     "cfd2d_fuse_gradient": (C.c_int, [C.c_void_p, C.c_int]),
+    "cfd2d_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                 C.POINTER(C.c_double)]),
     "cfd2d_timing_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "cfd2d_ctx_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cfd2d_ctx_stream": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
@@ -359,10 +361,19 @@ class Context:
         return (pinned_array((nc, 4, 4, 4), np.float64), pinned_array((nc, 4), np.int32),
                 pinned_array((nc, 4), np.float64), pinned_array((nc,), np.float64))
 
-    def fuse_gradient(self, enable=True):
-        """Unlimited steps: the edge pass forms the gradients (True, default)
-        or reads a separate gradient pass's (False); the same results."""
-        check(lib().cfd2d_fuse_gradient(self._h, 1 if enable else 0))
+    def fuse_gradient(self, mode=True):
+        """Where the gradients are formed: True/1 in the edge pass, False/0 in
+        their own pass, None/-1 the default (edge pass unless a limiter is
+        on); the same results."""
+        m = -1 if mode is None else (1 if mode else 0)
+        check(lib().cfd2d_fuse_gradient(self._h, m))
+
+    def autotune(self, trial_steps=3):
+        """Times the step variants on the current state (restored afterwards)
+        and keeps the fastest; returns dict(fuse, graph, ms_per_step)."""
+        f, g, ms = C.c_int(), C.c_int(), C.c_double()
+        check(lib().cfd2d_autotune(self._h, int(trial_steps), C.byref(f), C.byref(g), C.byref(ms)))
+        return dict(fuse=f.value, graph=g.value, ms_per_step=ms.value)
 
     def timing(self, enable=True, kernels=None):
         """Per-kernel HIP-event timing on or off (counters reset); kernels =

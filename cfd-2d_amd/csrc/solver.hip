@@ -368,7 +368,7 @@ __device__ __forceinline__ void store_edge_flux(double* F, int e, int gp, const 
 // the live state per lane doubles the waves a SIMD can hold.
 //
 // GR = 0 reads the gradients the gradient pass stored; GR = 1 (LSQ) or 2
-// (Green-Gauss) has the pair form them itself: lane 0 the gradient of c1,
+// (Green-Gauss), 3 or 4 with the minmod limiter, has the pair form them itself: lane 0 the gradient of c1,
 // lane 1 that of c2 (c1's on a boundary), with the gradient pass's own code,
 // and the lanes swap them.  Every cell's gradient is then formed by each of
 // its edges (three times) instead of once, but the gradient pass, its 64-byte
@@ -391,7 +391,8 @@ __global__ __launch_bounds__(BLOCK) void k_edge_flux(Bufs b, Phys ph)
 	if (GR) {
 		int face[3];
 		Prim nb[3];
-		cell_gradient<MM, GR == 1>(b, ph, cs, ps, face, nb, gs);
+		cell_gradient<MM, GR == 1 || GR == 3>(b, ph, cs, ps, face, nb, gs);
+		if (GR >= 3) minmod_limit(b, cs, face, ps, nb, gs);
 		for (int q = 0; q < 8; q++) go[q] = __shfl_xor(gs[q], 1);
 		// when the records are wanted, each cell's is stored by the lane that
 		// formed it on the cell's first face (lowest edge index): once
@@ -688,7 +689,9 @@ struct cfd2d_ctx {
 	// timing
 	bool timing = false;
 	unsigned timing_mask = ~0u;  // kernel ids (bit CFD2D_K_*) that get events
-	bool fuse_gradient = true;   // unlimited steps: the edge pass forms the gradients
+	int fuse_mode = -1;          // 1: the edge pass forms the gradients, 0: a separate
+	                             // gradient pass, -1: 1 without a limiter, 0 with one
+	bool prefer_graph = false;   // cfd2d_step replays a recorded step (chosen by cfd2d_autotune)
 	bool ev_open = false;        // rec_start recorded a start event
 	std::vector<hipEvent_t> ev0, ev1;
 	std::vector<int> ev_kernel;
@@ -835,23 +838,37 @@ void launch_edge(cfd2d_ctx* c)
 	else launch_edge_v<MM, false, 0>(c);
 }
 
-// gradient + edge passes of one stage; without a limiter the edge pass forms
-// the gradients itself (nothing reads the stored gradients afterwards)
+// gradient + edge passes of one stage: by default the edge pass forms the
+// gradients itself without a limiter (nothing reads the stored gradients
+// afterwards) and a separate pass forms them with one (three limiter
+// evaluations per cell cost more than the pass they save on large meshes);
+// cfd2d_fuse_gradient and cfd2d_autotune change the choice
 template <bool MM>
 void launch_gradient_edge(cfd2d_ctx* c)
 {
-	if (c->p.limiter != CFD2D_LIMITER_NONE || !c->fuse_gradient) {
+	bool lim = c->p.limiter != CFD2D_LIMITER_NONE;
+	bool fuse = c->fuse_mode < 0 ? !lim : c->fuse_mode != 0;
+	if (!fuse) {
 		launch_gradient<MM>(c);
 		launch_edge<MM>(c);
 		return;
 	}
 	bool lsq = c->p.gradient == CFD2D_GRAD_LSQ;
+	int gr = (lsq ? 1 : 2) + (lim ? 2 : 0);
 	if (c->p.viscous) {
-		if (lsq) launch_edge_v<MM, true, 1>(c);
-		else launch_edge_v<MM, true, 2>(c);
+		switch (gr) {
+		case 1: launch_edge_v<MM, true, 1>(c); break;
+		case 2: launch_edge_v<MM, true, 2>(c); break;
+		case 3: launch_edge_v<MM, true, 3>(c); break;
+		default: launch_edge_v<MM, true, 4>(c); break;
+		}
 	} else {
-		if (lsq) launch_edge_v<MM, false, 1>(c);
-		else launch_edge_v<MM, false, 2>(c);
+		switch (gr) {
+		case 1: launch_edge_v<MM, false, 1>(c); break;
+		case 2: launch_edge_v<MM, false, 2>(c); break;
+		case 3: launch_edge_v<MM, false, 3>(c); break;
+		default: launch_edge_v<MM, false, 4>(c); break;
+		}
 	}
 }
 
@@ -954,6 +971,14 @@ void lsq_weights(const Mesh& m, std::vector<double>& w)
 			w[(size_t)(2 * k + 1) * nc + i] = (a11 * d[k][1] - a12 * d[k][0]) / det;
 		}
 	}
+}
+
+void drop_graph(cfd2d_ctx* c)
+{
+	if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+	if (c->graph) (void)hipGraphDestroy(c->graph);
+	c->graph_exec = nullptr;
+	c->graph = nullptr;
 }
 
 void destroy_ctx(cfd2d_ctx* c)
@@ -1331,6 +1356,7 @@ int cfd2d_residual(cfd2d_ctx* c, double* r)
 int cfd2d_step(cfd2d_ctx* c, int nsteps)
 {
 	if (!c || nsteps < 0) return fail(CFD2D_ERR_ARG, "bad arguments");
+	if (c->prefer_graph && !c->timing) return cfd2d_step_graph(c, nsteps);
 	for (int s = 0; s < nsteps; s++) step_once(c);
 	return check_launch("cfd2d_step");
 }
@@ -1405,18 +1431,96 @@ int cfd2d_timing(cfd2d_ctx* c, int enable)
 	return CFD2D_OK;
 }
 
-int cfd2d_fuse_gradient(cfd2d_ctx* c, int enable)
+int cfd2d_fuse_gradient(cfd2d_ctx* c, int mode)
 {
-	if (!c) return fail(CFD2D_ERR_ARG, "null");
-	c->fuse_gradient = enable != 0;
-	if (c->graph_exec) {  // the recorded step holds the previous choice
-		(void)hipGraphExecDestroy(c->graph_exec);
-		c->graph_exec = nullptr;
+	if (!c || mode < -1 || mode > 1) return fail(CFD2D_ERR_ARG, "cfd2d_fuse_gradient: mode is -1, 0 or 1");
+	c->fuse_mode = mode;
+	drop_graph(c);  // the recorded step holds the previous choice
+	return CFD2D_OK;
+}
+
+int cfd2d_autotune(cfd2d_ctx* c, int trial_steps, int* fuse_mode, int* use_graph, double* ms_per_step)
+{
+	if (!c || trial_steps < 1) return fail(CFD2D_ERR_ARG, "cfd2d_autotune: trial_steps >= 1");
+	size_t nc = c->nc;
+	// everything a step reads and changes, restored after every trial
+	struct Part {
+		void* dev;
+		size_t bytes;
+	} parts[] = {{c->b.U, 4 * nc * sizeof(double)},      {c->b.P, 4 * nc * sizeof(double)},
+	             {c->b.flags, nc * sizeof(unsigned)},     {c->b.dtS, nc * sizeof(double)},
+	             {c->b.loc, nc * sizeof(double)},         {c->b.err, 2 * sizeof(unsigned long long)},
+	             {c->b.anyflag, 2 * sizeof(int)},         {c->b.blockflag, (size_t)c->b.nblocks * sizeof(int)}};
+	const int np = sizeof(parts) / sizeof(parts[0]);
+	size_t total = 0, off[np];
+	for (int k = 0; k < np; k++) {
+		off[k] = total;
+		total += (parts[k].bytes + 255) & ~(size_t)255;
 	}
-	if (c->graph) {
-		(void)hipGraphDestroy(c->graph);
-		c->graph = nullptr;
+	HIPCHK(hipStreamSynchronize(c->st));
+	char* save = nullptr;
+	HIPCHK(hipMalloc(&save, total));
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	bool timing = c->timing;
+	int fuse0 = c->fuse_mode;
+	bool graph0 = c->prefer_graph;
+	auto copy_all = [&](bool to_save) -> hipError_t {
+		for (int k = 0; k < np; k++) {
+			void* d = to_save ? (void*)(save + off[k]) : parts[k].dev;
+			const void* s2 = to_save ? parts[k].dev : (const void*)(save + off[k]);
+			hipError_t r = hipMemcpyAsync(d, s2, parts[k].bytes, hipMemcpyDeviceToDevice, c->st);
+			if (r != hipSuccess) return r;
+		}
+		return hipSuccess;
+	};
+	hipError_t err = copy_all(true);
+	if (err == hipSuccess) err = hipEventCreate(&e0);
+	if (err == hipSuccess) err = hipEventCreate(&e1);
+	c->timing = false;
+	float best = -1.0f;
+	int best_fuse = 0, best_graph = 0;
+	for (int v = 0; v < 4 && err == hipSuccess; v++) {
+		int fuse = v & 1, graph = v >> 1;
+		c->fuse_mode = fuse;
+		c->prefer_graph = graph != 0;
+		drop_graph(c);
+		if ((err = copy_all(false)) != hipSuccess) break;
+		if (cfd2d_step(c, 1) != CFD2D_OK) {  // warm-up (records the graph)
+			err = hipErrorUnknown;
+			break;
+		}
+		if ((err = hipEventRecord(e0, c->st)) != hipSuccess) break;
+		if (cfd2d_step(c, trial_steps) != CFD2D_OK) {
+			err = hipErrorUnknown;
+			break;
+		}
+		if ((err = hipEventRecord(e1, c->st)) != hipSuccess) break;
+		if ((err = hipEventSynchronize(e1)) != hipSuccess) break;
+		float ms = 0.0f;
+		if ((err = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+		if (best < 0.0f || ms < best) {
+			best = ms;
+			best_fuse = fuse;
+			best_graph = graph;
+		}
 	}
+	if (err == hipSuccess) err = copy_all(false);
+	if (err == hipSuccess) err = hipStreamSynchronize(c->st);
+	c->timing = timing;
+	drop_graph(c);
+	if (e0) (void)hipEventDestroy(e0);
+	if (e1) (void)hipEventDestroy(e1);
+	(void)hipFree(save);
+	if (err != hipSuccess) {
+		c->fuse_mode = fuse0;
+		c->prefer_graph = graph0;
+		return hip_fail(err, "cfd2d_autotune");
+	}
+	c->fuse_mode = best_fuse;
+	c->prefer_graph = best_graph != 0;
+	if (fuse_mode) *fuse_mode = best_fuse;
+	if (use_graph) *use_graph = best_graph;
+	if (ms_per_step) *ms_per_step = best / trial_steps;
 	return CFD2D_OK;
 }
 

@@ -146,10 +146,18 @@ int  cfd2d_status(cfd2d_ctx* c, long long* newton_cap_hits, int* nonfinite);
 int  cfd2d_wall_force(cfd2d_ctx* c, const int* edge_mask, double* fx, double* fy);
 /* per-kernel HIP-event timing on the context stream */
 int  cfd2d_timing(cfd2d_ctx* c, int enable);
-/* Steps without a limiter let the edge pass form the cell gradients itself
- * (default, enable = 1) instead of reading those of a separate gradient pass
- * (enable = 0); both give the same bits.  No reference counterpart. */
-int  cfd2d_fuse_gradient(cfd2d_ctx* c, int enable);
+/* Where the cell gradients are formed: mode 1 in the edge pass itself (each
+ * edge's lane pair forms its two cells'), 0 in a separate gradient pass
+ * whose records the edge pass reads, -1 (default) 1 without a limiter and 0
+ * with one.  All modes give the same bits.  No reference counterpart. */
+int  cfd2d_fuse_gradient(cfd2d_ctx* c, int mode);
+/* Autotuner (no reference counterpart): times trial_steps steps of each
+ * variant -- gradients in the edge pass or in their own pass, eager launches
+ * or a replayed recorded step -- on the context's current state, restores
+ * that state exactly, and keeps the fastest for cfd2d_step (same results
+ * either way).  Reports the choice and its ms per step (any pointer may be
+ * NULL).  Uses a temporary device copy of the state (~100 B per cell). */
+int  cfd2d_autotune(cfd2d_ctx* c, int trial_steps, int* fuse_mode, int* use_graph, double* ms_per_step);
 /* which kernels get events while timing is on: bit k = CFD2D_K_* id
  * (default all); fewer events disturb the stream less */
 int  cfd2d_timing_kernels(cfd2d_ctx* c, unsigned mask);

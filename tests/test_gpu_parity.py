@@ -132,6 +132,39 @@ def test_fused_gradient_edge_pass_is_bitwise(opts):
     assert np.array_equal(outs[0], outs[1])
 
 
+def test_fused_limited_gradient_is_bitwise():
+    """With the minmod limiter too (mode 1 forms and limits the gradients in
+    the edge pass; the default runs the separate pass)."""
+    case = cases.shock_tube(40, 6, seed=4, mild=False, limiter="MINMOD", max_newton=200)
+    outs = []
+    for mode in (True, None):
+        s = native.GpuSolver(case)
+        s.fuse_gradient(mode)
+        s.run(30)
+        outs.append((s.state(), s.flags()))
+        s.close()
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("name,opts", [("inlet", dict()), ("bounds", dict(limiter="MINMOD"))], ids=str)
+def test_autotune_restores_state_and_keeps_results(name, opts):
+    case = gc.case(name, **opts)
+    a = native.GpuSolver(case)
+    b = native.GpuSolver(case)
+    a.run(7)
+    b.run(7)
+    U, F = a.state(), a.flags()
+    choice = a.autotune(trial_steps=2)
+    assert choice["fuse"] in (0, 1) and choice["graph"] in (0, 1) and choice["ms_per_step"] > 0
+    assert np.array_equal(a.state(), U) and np.array_equal(a.flags(), F)
+    assert a.status() == b.status()
+    a.run(20)
+    b.run(20)
+    assert np.array_equal(a.state(), b.state()) and np.array_equal(a.flags(), b.flags())
+    a.close()
+    b.close()
+
+
 EXT = [  # (full Sod state?, options): unlimited variants on the mild tube, limited ones on Sod
     (False, dict(flux="LAX")), (False, dict(flux="HLLC")), (False, dict(flux="ROE")),
     (False, dict(gradient="LSQ")), (False, dict(rk_stages=3)), (False, dict(cfl_mode="SPECTRAL", cfl=0.4)),
