@@ -146,7 +146,8 @@ def test_fused_limited_gradient_is_bitwise():
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
 
 
-@pytest.mark.parametrize("name,opts", [("inlet", dict()), ("bounds", dict(limiter="MINMOD"))], ids=str)
+@pytest.mark.parametrize("name,opts", [("inlet", dict()), ("bounds", dict(limiter="MINMOD")), ("steady", dict()),
+                                       ("twomat", dict(gradient="LSQ", rk_stages=3))], ids=str)
 def test_autotune_restores_state_and_keeps_results(name, opts):
     case = gc.case(name, **opts)
     a = native.GpuSolver(case)
