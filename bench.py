@@ -34,8 +34,9 @@ METRIC = "FP64 cell-updates/s (2.1M-tri NS) + % HBM peak; batched/serial and adj
 # perfect): gradient 156 B/cell with LSQ (164 with Green-Gauss); edge flux
 # 128 B/edge (c1, c2, n, l, Gauss-point offsets read, the flux record written)
 # plus, reading stored gradients, 96 B/cell (P and G records) or, forming
-# them itself as unlimited steps do, 104 B/cell (P record, cface, cnb and the
-# 48 B of LSQ weights); update 212 B/cell
+# them itself as unlimited steps do, 104 B/cell (P record, the 24 B of faces
+# and neighbours and the 48 B of LSQ weights); update 212 B/cell (the 4 B of
+# padding in each 16-byte face/neighbour record is not counted)
 def alg_bytes(kernel, nc, ne, lsq=True, fused=True):
     if kernel == "gradient":
         return (156 if lsq else 164) * nc

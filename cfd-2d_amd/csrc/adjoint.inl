@@ -306,7 +306,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 	int t = xcd_block() * BLOCK + threadIdx.x;
 	int i = t >> 1, gp = t & 1;
 	if (i >= b.nc) return;  // both lanes of a pair leave together
-	size_t n = b.nc, ne = b.ne;
+	size_t ne = b.ne;
 	Prim pi = load_prim(b, i);
 	double wi[4];
 	double si = scale * b.dtS[i];
@@ -314,10 +314,11 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 	for (int q = 0; q < 4; q++) wi[q] *= si;
 	double pb[4] = {0, 0, 0, 0};
 	double gb[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	CellTop tp = load_top(b, i);
 	for (int k = 0; k < 3; k++) {
-		int f = b.cface[k * n + i];
+		int f = tp.f[k];
 		int e = f >> 1, side = f & 1;
-		int j = b.cnb[k * n + i];
+		int j = tp.nb[k];
 		double nx = b.enx[e], ny = b.eny[e];
 		double hl = b.el[e] * 0.5;
 		// F_e-bar = (Rb_c2 - Rb_c1) l/2; this cell is c1 (side 0) or c2 (side 1)
@@ -383,11 +384,12 @@ __global__ __launch_bounds__(BLOCK) void k_adj_cell(Bufs b, AdjBufs a)
 {
 	int i = xcd_block() * BLOCK + threadIdx.x;
 	if (i >= b.nc) return;
-	size_t n = b.nc, ne = b.ne;
+	size_t ne = b.ne;
 	double pb[4] = {0, 0, 0, 0};
 	double gb[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	CellTop tp = load_top(b, i);
 	for (int k = 0; k < 3; k++) {
-		int f = b.cface[k * n + i];
+		int f = tp.f[k];
 		int e = f >> 1, side = f & 1;
 		for (int gp = 0; gp < 2; gp++) {
 			double dx = b.edl[(size_t)(4 * side + 2 * gp) * ne + e];
@@ -399,7 +401,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_cell(Bufs b, AdjBufs a)
 				gb[2 * q + 1] += v * dy;
 			}
 		}
-		if (b.cnb[k * n + i] < 0)
+		if (tp.nb[k] < 0)
 			for (int gp = 0; gp < 2; gp++)
 				for (int q = 0; q < 4; q++) pb[q] += a.Ea[(size_t)(8 + 4 * gp + q) * ne + e];
 	}
@@ -426,13 +428,14 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 		double gb[8];
 		lda8(a.Gb, b.nc, i, gb);
 		double si = b.S[i];
+		CellTop tp = load_top(b, i);
 		for (int k = 0; k < 3; k++) {
-			int f = b.cface[k * n + i];
+			int f = tp.f[k];
 			int e = f >> 1, side = f & 1;
-			int j = b.cnb[k * n + i];
+			int j = tp.nb[k];
 			double nbb[4];  // adjoint of the neighbour value this face contributed
 			if (LSQ) {
-				double wx = b.lsqw[(size_t)(2 * k) * n + i], wy = b.lsqw[(size_t)(2 * k + 1) * n + i];
+				double wx = b.lsqw[6 * (size_t)i + 2 * k], wy = b.lsqw[6 * (size_t)i + 2 * k + 1];
 				for (int q = 0; q < 4; q++) {
 					nbb[q] = wx * gb[2 * q] + wy * gb[2 * q + 1];
 					pb[q] -= nbb[q];
@@ -451,7 +454,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 				double gj[8];
 				lda8(a.Gb, b.nc, j, gj);
 				if (LSQ) {
-					double wx = b.lsqw[(size_t)(2 * s) * n + j], wy = b.lsqw[(size_t)(2 * s + 1) * n + j];
+					double wx = b.lsqw[6 * (size_t)j + 2 * s], wy = b.lsqw[6 * (size_t)j + 2 * s + 1];
 					for (int q = 0; q < 4; q++) pb[q] += wx * gj[2 * q] + wy * gj[2 * q + 1];
 				} else {
 					double l = b.el[e], nx = b.enx[e], ny = b.eny[e];
@@ -620,19 +623,19 @@ int adj_topology(cfd2d_ctx* c)
 {
 	if (c->adj_nbedges >= 0) return 0;
 	int nc = c->nc, ne = c->ne;
-	std::vector<int> cface(3 * (size_t)nc), cnb(3 * (size_t)nc), ec2(ne);
+	std::vector<int> cface(4 * (size_t)nc), cnb(4 * (size_t)nc), ec2(ne);
 	HIPCHK(hipMemcpy(cface.data(), c->b.cface, cface.size() * sizeof(int), hipMemcpyDeviceToHost));
 	HIPCHK(hipMemcpy(cnb.data(), c->b.cnb, cnb.size() * sizeof(int), hipMemcpyDeviceToHost));
 	HIPCHK(hipMemcpy(ec2.data(), c->b.ec2, ec2.size() * sizeof(int), hipMemcpyDeviceToHost));
 	std::vector<int> cslot(3 * (size_t)nc, 0), bed;
 	for (int i = 0; i < nc; i++)
 		for (int k = 0; k < 3; k++) {
-			int j = cnb[(size_t)k * nc + i];
+			int j = cnb[4 * (size_t)i + k];
 			if (j < 0) continue;
-			int e = cface[(size_t)k * nc + i] >> 1;
+			int e = cface[4 * (size_t)i + k] >> 1;
 			int found = -1;
 			for (int s = 0; s < 3; s++)
-				if ((cface[(size_t)s * nc + j] >> 1) == e) found = s;
+				if ((cface[4 * (size_t)j + s] >> 1) == e) found = s;
 			if (found < 0) return fail(CFD2D_ERR_MESH, "adjoint: inconsistent face lists");
 			cslot[(size_t)k * nc + i] = found;
 		}

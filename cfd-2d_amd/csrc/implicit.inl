@@ -42,13 +42,13 @@ __global__ __launch_bounds__(BLOCK) void k_impl_gradient(Bufs b, Phys ph)
 	int i = c0 + threadIdx.x;
 	double g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 	if (i < b.nc) {
-		size_t n = b.nc;
 		Prim pi = load_prim(b, i);
+		CellTop tp = load_top(b, i);
 		for (int k = 0; k < 3; k++) {
-			int j = b.cnb[k * n + i];
-			Prim q = j >= 0 ? load_prim(b, j) : ghost_impl_of<MM>(b, ph, b.cface[k * n + i] >> 1, i, pi);
+			int j = tp.nb[k];
+			Prim q = j >= 0 ? load_prim(b, j) : ghost_impl_of<MM>(b, ph, tp.f[k] >> 1, i, pi);
 			// n l out of cell i (the reference adds for c1 and subtracts for c2)
-			double fx = b.cfn[(size_t)(2 * k) * n + i], fy = b.cfn[(size_t)(2 * k + 1) * n + i];
+			double fx = b.cfn[6 * (size_t)i + 2 * k], fy = b.cfn[6 * (size_t)i + 2 * k + 1];
 			double s[4] = {pi.r + q.r, pi.p + q.p, pi.u + q.u, pi.v + q.v};
 			for (int c = 0; c < 4; c++) {
 				g[2 * c] += s[c] / 2 * fx;
@@ -134,7 +134,6 @@ __global__ __launch_bounds__(BLOCK) void k_impl_assemble(Bufs b, Phys ph, ImplOu
 	int c0 = xcd_block() * BLOCK;
 	int nvalid = min(BLOCK, b.nc - c0);
 	int i = c0 + (threadIdx.x < nvalid ? threadIdx.x : 0);
-	size_t n = b.nc;
 	Prim pi = load_prim(b, i);
 	RoeSide si = roe_side(pi.r);
 	double diag[16];
@@ -142,10 +141,11 @@ __global__ __launch_bounds__(BLOCK) void k_impl_assemble(Bufs b, Phys ph, ImplOu
 	double lam = 0.0;
 	int cl[4];
 	cl[0] = i;
+	CellTop tp = load_top(b, i);
 	for (int k = 0; k < 3; k++) {
-		int f = b.cface[k * n + i];
+		int f = tp.f[k];
 		int e = f >> 1, side = f & 1;
-		int j = b.cnb[k * n + i];
+		int j = tp.nb[k];
 		// the edge's (c1, c2) centre states, as reconstruct(iEdge, L, R) gives them
 		Prim q = j >= 0 ? load_prim(b, j) : ghost_impl_of<MM>(b, ph, e, i, pi);
 		RoeSide sq = roe_side(q.r);

@@ -9,10 +9,13 @@
 //          record per cell, because they are gathered by neighbours: one
 //          record is two or four 16-byte loads instead of four or eight
 //          8-byte gathers; dtS[nc] = tau_i / S_i,
-//          S[nc], faces cface[3][nc] = edge*2 + side in ascending edge order,
-//          cnb[3][nc] neighbour across each face (-1 on a boundary),
-//          lsqw[6][nc] least-squares weights, cfn[6][nc] the faces'
-//          outward n l (Green-Gauss weights), cgp[12][nc] the cell's offsets
+//          S[nc], cface[nc][4] faces = edge*2 + side in ascending edge
+//          order and cnb[nc][4] the neighbour across each face (-1 on a
+//          boundary), each one 16 B record per cell (the fourth int unused;
+//          the update pass reads only the faces),
+//          lsqw[nc][6] least-squares weights (wx wy per face), cfn[nc][6]
+//          the faces' outward n l (Green-Gauss weights), both one 48 B
+//          record per cell (three 16-byte loads), cgp[12][nc] the cell's offsets
 //          to its faces' Gauss points (limiter only), flags[nc]
 //   edges  ec1/ec2 (ec2 = -(bc+1) on a boundary), enx, eny, el,
 //          edl[8][ne] Gauss-point offsets (c1: gp1 x,y gp2 x,y; c2: same),
@@ -115,6 +118,22 @@ __device__ __forceinline__ void ld8(const double* base, size_t i, double* v)
 {
 	ld4(base, 2 * i, v);
 	ld4(base, 2 * i + 1, v + 4);
+}
+
+// the cell's faces and neighbours (one 16-byte record each)
+struct CellTop {
+	int f[3], nb[3];
+};
+
+__device__ __forceinline__ CellTop load_top(const Bufs& b, size_t i)
+{
+	int4 a = reinterpret_cast<const int4*>(b.cface)[i], c = reinterpret_cast<const int4*>(b.cnb)[i];
+	return CellTop{{a.x, a.y, a.z}, {c.x, c.y, c.z}};
+}
+
+__device__ __forceinline__ int4 load_faces(const Bufs& b, size_t i)
+{
+	return reinterpret_cast<const int4*>(b.cface)[i];
 }
 
 // Stores the W-double records of the block's cells [c0, c0 + nvalid) held in
@@ -260,18 +279,19 @@ template <bool MM, bool LSQ>
 __device__ __forceinline__ void cell_gradient(const Bufs& b, const Phys& ph, int i, Prim& pi, int* face, Prim* nb,
                                               double* g)
 {
-	size_t n = b.nc;
 	pi = load_prim(b, i);
 	for (int q = 0; q < 8; q++) g[q] = 0.0;
+	CellTop tp = load_top(b, i);
 	for (int k = 0; k < 3; k++) {
-		face[k] = b.cface[k * n + i];
-		int j = b.cnb[k * n + i];
+		face[k] = tp.f[k];
+		int j = tp.nb[k];
 		if (j >= 0) nb[k] = load_prim(b, j);
 		else nb[k] = ghost_of<MM>(b, ph, face[k] >> 1, i, pi, pi);
 	}
 	if (LSQ) {
+		const double* w = static_cast<const double*>(__builtin_assume_aligned(b.lsqw + 6 * (size_t)i, 16));
 		for (int k = 0; k < 3; k++) {
-			double wx = b.lsqw[(size_t)(2 * k) * n + i], wy = b.lsqw[(size_t)(2 * k + 1) * n + i];
+			double wx = w[2 * k], wy = w[2 * k + 1];
 			double dphi[4] = {nb[k].r - pi.r, nb[k].p - pi.p, nb[k].u - pi.u, nb[k].v - pi.v};
 			for (int q = 0; q < 4; q++) {
 				g[2 * q] += wx * dphi[q];
@@ -279,9 +299,10 @@ __device__ __forceinline__ void cell_gradient(const Bufs& b, const Phys& ph, int
 			}
 		}
 	} else {
+		const double* w = static_cast<const double*>(__builtin_assume_aligned(b.cfn + 6 * (size_t)i, 16));
 		for (int k = 0; k < 3; k++) {
 			// n l out of cell i (the reference adds for c1 and subtracts for c2)
-			double fx = b.cfn[(size_t)(2 * k) * n + i], fy = b.cfn[(size_t)(2 * k + 1) * n + i];
+			double fx = w[2 * k], fy = w[2 * k + 1];
 			double s[4] = {pi.r + nb[k].r, pi.p + nb[k].p, pi.u + nb[k].u, pi.v + nb[k].v};
 			for (int q = 0; q < 4; q++) {
 				g[2 * q] += s[q] / 2 * fx;
@@ -481,10 +502,11 @@ __global__ __launch_bounds__(BLOCK) void k_edge_flux(Bufs b, Phys ph)
 // one 32-byte record, two 16-byte loads
 __device__ __forceinline__ void gather_residual(const Bufs& b, int i, double* r)
 {
-	size_t n = b.nc;
 	r[0] = r[1] = r[2] = r[3] = 0.0;
+	int4 fc = load_faces(b, i);
+	int fs[3] = {fc.x, fc.y, fc.z};
 	for (int k = 0; k < 3; k++) {
-		int f = b.cface[k * n + i];
+		int f = fs[k];
 		double v[4];
 		ld4(b.F, f >> 1, v);
 		if (f & 1)
@@ -600,8 +622,10 @@ __global__ __launch_bounds__(BLOCK) void k_remediate(Bufs b, Phys ph)
 					unsigned fl = b.flags[i];
 					if (!(fl & 0x2u)) continue;
 					double sr[4] = {0.0, 0.0, 0.0, 0.0}, S = 0.0;
+					int4 fc = load_faces(b, i);
+					int fs[3] = {fc.x, fc.y, fc.z};
 					for (int k = 0; k < 3; k++) {
-						int e = b.cface[k * n + i] >> 1;
+						int e = fs[k] >> 1;
 						int j = b.ec2[e];
 						if (j >= 0) {
 							double s = b.S[j];
@@ -635,7 +659,6 @@ __global__ __launch_bounds__(BLOCK) void k_tau(Bufs b, Phys ph)
 {
 	int i = xcd_block() * BLOCK + threadIdx.x;
 	if (i >= b.nc) return;
-	size_t n = b.nc;
 	Prim p = load_prim(b, i);
 	MatV m = mat_of<MM>(b, ph, i);
 	double cz = sqrt(m.gam * p.p / p.r);
@@ -644,10 +667,11 @@ __global__ __launch_bounds__(BLOCK) void k_tau(Bufs b, Phys ph)
 	if (ph.cfl_mode == CFD2D_CFL_SPECTRAL) {
 		double li = sqrt(p.u * p.u + p.v * p.v) + cz;
 		double sum = 0.0;
+		CellTop tp = load_top(b, i);
 		for (int k = 0; k < 3; k++) {
-			int f = b.cface[k * n + i];
+			int f = tp.f[k];
 			int e = f >> 1;
-			int j = b.cnb[k * n + i];
+			int j = tp.nb[k];
 			Prim q;
 			double czj;
 			if (j >= 0) {
@@ -967,8 +991,8 @@ void lsq_weights(const Mesh& m, std::vector<double>& w)
 		}
 		double det = a11 * a22 - a12 * a12;
 		for (int k = 0; k < 3; k++) {
-			w[(size_t)(2 * k) * nc + i] = (a22 * d[k][0] - a12 * d[k][1]) / det;
-			w[(size_t)(2 * k + 1) * nc + i] = (a11 * d[k][1] - a12 * d[k][0]) / det;
+			w[6 * (size_t)i + 2 * k] = (a22 * d[k][0] - a12 * d[k][1]) / det;
+			w[6 * (size_t)i + 2 * k + 1] = (a11 * d[k][1] - a12 * d[k][0]) / det;
 		}
 	}
 }
@@ -1091,17 +1115,17 @@ int cfd2d_ctx_create(const cfd2d_mesh* mesh, const cfd2d_params* p, int n_bc, co
 	c->ph.steady = p->steady;
 
 	int nc = m.nc, ne = m.ne;
-	std::vector<int> cface(3 * (size_t)nc), cnb(3 * (size_t)nc);
+	std::vector<int> cface(4 * (size_t)nc, -1), cnb(4 * (size_t)nc, -1);
 	std::vector<double> cfn(6 * (size_t)nc);
 	for (int i = 0; i < nc; i++)
 		for (int k = 0; k < 3; k++) {
 			int e = m.cedges[3 * (size_t)i + k];
 			int side = (m.c1[e] == i) ? 0 : 1;
-			cface[(size_t)k * nc + i] = 2 * e + side;
-			cnb[(size_t)k * nc + i] = side == 0 ? m.c2[e] : m.c1[e];
+			cface[4 * (size_t)i + k] = 2 * e + side;
+			cnb[4 * (size_t)i + k] = side == 0 ? m.c2[e] : m.c1[e];
 			double sg = side == 0 ? 1.0 : -1.0;
-			cfn[(size_t)(2 * k) * nc + i] = sg * (m.nx[e] * m.l[e]);
-			cfn[(size_t)(2 * k + 1) * nc + i] = sg * (m.ny[e] * m.l[e]);
+			cfn[6 * (size_t)i + 2 * k] = sg * (m.nx[e] * m.l[e]);
+			cfn[6 * (size_t)i + 2 * k + 1] = sg * (m.ny[e] * m.l[e]);
 		}
 	std::vector<double> edl(8 * (size_t)ne, 0.0);
 	for (int e = 0; e < ne; e++) {
@@ -1147,7 +1171,7 @@ int cfd2d_ctx_create(const cfd2d_mesh* mesh, const cfd2d_params* p, int n_bc, co
 		std::vector<double> cgp(12 * (size_t)nc);
 		for (int i = 0; i < nc; i++)
 			for (int k = 0; k < 3; k++) {
-				int f = cface[(size_t)k * nc + i];
+				int f = cface[4 * (size_t)i + k];
 				int e = f >> 1, side = f & 1;
 				for (int g = 0; g < 2; g++) {
 					cgp[(size_t)(2 * (2 * k + g)) * nc + i] = edl[(size_t)(4 * side + 2 * g) * ne + e];
