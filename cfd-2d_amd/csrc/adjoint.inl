@@ -251,9 +251,10 @@ __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, 
 	int e = t >> 1, gp = t & 1;
 	if (e >= b.ne) return;  // both lanes of a pair leave together
 	size_t ne = b.ne;
-	int c1 = b.ec1[e], c2 = b.ec2[e];
-	double nx = b.enx[e], ny = b.eny[e];
-	double hl = b.el[e] * 0.5;
+	int2 ce = edge_cells(b, e);
+	int c1 = ce.x, c2 = ce.y;
+	double nx = edge_geo(b, e)[0], ny = edge_geo(b, e)[1];
+	double hl = edge_geo(b, e)[2] * 0.5;
 	double fb[4];
 	double s1 = scale * b.dtS[c1], s2 = c2 >= 0 ? scale * b.dtS[c2] : 0.0;
 	double w1[4], w2[4] = {0, 0, 0, 0};
@@ -266,10 +267,10 @@ __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, 
 	}
 	Prim p1 = load_prim(b, c1);
 	Prim L = p1, R;
-	recon(L, b, c1, b.edl[(size_t)(2 * gp) * ne + e], b.edl[(size_t)(2 * gp + 1) * ne + e]);
+	recon(L, b, c1, b.edl[8 * (size_t)e + 4 * gp], b.edl[8 * (size_t)e + 4 * gp + 1]);
 	if (c2 >= 0) {
 		R = load_prim(b, c2);
-		recon(R, b, c2, b.edl[(size_t)(4 + 2 * gp) * ne + e], b.edl[(size_t)(5 + 2 * gp) * ne + e]);
+		recon(R, b, c2, b.edl[8 * (size_t)e + 4 * gp + 2], b.edl[8 * (size_t)e + 4 * gp + 3]);
 	} else {
 		R = ghost_of<MM>(b, ph, e, c1, L, p1);
 	}
@@ -319,8 +320,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 		int f = tp.f[k];
 		int e = f >> 1, side = f & 1;
 		int j = tp.nb[k];
-		double nx = b.enx[e], ny = b.eny[e];
-		double hl = b.el[e] * 0.5;
+		double nx = edge_geo(b, e)[0], ny = edge_geo(b, e)[1];
+		double hl = edge_geo(b, e)[2] * 0.5;
 		// F_e-bar = (Rb_c2 - Rb_c1) l/2; this cell is c1 (side 0) or c2 (side 1)
 		double fb[4];
 		Prim pj = pi;
@@ -337,13 +338,13 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 			for (int q = 0; q < 4; q++) fb[q] = -wi[q] * hl;
 		}
 		int c1 = side ? j : i, c2 = side ? i : j;
-		double d1x = b.edl[(size_t)(2 * gp) * ne + e], d1y = b.edl[(size_t)(2 * gp + 1) * ne + e];
+		double d1x = b.edl[8 * (size_t)e + 4 * gp], d1y = b.edl[8 * (size_t)e + 4 * gp + 1];
 		Prim L = side ? pj : pi, R;
 		recon(L, b, c1, d1x, d1y);
 		double d2x = 0.0, d2y = 0.0;
 		if (j >= 0) {
-			d2x = b.edl[(size_t)(4 + 2 * gp) * ne + e];
-			d2y = b.edl[(size_t)(5 + 2 * gp) * ne + e];
+			d2x = b.edl[8 * (size_t)e + 4 * gp + 2];
+			d2y = b.edl[8 * (size_t)e + 4 * gp + 3];
 			R = side ? pi : pj;
 			recon(R, b, c2, d2x, d2y);
 		} else {
@@ -353,7 +354,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 		if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb);
 		else lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb);
 		if (j < 0) {
-			int bc = -b.ec2[e] - 1;
+			int bc = -edge_cells(b, e).y - 1;
 			double dL[4], dC[4], dpar[4];
 			ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], nx, ny, L, pi, mat_of<MM>(b, ph, i), Rb, dL, dC, dpar);
 			for (int q = 0; q < 4; q++) {
@@ -392,8 +393,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_cell(Bufs b, AdjBufs a)
 		int f = tp.f[k];
 		int e = f >> 1, side = f & 1;
 		for (int gp = 0; gp < 2; gp++) {
-			double dx = b.edl[(size_t)(4 * side + 2 * gp) * ne + e];
-			double dy = b.edl[(size_t)(4 * side + 2 * gp + 1) * ne + e];
+			double dx = b.edl[8 * (size_t)e + 4 * gp + 2 * side];
+			double dy = b.edl[8 * (size_t)e + 4 * gp + 2 * side + 1];
 			for (int q = 0; q < 4; q++) {
 				double v = a.Ea[(size_t)(8 * side + 4 * gp + q) * ne + e];
 				pb[q] += v;
@@ -441,7 +442,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 					pb[q] -= nbb[q];
 				}
 			} else {
-				double l = b.el[e], nx = b.enx[e], ny = b.eny[e];
+				double l = edge_geo(b, e)[2], nx = edge_geo(b, e)[0], ny = edge_geo(b, e)[1];
 				double sg = side ? -1.0 : 1.0;
 				for (int q = 0; q < 4; q++) {
 					nbb[q] = sg * 0.5 * l * (nx * gb[2 * q] + ny * gb[2 * q + 1]) / si;
@@ -457,16 +458,16 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 					double wx = b.lsqw[6 * (size_t)j + 2 * s], wy = b.lsqw[6 * (size_t)j + 2 * s + 1];
 					for (int q = 0; q < 4; q++) pb[q] += wx * gj[2 * q] + wy * gj[2 * q + 1];
 				} else {
-					double l = b.el[e], nx = b.enx[e], ny = b.eny[e];
+					double l = edge_geo(b, e)[2], nx = edge_geo(b, e)[0], ny = edge_geo(b, e)[1];
 					double sg = side ? 1.0 : -1.0;  // the neighbour sits on the other side
 					double sj = b.S[j];
 					for (int q = 0; q < 4; q++) pb[q] += sg * 0.5 * l * (nx * gj[2 * q] + ny * gj[2 * q + 1]) / sj;
 				}
 			} else {
 				// ghost of the cell-centre state across a boundary edge
-				int bc = -b.ec2[e] - 1;
+				int bc = -edge_cells(b, e).y - 1;
 				double dL[4], dC[4], dpar[4];
-				ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], b.enx[e], b.eny[e], pc, pc, m, nbb, dL, dC, dpar);
+				ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], edge_geo(b, e)[0], edge_geo(b, e)[1], pc, pc, m, nbb, dL, dC, dpar);
 				for (int q = 0; q < 4; q++) {
 					pb[q] += dL[q] + dC[q];
 					a.Ebc[(size_t)q * ne + e] += dpar[q];
@@ -508,7 +509,7 @@ __global__ __launch_bounds__(BLOCK) void k_bc_reduce(Bufs b, AdjBufs a, int nbc)
 		double s[4] = {0, 0, 0, 0};
 		for (int k = threadIdx.x; k < a.nbedges; k += BLOCK) {
 			int e = a.bedges[k];
-			if (-b.ec2[e] - 1 != bc) continue;
+			if (-edge_cells(b, e).y - 1 != bc) continue;
 			for (int q = 0; q < 4; q++) s[q] += a.Ebc[(size_t)q * ne + e] + a.Ebc[(size_t)(4 + q) * ne + e];
 		}
 		for (int q = 0; q < 4; q++) part[threadIdx.x][q] = s[q];
@@ -623,10 +624,11 @@ int adj_topology(cfd2d_ctx* c)
 {
 	if (c->adj_nbedges >= 0) return 0;
 	int nc = c->nc, ne = c->ne;
-	std::vector<int> cface(4 * (size_t)nc), cnb(4 * (size_t)nc), ec2(ne);
+	std::vector<int> cface(4 * (size_t)nc), cnb(4 * (size_t)nc);
+	std::vector<double> eg(4 * (size_t)ne);
 	HIPCHK(hipMemcpy(cface.data(), c->b.cface, cface.size() * sizeof(int), hipMemcpyDeviceToHost));
 	HIPCHK(hipMemcpy(cnb.data(), c->b.cnb, cnb.size() * sizeof(int), hipMemcpyDeviceToHost));
-	HIPCHK(hipMemcpy(ec2.data(), c->b.ec2, ec2.size() * sizeof(int), hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(eg.data(), c->b.eg, eg.size() * sizeof(double), hipMemcpyDeviceToHost));
 	std::vector<int> cslot(3 * (size_t)nc, 0), bed;
 	for (int i = 0; i < nc; i++)
 		for (int k = 0; k < 3; k++) {
@@ -639,8 +641,11 @@ int adj_topology(cfd2d_ctx* c)
 			if (found < 0) return fail(CFD2D_ERR_MESH, "adjoint: inconsistent face lists");
 			cslot[(size_t)k * nc + i] = found;
 		}
-	for (int e = 0; e < ne; e++)
-		if (ec2[e] < 0) bed.push_back(e);
+	for (int e = 0; e < ne; e++) {
+		int cells[2];
+		memcpy(cells, &eg[4 * (size_t)e + 3], sizeof(cells));
+		if (cells[1] < 0) bed.push_back(e);
+	}
 	int rc;
 	if ((rc = upload(c, &c->adj_cslot, cslot)) || (rc = upload(c, &c->adj_bedges, bed))) return rc;
 	c->adj_nbedges = (int)bed.size();

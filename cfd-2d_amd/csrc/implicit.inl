@@ -28,8 +28,8 @@ namespace {
 template <bool MM>
 __device__ __forceinline__ Prim ghost_impl_of(const Bufs& b, const Phys& ph, int e, int c1, const Prim& inside)
 {
-	int bc = -b.ec2[e] - 1;
-	return ghost_implicit(b.bctype[bc], &b.bcpar[4 * bc], b.enx[e], b.eny[e], inside, mat_of<MM>(b, ph, c1));
+	int bc = -edge_cells(b, e).y - 1;
+	return ghost_implicit(b.bctype[bc], &b.bcpar[4 * bc], edge_geo(b, e)[0], edge_geo(b, e)[1], inside, mat_of<MM>(b, ph, c1));
 }
 
 // FVM_TVD_IMPLICIT::calcGrad (fvm_tvd_implicit.cpp:186-262): Green-Gauss of
@@ -71,14 +71,14 @@ __global__ __launch_bounds__(BLOCK) void k_impl_edge(Bufs b, Phys ph)
 	int t = xcd_block() * BLOCK + threadIdx.x;
 	int e = t >> 1, gp = t & 1;
 	if (e >= b.ne) return;  // both lanes of a pair leave together
-	size_t ne = b.ne;
-	int c1 = b.ec1[e], c2 = b.ec2[e];
-	double nx = b.enx[e], ny = b.eny[e];
+	int2 ce = edge_cells(b, e);
+	int c1 = ce.x, c2 = ce.y;
+	double nx = edge_geo(b, e)[0], ny = edge_geo(b, e)[1];
 	Prim L = load_prim(b, c1), R;
 	if (c2 >= 0) {
-		recon(L, b, c1, b.edl[(size_t)(2 * gp) * ne + e], b.edl[(size_t)(2 * gp + 1) * ne + e]);
+		recon(L, b, c1, b.edl[8 * (size_t)e + 4 * gp], b.edl[8 * (size_t)e + 4 * gp + 1]);
 		R = load_prim(b, c2);
-		recon(R, b, c2, b.edl[(size_t)(4 + 2 * gp) * ne + e], b.edl[(size_t)(5 + 2 * gp) * ne + e]);
+		recon(R, b, c2, b.edl[8 * (size_t)e + 4 * gp + 2], b.edl[8 * (size_t)e + 4 * gp + 3]);
 	} else {
 		R = ghost_impl_of<MM>(b, ph, e, c1, L);
 	}
@@ -89,7 +89,7 @@ __global__ __launch_bounds__(BLOCK) void k_impl_edge(Bufs b, Phys ph)
 	if (hit) atomicAdd(&b.err[0], 1ull);
 	double o[4];
 	for (int q = 0; q < 4; q++) o[q] = __shfl_xor(f[q], 1);
-	store_edge_flux(b.F, e, gp, f, o, b.el[e] * 0.5);
+	store_edge_flux(b.F, e, gp, f, o, edge_geo(b, e)[2] * 0.5);
 }
 
 struct ImplOut {
@@ -150,11 +150,11 @@ __global__ __launch_bounds__(BLOCK) void k_impl_assemble(Bufs b, Phys ph, ImplOu
 		Prim q = j >= 0 ? load_prim(b, j) : ghost_impl_of<MM>(b, ph, e, i, pi);
 		RoeSide sq = roe_side(q.r);
 		RoeAvg a = side ? roe_average(q, sq, pi, si, ph.gam_flux) : roe_average(pi, si, q, sq, ph.gam_flux);
-		double l = b.el[e];
+		double l = edge_geo(b, e)[2];
 		if (o.steady) lam += (sqrt(a.u * a.u + a.v * a.v) + a.c) * l;
 		double sn = side ? -1.0 : 1.0;
 		double am[16];
-		split_face(a, sn * b.enx[e], sn * b.eny[e], ph.gam_flux, l, diag, am);
+		split_face(a, sn * edge_geo(b, e)[0], sn * edge_geo(b, e)[1], ph.gam_flux, l, diag, am);
 		if (j < 0)
 			for (int q2 = 0; q2 < 16; q2++) am[q2] = 0.0;
 		cl[k + 1] = j;

@@ -17,8 +17,11 @@
 //          the faces' outward n l (Green-Gauss weights), both one 48 B
 //          record per cell (three 16-byte loads), cgp[12][nc] the cell's offsets
 //          to its faces' Gauss points (limiter only), flags[nc]
-//   edges  ec1/ec2 (ec2 = -(bc+1) on a boundary), enx, eny, el,
-//          edl[8][ne] Gauss-point offsets (c1: gp1 x,y gp2 x,y; c2: same),
+//   edges  eg[ne][4] one 32 B record per edge: unit normal out of c1 and
+//          length (nx, ny, l), then the two cells c1, c2 as two ints in the
+//          last 8 bytes (c2 = -(bc+1) on a boundary),
+//          edl[ne][8] Gauss-point offsets, per Gauss point g a 32 B
+//          record (c1 x,y, c2 x,y; zero for c2 on a boundary),
 //          F[ne][4] = (flux at gp1 + flux at gp2) * l/2, one record per edge
 //          written once
 // Each RK stage is three passes: gradient (cell gather), edge flux (a lane
@@ -57,8 +60,7 @@ struct Bufs {
 	int* blockflag;
 	int* anyflag;  // [0] set by a final-stage update block holding a LIM cell,
 	               // cleared by remediation; [1] the same, cleared only by the host
-	int *ec1, *ec2;
-	double *enx, *eny, *el, *edl, *F;
+	double *eg, *edl, *F;  // eg [ne][4] (nx, ny, l, (c1, c2) as two ints)
 	int* bctype;
 	double* bcpar;
 	double* mtab;  // 5 per material: M, Cv, gam, mu, kc
@@ -118,6 +120,18 @@ __device__ __forceinline__ void ld8(const double* base, size_t i, double* v)
 {
 	ld4(base, 2 * i, v);
 	ld4(base, 2 * i + 1, v + 4);
+}
+
+// An edge's record (32 bytes, two 16-byte loads): unit normal out of c1,
+// length, and its two cells (c2 = -(bc+1) on a boundary) in the last 8 bytes.
+__device__ __forceinline__ const double* edge_geo(const Bufs& b, size_t e)
+{
+	return static_cast<const double*>(__builtin_assume_aligned(b.eg + 4 * e, 32));
+}
+
+__device__ __forceinline__ int2 edge_cells(const Bufs& b, size_t e)
+{
+	return *reinterpret_cast<const int2*>(edge_geo(b, e) + 3);
 }
 
 // the cell's faces and neighbours (one 16-byte record each)
@@ -198,11 +212,11 @@ template <bool MM>
 __device__ __forceinline__ Prim ghost_of(const Bufs& b, const Phys& ph, int e, int c1, const Prim& inside,
                                          const Prim& centre)
 {
-	int bc = -b.ec2[e] - 1;
+	int bc = -edge_cells(b, e).y - 1;
 	MatV m = mat_of<MM>(b, ph, c1);
 	double Tc = temperature(centre, m);
 	double czc = sqrt(m.gam * centre.p / centre.r);
-	return ghost_state(b.bctype[bc], &b.bcpar[4 * bc], b.enx[e], b.eny[e], inside, Tc, czc, m);
+	return ghost_state(b.bctype[bc], &b.bcpar[4 * bc], edge_geo(b, e)[0], edge_geo(b, e)[1], inside, Tc, czc, m);
 }
 
 // ---------------------------------------------------------------- kernels
@@ -402,9 +416,9 @@ __global__ __launch_bounds__(BLOCK) void k_edge_flux(Bufs b, Phys ph)
 	int t = xcd_block() * BLOCK + threadIdx.x;
 	int e = t >> 1, gp = t & 1;
 	if (e >= b.ne) return;  // both lanes of a pair leave together
-	size_t ne = b.ne;
-	int c1 = b.ec1[e], c2 = b.ec2[e];
-	double nx = b.enx[e], ny = b.eny[e];
+	int2 ce = edge_cells(b, e);
+	int c1 = ce.x, c2 = ce.y;
+	double nx = edge_geo(b, e)[0], ny = edge_geo(b, e)[1];
 	bool two = gp == 1 && c2 >= 0;
 	int cs = two ? c2 : c1;  // the cell whose shared work this lane does
 	double gs[8], go[8];     // gradients of cs, and of the partner's cell
@@ -426,16 +440,16 @@ __global__ __launch_bounds__(BLOCK) void k_edge_flux(Bufs b, Phys ph)
 	const double* g2 = gp ? gs : go;  // c2's (c1's on a boundary)
 	Prim p1 = load_prim(b, c1);
 	Prim L = p1, R;
-	double d1x = b.edl[(size_t)(2 * gp) * ne + e], d1y = b.edl[(size_t)(2 * gp + 1) * ne + e];
-	if (GR) recon_g(L, g1, d1x, d1y);
-	else recon(L, b, c1, d1x, d1y);
+	double dl[4];  // this Gauss point's offsets from c1 and c2 (zero on a boundary)
+	ld4(b.edl, 2 * (size_t)e + gp, dl);
+	if (GR) recon_g(L, g1, dl[0], dl[1]);
+	else recon(L, b, c1, dl[0], dl[1]);
 	Prim p2;
 	if (c2 >= 0) {
 		p2 = load_prim(b, c2);
 		R = p2;
-		double d2x = b.edl[(size_t)(4 + 2 * gp) * ne + e], d2y = b.edl[(size_t)(5 + 2 * gp) * ne + e];
-		if (GR) recon_g(R, g2, d2x, d2y);
-		else recon(R, b, c2, d2x, d2y);
+		if (GR) recon_g(R, g2, dl[2], dl[3]);
+		else recon(R, b, c2, dl[2], dl[3]);
 	} else {
 		p2 = p1;
 		R = ghost_of<MM>(b, ph, e, c1, L, p1);
@@ -494,7 +508,7 @@ __global__ __launch_bounds__(BLOCK) void k_edge_flux(Bufs b, Phys ph)
 	}
 	double o[4];
 	for (int q = 0; q < 4; q++) o[q] = __shfl_xor(f[q], 1);
-	store_edge_flux(b.F, e, gp, f, o, b.el[e] * 0.5);
+	store_edge_flux(b.F, e, gp, f, o, edge_geo(b, e)[2] * 0.5);
 }
 
 // residual of cell i: -F on c1's side, +F on c2's, ascending edge order
@@ -626,7 +640,7 @@ __global__ __launch_bounds__(BLOCK) void k_remediate(Bufs b, Phys ph)
 					int fs[3] = {fc.x, fc.y, fc.z};
 					for (int k = 0; k < 3; k++) {
 						int e = fs[k] >> 1;
-						int j = b.ec2[e];
+						int j = edge_cells(b, e).y;
 						if (j >= 0) {
 							double s = b.S[j];
 							S += s;
@@ -683,7 +697,7 @@ __global__ __launch_bounds__(BLOCK) void k_tau(Bufs b, Phys ph)
 				czj = sqrt(m.gam * q.p / q.r);
 			}
 			double lj = sqrt(q.u * q.u + q.v * q.v) + czj;
-			sum += 0.5 * (li + lj) * b.el[e];
+			sum += 0.5 * (li + lj) * edge_geo(b, e)[2];
 		}
 		loc = 2.0 * ph.cfl * S / sum;
 	} else {
@@ -1127,16 +1141,24 @@ int cfd2d_ctx_create(const cfd2d_mesh* mesh, const cfd2d_params* p, int n_bc, co
 			cfn[6 * (size_t)i + 2 * k] = sg * (m.nx[e] * m.l[e]);
 			cfn[6 * (size_t)i + 2 * k + 1] = sg * (m.ny[e] * m.l[e]);
 		}
+	std::vector<double> eg(4 * (size_t)ne, 0.0);
+	for (int e = 0; e < ne; e++) {
+		eg[4 * (size_t)e] = m.nx[e];
+		eg[4 * (size_t)e + 1] = m.ny[e];
+		eg[4 * (size_t)e + 2] = m.l[e];
+		int cells[2] = {m.c1[e], ec2[e]};
+		memcpy(&eg[4 * (size_t)e + 3], cells, sizeof(cells));
+	}
 	std::vector<double> edl(8 * (size_t)ne, 0.0);
 	for (int e = 0; e < ne; e++) {
 		int c1 = m.c1[e], c2 = m.c2[e];
 		for (int g = 0; g < 2; g++) {
 			double px = m.gp[6 * (size_t)e + 2 + 2 * g], py = m.gp[6 * (size_t)e + 3 + 2 * g];
-			edl[(size_t)(2 * g) * ne + e] = px - m.cx[c1];
-			edl[(size_t)(2 * g + 1) * ne + e] = py - m.cy[c1];
+			edl[8 * (size_t)e + 4 * g] = px - m.cx[c1];
+			edl[8 * (size_t)e + 4 * g + 1] = py - m.cy[c1];
 			if (c2 >= 0) {
-				edl[(size_t)(4 + 2 * g) * ne + e] = px - (m.cx[c2] + m.shx[e]);
-				edl[(size_t)(5 + 2 * g) * ne + e] = py - (m.cy[c2] + m.shy[e]);
+				edl[8 * (size_t)e + 4 * g + 2] = px - (m.cx[c2] + m.shx[e]);
+				edl[8 * (size_t)e + 4 * g + 3] = py - (m.cy[c2] + m.shy[e]);
 			}
 		}
 	}
@@ -1157,9 +1179,7 @@ int cfd2d_ctx_create(const cfd2d_mesh* mesh, const cfd2d_params* p, int n_bc, co
 	    (rc = dalloc(c, &b.G, 2 * n4)) || (rc = dalloc(c, &b.dtS, nc)) || (rc = upload(c, &b.S, m.S)) ||
 	    (rc = dalloc(c, &b.loc, nc)) || (rc = dalloc(c, &b.R, n4)) || (rc = upload(c, &b.cface, cface)) ||
 	    (rc = upload(c, &b.cnb, cnb)) || (rc = upload(c, &b.cfn, cfn)) || (rc = upload(c, &b.cmat, cmat)) || (rc = dalloc(c, &b.flags, nc)) ||
-	    (rc = dalloc(c, &b.blockflag, b.nblocks)) || (rc = dalloc(c, &b.anyflag, 2)) || (rc = upload(c, &b.ec1, m.c1)) ||
-	    (rc = upload(c, &b.ec2, ec2)) || (rc = upload(c, &b.enx, m.nx)) || (rc = upload(c, &b.eny, m.ny)) ||
-	    (rc = upload(c, &b.el, m.l)) || (rc = upload(c, &b.edl, edl)) || (rc = dalloc(c, &b.F, 4 * (size_t)ne)) ||
+	    (rc = dalloc(c, &b.blockflag, b.nblocks)) || (rc = dalloc(c, &b.anyflag, 2)) || (rc = upload(c, &b.eg, eg)) || (rc = upload(c, &b.edl, edl)) || (rc = dalloc(c, &b.F, 4 * (size_t)ne)) ||
 	    (rc = upload(c, &b.bctype, bct)) || (rc = upload(c, &b.bcpar, bcp)) || (rc = upload(c, &b.mtab, mtab)) ||
 	    (rc = dalloc(c, &b.err, 2))) {
 		destroy_ctx(c);
@@ -1174,8 +1194,8 @@ int cfd2d_ctx_create(const cfd2d_mesh* mesh, const cfd2d_params* p, int n_bc, co
 				int f = cface[4 * (size_t)i + k];
 				int e = f >> 1, side = f & 1;
 				for (int g = 0; g < 2; g++) {
-					cgp[(size_t)(2 * (2 * k + g)) * nc + i] = edl[(size_t)(4 * side + 2 * g) * ne + e];
-					cgp[(size_t)(2 * (2 * k + g) + 1) * nc + i] = edl[(size_t)(4 * side + 2 * g + 1) * ne + e];
+					cgp[(size_t)(2 * (2 * k + g)) * nc + i] = edl[8 * (size_t)e + 4 * g + 2 * side];
+					cgp[(size_t)(2 * (2 * k + g) + 1) * nc + i] = edl[8 * (size_t)e + 4 * g + 2 * side + 1];
 				}
 			}
 		if ((rc = upload(c, &b.cgp, cgp))) {
