@@ -162,7 +162,8 @@ def batched_vs_serial(n_cases, steps, seed=7):
                 serial_graph=rate(t_graph), ratio_vs_eager=t_serial / t_batch, ratio_vs_graph=t_graph / t_batch,
                 unit="cell-updates/s",
                 autotune=dict(batch=tuned, serial_fused=sum(p["fuse"] for p in picks),
-                              serial_graph=sum(p["graph"] for p in picks), serial_cases=len(picks)))
+                              serial_graph=sum(p["graph"] for p in picks),
+                              serial_stream=sum(p["stream"] for p in picks), serial_cases=len(picks)))
 
 
 def adjoint_vs_forward(steps, ckpt, ny=289):

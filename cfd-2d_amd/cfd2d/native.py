@@ -88,8 +88,9 @@ SIGNATURES = {
     "cfd2d_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "cfd2d_timing_kernels": (C.c_int, [C.c_void_p, C.c_uint]),
     "cfd2d_fuse_gradient": (C.c_int, [C.c_void_p, C.c_int]),
+    "cfd2d_stream_state": (C.c_int, [C.c_void_p, C.c_int]),
     "cfd2d_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                 C.POINTER(C.c_double)]),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "cfd2d_timing_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "cfd2d_ctx_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cfd2d_ctx_stream": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
@@ -368,12 +369,20 @@ class Context:
         m = -1 if mode is None else (1 if mode else 0)
         check(lib().cfd2d_fuse_gradient(self._h, m))
 
+    def stream_state(self, mode=True):
+        """Whether the update pass reads and writes the conserved state
+        non-temporally: True/1 yes, False/0 no, None/-1 the default (yes when
+        the context's arrays exceed twice the 256 MB Infinity Cache); the same
+        results."""
+        m = -1 if mode is None else (1 if mode else 0)
+        check(lib().cfd2d_stream_state(self._h, m))
+
     def autotune(self, trial_steps=3):
         """Times the step variants on the current state (restored afterwards)
-        and keeps the fastest; returns dict(fuse, graph, ms_per_step)."""
-        f, g, ms = C.c_int(), C.c_int(), C.c_double()
-        check(lib().cfd2d_autotune(self._h, int(trial_steps), C.byref(f), C.byref(g), C.byref(ms)))
-        return dict(fuse=f.value, graph=g.value, ms_per_step=ms.value)
+        and keeps the fastest; returns dict(fuse, graph, stream, ms_per_step)."""
+        f, g, st, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        check(lib().cfd2d_autotune(self._h, int(trial_steps), C.byref(f), C.byref(g), C.byref(st), C.byref(ms)))
+        return dict(fuse=f.value, graph=g.value, stream=st.value, ms_per_step=ms.value)
 
     def timing(self, enable=True, kernels=None):
         """Per-kernel HIP-event timing on or off (counters reset); kernels =

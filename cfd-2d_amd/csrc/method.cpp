@@ -343,7 +343,7 @@ void FVM_TVD::run()
 	if (!ctx) throw Exception("run() before init()", 1);
 	// long runs first let the autotuner pick the step variant (it restores
 	// the state it trials on, and every variant gives the same results)
-	if (STEP_MAX >= 200) check(cfd2d_autotune(ctx, 3, nullptr, nullptr, nullptr));
+	if (STEP_MAX >= 200) check(cfd2d_autotune(ctx, 3, nullptr, nullptr, nullptr, nullptr));
 	double t = 0.0;
 	long step = 0;
 	while (t < TMAX && step < STEP_MAX) {

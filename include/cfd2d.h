@@ -151,13 +151,21 @@ int  cfd2d_timing(cfd2d_ctx* c, int enable);
  * whose records the edge pass reads, -1 (default) 1 without a limiter and 0
  * with one.  All modes give the same bits.  No reference counterpart. */
 int  cfd2d_fuse_gradient(cfd2d_ctx* c, int mode);
+/* Whether the update pass reads and writes the conserved state with
+ * non-temporal loads and stores (kept out of the caches, leaving them to the
+ * data the edge pass gathers): 1 yes, 0 no, -1 (default) when the context's
+ * device arrays are over twice the 256 MB Infinity Cache.  The same bits
+ * either way.  No reference counterpart. */
+int  cfd2d_stream_state(cfd2d_ctx* c, int mode);
 /* Autotuner (no reference counterpart): times trial_steps steps of each
  * variant -- gradients in the edge pass or in their own pass, eager launches
- * or a replayed recorded step -- on the context's current state, restores
- * that state exactly, and keeps the fastest for cfd2d_step (same results
- * either way).  Reports the choice and its ms per step (any pointer may be
- * NULL).  Uses a temporary device copy of the state (~100 B per cell). */
-int  cfd2d_autotune(cfd2d_ctx* c, int trial_steps, int* fuse_mode, int* use_graph, double* ms_per_step);
+ * or a replayed recorded step, cached or streamed state in the update pass
+ * -- on the context's current state, restores that state exactly, and
+ * keeps the fastest for cfd2d_step (same results either way).  Reports the
+ * choice and its ms per step (any pointer may be NULL).  Uses a temporary
+ * device copy of the state (~100 B per cell). */
+int  cfd2d_autotune(cfd2d_ctx* c, int trial_steps, int* fuse_mode, int* use_graph, int* stream_state,
+                    double* ms_per_step);
 /* which kernels get events while timing is on: bit k = CFD2D_K_* id
  * (default all); fewer events disturb the stream less */
 int  cfd2d_timing_kernels(cfd2d_ctx* c, unsigned mask);

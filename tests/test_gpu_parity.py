@@ -132,6 +132,24 @@ def test_fused_gradient_edge_pass_is_bitwise(opts):
     assert np.array_equal(outs[0], outs[1])
 
 
+@pytest.mark.parametrize("name,opts", [("inlet", dict()), ("bounds", dict(limiter="MINMOD")),
+                                       ("twomat", dict(gradient="LSQ", rk_stages=3))], ids=str)
+def test_streamed_state_is_bitwise(name, opts):
+    """Non-temporal loads and stores of the state in the update pass change
+    only where the data is cached, not the values (RK2 and RK3 stages, LIM
+    flags, two materials)."""
+    case = gc.case(name, **opts)
+    outs = []
+    for mode in (True, False, None):
+        s = native.GpuSolver(case)
+        s.stream_state(mode)
+        s.run(25)
+        outs.append((s.state(), s.flags()))
+        s.close()
+    for o in outs[1:]:
+        assert np.array_equal(outs[0][0], o[0]) and np.array_equal(outs[0][1], o[1])
+
+
 def test_fused_limited_gradient_is_bitwise():
     """With the minmod limiter too (mode 1 forms and limits the gradients in
     the edge pass; the default runs the separate pass)."""
@@ -156,7 +174,8 @@ def test_autotune_restores_state_and_keeps_results(name, opts):
     b.run(7)
     U, F = a.state(), a.flags()
     choice = a.autotune(trial_steps=2)
-    assert choice["fuse"] in (0, 1) and choice["graph"] in (0, 1) and choice["ms_per_step"] > 0
+    assert choice["fuse"] in (0, 1) and choice["graph"] in (0, 1) and choice["stream"] in (0, 1)
+    assert choice["ms_per_step"] > 0
     assert np.array_equal(a.state(), U) and np.array_equal(a.flags(), F)
     assert a.status() == b.status()
     a.run(20)
