@@ -267,10 +267,10 @@ __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, 
 	}
 	Prim p1 = load_prim(b, c1);
 	Prim L = p1, R;
-	recon(L, b, c1, b.edl[8 * (size_t)e + 4 * gp], b.edl[8 * (size_t)e + 4 * gp + 1]);
+	recon(L, b, c1, b.edl[edl_index(e, 0, gp)], b.edl[edl_index(e, 0, gp) + 1]);
 	if (c2 >= 0) {
 		R = load_prim(b, c2);
-		recon(R, b, c2, b.edl[8 * (size_t)e + 4 * gp + 2], b.edl[8 * (size_t)e + 4 * gp + 3]);
+		recon(R, b, c2, b.edl[edl_index(e, 1, gp)], b.edl[edl_index(e, 1, gp) + 1]);
 	} else {
 		R = ghost_of<MM>(b, ph, e, c1, L, p1);
 	}
@@ -338,13 +338,13 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 			for (int q = 0; q < 4; q++) fb[q] = -wi[q] * hl;
 		}
 		int c1 = side ? j : i, c2 = side ? i : j;
-		double d1x = b.edl[8 * (size_t)e + 4 * gp], d1y = b.edl[8 * (size_t)e + 4 * gp + 1];
+		double d1x = b.edl[edl_index(e, 0, gp)], d1y = b.edl[edl_index(e, 0, gp) + 1];
 		Prim L = side ? pj : pi, R;
 		recon(L, b, c1, d1x, d1y);
 		double d2x = 0.0, d2y = 0.0;
 		if (j >= 0) {
-			d2x = b.edl[8 * (size_t)e + 4 * gp + 2];
-			d2y = b.edl[8 * (size_t)e + 4 * gp + 3];
+			d2x = b.edl[edl_index(e, 1, gp)];
+			d2y = b.edl[edl_index(e, 1, gp) + 1];
 			R = side ? pi : pj;
 			recon(R, b, c2, d2x, d2y);
 		} else {
@@ -393,8 +393,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_cell(Bufs b, AdjBufs a)
 		int f = tp.f[k];
 		int e = f >> 1, side = f & 1;
 		for (int gp = 0; gp < 2; gp++) {
-			double dx = b.edl[8 * (size_t)e + 4 * gp + 2 * side];
-			double dy = b.edl[8 * (size_t)e + 4 * gp + 2 * side + 1];
+			double dx = b.edl[edl_index(e, side, gp)];
+			double dy = b.edl[edl_index(e, side, gp) + 1];
 			for (int q = 0; q < 4; q++) {
 				double v = a.Ea[(size_t)(8 * side + 4 * gp + q) * ne + e];
 				pb[q] += v;

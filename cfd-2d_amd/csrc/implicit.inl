@@ -76,9 +76,9 @@ __global__ __launch_bounds__(BLOCK) void k_impl_edge(Bufs b, Phys ph)
 	double nx = edge_geo(b, e)[0], ny = edge_geo(b, e)[1];
 	Prim L = load_prim(b, c1), R;
 	if (c2 >= 0) {
-		recon(L, b, c1, b.edl[8 * (size_t)e + 4 * gp], b.edl[8 * (size_t)e + 4 * gp + 1]);
+		recon(L, b, c1, b.edl[edl_index(e, 0, gp)], b.edl[edl_index(e, 0, gp) + 1]);
 		R = load_prim(b, c2);
-		recon(R, b, c2, b.edl[8 * (size_t)e + 4 * gp + 2], b.edl[8 * (size_t)e + 4 * gp + 3]);
+		recon(R, b, c2, b.edl[edl_index(e, 1, gp)], b.edl[edl_index(e, 1, gp) + 1]);
 	} else {
 		R = ghost_impl_of<MM>(b, ph, e, c1, L);
 	}

@@ -20,8 +20,10 @@
 //   edges  eg[ne][4] one 32 B record per edge: unit normal out of c1 and
 //          length (nx, ny, l), then the two cells c1, c2 as two ints in the
 //          last 8 bytes (c2 = -(bc+1) on a boundary),
-//          edl[ne][8] Gauss-point offsets, per Gauss point g a 32 B
-//          record (c1 x,y, c2 x,y; zero for c2 on a boundary),
+//          edl[ne][2][4] Gauss-point offsets from the cell centroids, one
+//          32 B record per lane of the edge's pair (edl_index): [0] c1 at
+//          Gauss points 1, 2; [1] c2 at Gauss points 2, 1 (on a boundary
+//          c1 at Gauss point 2, then zeros),
 //          F[ne][4] = (flux at gp1 + flux at gp2) * l/2, one record per edge
 //          written once
 // Each RK stage is three passes: gradient (cell gather), edge flux (a lane
@@ -132,6 +134,15 @@ __device__ __forceinline__ const double* edge_geo(const Bufs& b, size_t e)
 __device__ __forceinline__ int2 edge_cells(const Bufs& b, size_t e)
 {
 	return *reinterpret_cast<const int2*>(edge_geo(b, e) + 3);
+}
+
+// Where the offset (x, y) of Gauss point g (0, 1) from the centroid of the
+// edge's side-th cell lies in edl: each lane of the edge's pair reads one
+// 32-byte record holding its own cell's offsets at its own Gauss point
+// first and at the partner's second.
+__host__ __device__ __forceinline__ size_t edl_index(size_t e, int side, int g)
+{
+	return side ? 8 * e + 4 + 2 * (1 - g) : 8 * e + 2 * g;
 }
 
 // the cell's faces and neighbours (one 16-byte record each)
@@ -372,13 +383,6 @@ __device__ __forceinline__ void grad_T_g(const double* g, const Prim& q, const M
 	ty = (g[3] - pr * g[1]) * s;
 }
 
-__device__ __forceinline__ void grad_T(const Bufs& b, int c, const Prim& q, const MatV& m, double& tx, double& ty)
-{
-	double g[4];
-	ld4(b.G, 2 * (size_t)c, g);  // rx ry px py
-	grad_T_g(g, q, m, tx, ty);
-}
-
 // The lane pair's (f_gp1 + f_gp2) l/2 as the edge's 32-byte flux record: the
 // even lane adds its partner's flux to its own (the reference's order), the
 // odd lane the other way round (the same sums), and lane gp writes
@@ -395,21 +399,23 @@ __device__ __forceinline__ void store_edge_flux(double* F, int e, int gp, const 
 
 // Edge flux pass, FVM_TVD::run edge loop + reconstruct + calcFlux
 // (src/methods/fvm_tvd.cpp:329-352, 646-691, 602-622).  Two adjacent lanes
-// serve one edge, one Gauss point each: each reconstructs (r, p, u, v)
-// linearly from both sides' centroids (the ghost of the reconstructed left
-// state on a boundary), evaluates the numerical flux and subtracts the
-// viscous flux (extension); the even lane adds its partner's flux to its own
-// (f1 + f2, the reference's order) and writes (f1 + f2) * l/2 once.  Halving
-// the live state per lane doubles the waves a SIMD can hold.
+// serve one edge, one Gauss point each: lane 0 works with c1, lane 1 with c2
+// (c1 on a boundary); each reconstructs (r, p, u, v) of its cell linearly at
+// both Gauss points and the pair swaps one of them, so each lane holds both
+// sides at its Gauss point (the ghost of the reconstructed left state on a
+// boundary); each evaluates the numerical flux and subtracts the viscous flux
+// (extension); the even lane adds its partner's flux to its own (f1 + f2,
+// the reference's order) and writes (f1 + f2) * l/2 once.  Halving the live
+// state per lane doubles the waves a SIMD can hold.
 //
 // GR = 0 reads the gradients the gradient pass stored; GR = 1 (LSQ) or 2
-// (Green-Gauss), 3 or 4 with the minmod limiter, has the pair form them itself: lane 0 the gradient of c1,
-// lane 1 that of c2 (c1's on a boundary), with the gradient pass's own code,
-// and the lanes swap them.  Every cell's gradient is then formed by each of
-// its edges (three times) instead of once, but the gradient pass, its 64-byte
-// record per cell written and read back, and its launch disappear; the extra
-// arithmetic runs inside a pass that is bound by FP64 issue anyway, beside its
-// memory traffic.  Results are the same bits as with GR = 0.
+// (Green-Gauss), 3 or 4 with the minmod limiter, has each lane form its own
+// cell's gradient with the gradient pass's own code.  Every cell's gradient
+// is then formed by each of its edges (three times) instead of once, but the
+// gradient pass, its 64-byte record per cell written and read back, and its
+// launch disappear; the extra arithmetic runs inside a pass that is bound by
+// FP64 issue anyway, beside its memory traffic.  Results are the same bits as
+// with GR = 0.
 template <bool MM, int FLUX, bool VISC, int GR>
 __global__ __launch_bounds__(BLOCK) void k_edge_flux(Bufs b, Phys ph)
 {
@@ -419,71 +425,69 @@ __global__ __launch_bounds__(BLOCK) void k_edge_flux(Bufs b, Phys ph)
 	int2 ce = edge_cells(b, e);
 	int c1 = ce.x, c2 = ce.y;
 	double nx = edge_geo(b, e)[0], ny = edge_geo(b, e)[1];
-	bool two = gp == 1 && c2 >= 0;
-	int cs = two ? c2 : c1;  // the cell whose shared work this lane does
-	double gs[8], go[8];     // gradients of cs, and of the partner's cell
-	Prim ps;
+	bool inner = c2 >= 0;
+	int cs = gp == 1 && inner ? c2 : c1;  // the cell this lane reconstructs
+	double gs[8];                          // its gradients
+	Prim ps;                               // and primitives
 	if (GR) {
 		int face[3];
 		Prim nb[3];
 		cell_gradient<MM, GR == 1 || GR == 3>(b, ph, cs, ps, face, nb, gs);
 		if (GR >= 3) minmod_limit(b, cs, face, ps, nb, gs);
-		for (int q = 0; q < 8; q++) go[q] = __shfl_xor(gs[q], 1);
 		// when the records are wanted, each cell's is stored by the lane that
 		// formed it on the cell's first face (lowest edge index): once
-		if (ph.store_g && (face[0] >> 1) == e && !(gp == 1 && c2 < 0)) {
+		if (ph.store_g && (face[0] >> 1) == e && !(gp == 1 && !inner)) {
 			double* o = static_cast<double*>(__builtin_assume_aligned(b.G + 8 * (size_t)cs, 32));
 			for (int q = 0; q < 8; q++) o[q] = gs[q];
 		}
-	}
-	const double* g1 = gp ? go : gs;  // c1's gradients (GR != 0)
-	const double* g2 = gp ? gs : go;  // c2's (c1's on a boundary)
-	Prim p1 = load_prim(b, c1);
-	Prim L = p1, R;
-	double dl[4];  // this Gauss point's offsets from c1 and c2 (zero on a boundary)
-	ld4(b.edl, 2 * (size_t)e + gp, dl);
-	if (GR) recon_g(L, g1, dl[0], dl[1]);
-	else recon(L, b, c1, dl[0], dl[1]);
-	Prim p2;
-	if (c2 >= 0) {
-		p2 = load_prim(b, c2);
-		R = p2;
-		if (GR) recon_g(R, g2, dl[2], dl[3]);
-		else recon(R, b, c2, dl[2], dl[3]);
 	} else {
-		p2 = p1;
-		R = ghost_of<MM>(b, ph, e, c1, L, p1);
+		ps = load_prim(b, cs);
+		ld8(b.G, cs, gs);
+	}
+	// Each lane extrapolates its own cell to its own Gauss point (kept) and to
+	// the partner's (sent); on an inner edge the pair swaps the sent states,
+	// so lane 0 holds (c1, c2) at Gauss point 1 and lane 1 at Gauss point 2 --
+	// the same operations as each lane reconstructing both cells.  On a
+	// boundary both lanes reconstruct c1 and build the ghost from it.
+	double dl[4];  // this lane's record: offsets of cs at its own, then the partner's Gauss point
+	ld4(b.edl, 2 * (size_t)e + gp, dl);
+	Prim keep = ps, send = ps, recv;
+	recon_g(keep, gs, dl[0], dl[1]);
+	recon_g(send, gs, dl[2], dl[3]);
+	recv.r = __shfl_xor(send.r, 1);
+	recv.p = __shfl_xor(send.p, 1);
+	recv.u = __shfl_xor(send.u, 1);
+	recv.v = __shfl_xor(send.v, 1);
+	Prim L, R;
+	if (inner) {
+		L = gp ? recv : keep;
+		R = gp ? keep : recv;
+	} else {
+		L = keep;
+		R = ghost_of<MM>(b, ph, e, c1, L, ps);
 	}
 	// Viscous terms first (only sx, sy and the energy term stay live through
 	// the Riemann solve).  Face gradient = mean of the two cell gradients (the
 	// cell's own on a boundary); face velocity = mean of the reconstructed
-	// states.  The pair's lanes share the cell work: lane 0 takes c1's
-	// velocity gradient and forms its grad T, lane 1 does c2's (c1's on a
-	// boundary), and they swap them; the sums are the same as one lane
-	// forming both.
+	// states.  Each lane forms grad T of its own cell and the pair swaps the
+	// velocity and temperature gradients; a + b = b + a, so both lanes form
+	// the same means.
 	double sx = 0.0, sy = 0.0, ev = 0.0;
 	if (VISC) {
 		MatV m = mat_of<MM>(b, ph, c1);
 		double own[6];
-		if (GR) {
-			for (int q = 0; q < 4; q++) own[q] = gs[4 + q];  // ux uy vx vy
-			grad_T_g(gs, ps, mat_of<MM>(b, ph, cs), own[4], own[5]);
-		} else {
-			ld4(b.G, 2 * (size_t)cs + 1, own);  // ux uy vx vy
-			grad_T(b, cs, two ? p2 : p1, mat_of<MM>(b, ph, cs), own[4], own[5]);
-		}
+		for (int q = 0; q < 4; q++) own[q] = gs[4 + q];  // ux uy vx vy
+		grad_T_g(gs, ps, mat_of<MM>(b, ph, cs), own[4], own[5]);
 		double oth[6];
 		for (int q = 0; q < 6; q++) oth[q] = __shfl_xor(own[q], 1);
-		const double* v1 = gp ? oth : own;  // c1's values
-		const double* v2 = gp ? own : oth;  // c2's values (c1's on a boundary)
-		double ux = v1[0], uy = v1[1], vx = v1[2], vy = v1[3], tx = v1[4], ty = v1[5];
-		if (c2 >= 0) {
-			ux = 0.5 * (ux + v2[0]);
-			uy = 0.5 * (uy + v2[1]);
-			vx = 0.5 * (vx + v2[2]);
-			vy = 0.5 * (vy + v2[3]);
-			tx = 0.5 * (tx + v2[4]);
-			ty = 0.5 * (ty + v2[5]);
+		double ux = own[0], uy = own[1], vx = own[2], vy = own[3], tx = own[4], ty = own[5];
+		if (inner) {
+			ux = 0.5 * (ux + oth[0]);
+			uy = 0.5 * (uy + oth[1]);
+			vx = 0.5 * (vx + oth[2]);
+			vy = 0.5 * (vy + oth[3]);
+			tx = 0.5 * (tx + oth[4]);
+			ty = 0.5 * (ty + oth[5]);
 		}
 		double div = ux + vy;
 		double txx = m.mu * (2.0 * ux - (2.0 / 3.0) * div);
@@ -1192,11 +1196,14 @@ int cfd2d_ctx_create(const cfd2d_mesh* mesh, const cfd2d_params* p, int n_bc, co
 		int c1 = m.c1[e], c2 = m.c2[e];
 		for (int g = 0; g < 2; g++) {
 			double px = m.gp[6 * (size_t)e + 2 + 2 * g], py = m.gp[6 * (size_t)e + 3 + 2 * g];
-			edl[8 * (size_t)e + 4 * g] = px - m.cx[c1];
-			edl[8 * (size_t)e + 4 * g + 1] = py - m.cy[c1];
+			edl[edl_index(e, 0, g)] = px - m.cx[c1];
+			edl[edl_index(e, 0, g) + 1] = py - m.cy[c1];
 			if (c2 >= 0) {
-				edl[8 * (size_t)e + 4 * g + 2] = px - (m.cx[c2] + m.shx[e]);
-				edl[8 * (size_t)e + 4 * g + 3] = py - (m.cy[c2] + m.shy[e]);
+				edl[edl_index(e, 1, g)] = px - (m.cx[c2] + m.shx[e]);
+				edl[edl_index(e, 1, g) + 1] = py - (m.cy[c2] + m.shy[e]);
+			} else if (g == 1) {  // the second lane of a boundary edge reconstructs c1
+				edl[8 * (size_t)e + 4] = px - m.cx[c1];
+				edl[8 * (size_t)e + 5] = py - m.cy[c1];
 			}
 		}
 	}
@@ -1232,8 +1239,8 @@ int cfd2d_ctx_create(const cfd2d_mesh* mesh, const cfd2d_params* p, int n_bc, co
 				int f = cface[4 * (size_t)i + k];
 				int e = f >> 1, side = f & 1;
 				for (int g = 0; g < 2; g++) {
-					cgp[(size_t)(2 * (2 * k + g)) * nc + i] = edl[8 * (size_t)e + 4 * g + 2 * side];
-					cgp[(size_t)(2 * (2 * k + g) + 1) * nc + i] = edl[8 * (size_t)e + 4 * g + 2 * side + 1];
+					cgp[(size_t)(2 * (2 * k + g)) * nc + i] = edl[edl_index(e, side, g)];
+					cgp[(size_t)(2 * (2 * k + g) + 1) * nc + i] = edl[edl_index(e, side, g) + 1];
 				}
 			}
 		if ((rc = upload(c, &b.cgp, cgp))) {
