@@ -57,6 +57,7 @@ constexpr int NXCD = 8;
 struct Bufs {
 	int nc, ne, nblocks, nmat;
 	double *U, *U0, *P, *Po, *G, *dtS, *S, *lsqw, *cfn, *cgp, *loc, *R;
+	double* Un;  // where the update pass writes the stage's state (launch_update)
 	int *cface, *cnb, *cmat;
 	unsigned* flags;
 	int* blockflag;
@@ -566,9 +567,11 @@ __device__ __forceinline__ void st_state(double* p, double v)
 }
 
 // Update pass (fvm_tvd.cpp:366-374, 419-447): LIM-flagged cells stay frozen;
-// the first stage also saves U0 (the U_old copy of 318-321); the last stage
-// forms the RK combination, refreshes the primitives and sets the bound
-// flags (clamped values are not written back, 439-446).  NT: see ld_state.
+// the last stage forms the RK combination, refreshes the primitives and sets
+// the bound flags (clamped values are not written back, 439-446).  The stage
+// reads its input from b.U and the step-start state from b.U0 and writes
+// b.Un; launch_update points them so that the step-start state is never
+// copied (the reference's U_old copy, 318-321).  NT: see ld_state.
 template <bool MM, int MODE, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_update(Bufs b, Phys ph)
 {
@@ -583,8 +586,8 @@ __global__ __launch_bounds__(BLOCK) void k_update(Bufs b, Phys ph)
 		bool lim = (fl & 0x2u) != 0;
 		double u[4], u0[4];
 		for (int q = 0; q < 4; q++) u[q] = ld_state<NT>(&b.U[q * n + i]);
-		if (MODE == UPD_FIRST)
-			for (int q = 0; q < 4; q++) st_state<NT>(&b.U0[q * n + i], u[q]);
+		if (MODE == UPD_FIRST && lim)  // a frozen cell's state still moves to the stage buffer
+			for (int q = 0; q < 4; q++) st_state<NT>(&b.Un[q * n + i], u[q]);
 		if (!lim) {
 			double r[4];
 			gather_residual(b, i, r);
@@ -600,7 +603,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(Bufs b, Phys ph)
 			}
 			bool finite = true;
 			for (int q = 0; q < 4; q++) {
-				st_state<NT>(&b.U[q * n + i], u[q]);
+				st_state<NT>(&b.Un[q * n + i], u[q]);
 				finite = finite && isfinite(u[q]);
 			}
 			if (!finite) atomicAdd(&b.err[1], 1ull);
@@ -941,13 +944,13 @@ void launch_gradient_edge(cfd2d_ctx* c)
 }
 
 template <bool MM, bool NT>
-void launch_update_nt(cfd2d_ctx* c, int mode)
+void launch_update_nt(cfd2d_ctx* c, int mode, const Bufs& b)
 {
 	switch (mode) {
-	case UPD_FIRST: LAUNCH(c, CFD2D_K_UPDATE, (k_update<MM, UPD_FIRST, NT>), c->nc, c->b, c->ph); break;
-	case UPD_RK2_LAST: LAUNCH(c, CFD2D_K_UPDATE, (k_update<MM, UPD_RK2_LAST, NT>), c->nc, c->b, c->ph); break;
-	case UPD_RK3_MID: LAUNCH(c, CFD2D_K_UPDATE, (k_update<MM, UPD_RK3_MID, NT>), c->nc, c->b, c->ph); break;
-	default: LAUNCH(c, CFD2D_K_UPDATE, (k_update<MM, UPD_RK3_LAST, NT>), c->nc, c->b, c->ph); break;
+	case UPD_FIRST: LAUNCH(c, CFD2D_K_UPDATE, (k_update<MM, UPD_FIRST, NT>), c->nc, b, c->ph); break;
+	case UPD_RK2_LAST: LAUNCH(c, CFD2D_K_UPDATE, (k_update<MM, UPD_RK2_LAST, NT>), c->nc, b, c->ph); break;
+	case UPD_RK3_MID: LAUNCH(c, CFD2D_K_UPDATE, (k_update<MM, UPD_RK3_MID, NT>), c->nc, b, c->ph); break;
+	default: LAUNCH(c, CFD2D_K_UPDATE, (k_update<MM, UPD_RK3_LAST, NT>), c->nc, b, c->ph); break;
 	}
 }
 
@@ -956,11 +959,22 @@ void launch_update_nt(cfd2d_ctx* c, int mode)
 // cfd2d_autotune set it
 bool stream_state(const cfd2d_ctx* c) { return c->stream_mode < 0 ? c->stream_auto : c->stream_mode != 0; }
 
+// The state of a step lives in U; U0 is the stage buffer.  The first stage
+// reads U and writes U0, so U keeps the step-start state; later stages read
+// U0 (their input) and U (the step start) and write U0, the last one U:
+// every stage reads and writes each cell's values in one lane, so the
+// in-place stages are safe, and no step-start copy is made.
 template <bool MM>
 void launch_update(cfd2d_ctx* c, int mode)
 {
-	if (stream_state(c)) launch_update_nt<MM, true>(c, mode);
-	else launch_update_nt<MM, false>(c, mode);
+	Bufs b = c->b;
+	double *state = c->b.U, *stage = c->b.U0;
+	bool last = mode == UPD_RK2_LAST || mode == UPD_RK3_LAST;
+	b.U = mode == UPD_FIRST ? state : stage;
+	b.U0 = state;
+	b.Un = last ? state : stage;
+	if (stream_state(c)) launch_update_nt<MM, true>(c, mode, b);
+	else launch_update_nt<MM, false>(c, mode, b);
 }
 
 template <bool MM>
