@@ -16,6 +16,28 @@ struct MatV {
 	double M, Cv, gam, mu, kc;  // molar mass, Cv, gamma, viscosity, conductivity
 };
 
+// Square root for the device passes: the hardware reciprocal square root
+// refined by two coupled Newton-Raphson steps for g = sqrt(x) and
+// h = 1/(2 sqrt(x)) (Markstein), the operations the compiler's sqrt performs
+// for x >= 2^-767, without its rescaling of smaller arguments (no density,
+// pressure or energy here comes near 1e-231); 0 and +inf return themselves,
+// a negative x gives NaN.  The explicit passes give the same bits with
+// either; the implicit assembly differs by 1 ulp in places, where the
+// compiler arranged the surrounding arithmetic differently.
+__device__ __forceinline__ double sqrt_dev(double x)
+{
+	double y = __builtin_amdgcn_rsq(x);
+	double g = x * y, h = y * 0.5;
+	double r = fma(-h, g, 0.5);
+	g = fma(g, r, g);
+	double d = fma(-g, g, x);
+	h = fma(h, r, h);
+	g = fma(d, h, g);
+	d = fma(-g, g, x);
+	g = fma(d, h, g);
+	return (x == 0.0 || x == INFINITY) ? x : g;
+}
+
 struct Prim {
 	double r, p, u, v;
 };
@@ -155,7 +177,7 @@ __device__ __forceinline__ double pow_ogam(double x, const RimConst& K)
 // = x^2 sqrt(x) for GAM = 1.4
 __device__ __forceinline__ double pow_iagam(double x, const RimConst& K)
 {
-	if (K.gamma14 && x >= 0.0) return x * x * sqrt(x);
+	if (K.gamma14 && x >= 0.0) return x * x * sqrt_dev(x);
 	return exp(log(x) * K.IAGAM);
 }
 
@@ -170,8 +192,8 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 	double US = 0.0, UF = 0.0;
 	double RF, RS, EF, ES, SBL, SFL, SSL, SEL;
 	const double IRB = 1.0 / RB, IRE = 1.0 / RE;
-	double CB = sqrt(GAM * PB * IRB);
-	double CE = sqrt(GAM * PE * IRE);
+	double CB = sqrt_dev(GAM * PB * IRB);
+	double CE = sqrt_dev(GAM * PE * IRE);
 	double EB = CB * CB * K.ISGAM;
 	double EE = CE * CE * K.ISGAM;
 	const double IPB = 1.0 / PB, IPE = 1.0 / PE;
@@ -198,7 +220,7 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 				FS1 = ZFB / (GAM * P);
 			} else {
 				double PKB = PGAM * PPB + OGAM;
-				double ZNB = RCB * sqrt(PKB);
+				double ZNB = RCB * sqrt_dev(PKB);
 				double IZ = 1.0 / (ZNB * PKB);
 				F1 = (P - PB) * PKB * IZ;
 				FS1 = (QGAM * PPB + FGAM) * IZ * K.IRGAM;
@@ -210,7 +232,7 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 				FS2 = ZFE / (GAM * P);
 			} else {
 				double PKE = PGAM * PPE + OGAM;
-				double ZNE = RCE * sqrt(PKE);
+				double ZNE = RCE * sqrt_dev(PKE);
 				double IZ = 1.0 / (ZNE * PKE);
 				F2 = (P - PE) * PKE * IZ;
 				FS2 = (QGAM * PPE + FGAM) * IZ * K.IRGAM;
@@ -234,7 +256,7 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 			SBL = UB - CB;
 			SFL = UF - ZFB;
 		} else {
-			double D = UB - sqrt((TGAM * P + HGAM * PB) * IRB);
+			double D = UB - sqrt_dev((TGAM * P + HGAM * PB) * IRB);
 			double UBD = UB - D;
 			double RUBD = RB * UBD;
 			double Q = PB - P + RUBD * UBD;
@@ -253,7 +275,7 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 			SSL = US + ZFE;
 			SEL = UE + CE;
 		} else {
-			double D = UE + sqrt((TGAM * P + HGAM * PE) * IRE);
+			double D = UE + sqrt_dev((TGAM * P + HGAM * PE) * IRE);
 			double UED = UE - D;
 			double RUED = RE * UED;
 			double Q = PE - P + RUED * UED;
@@ -276,12 +298,12 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 			RI = RS; EI = ES; UI = US; VI = VE;
 		}
 	} else if (SFL > 0.0) {
-		UI = (UB + K.DGGAM * sqrt(EB)) / K.DGAM1;
+		UI = (UB + K.DGGAM * sqrt_dev(EB)) / K.DGAM1;
 		VI = VB;
 		EI = (UI * UI) / SGAM;
 		RI = RB * pow_iagam(EI / EB, K);
 	} else {
-		UI = (UE - K.DGGAM * sqrt(EE)) / K.DGAM1;
+		UI = (UE - K.DGGAM * sqrt_dev(EE)) / K.DGAM1;
 		VI = VE;
 		EI = (UI * UI) / SGAM;
 		RI = RE * pow_iagam(EI / EE, K);
@@ -312,8 +334,8 @@ __device__ __forceinline__ void flux_lax(double* f, const Prim& L, const Prim& R
 {
 	double unl = L.u * nx + L.v * ny;
 	double unr = R.u * nx + R.v * ny;
-	double al = fabs(unl) + sqrt(GAM * L.p / L.r);
-	double ar = fabs(unr) + sqrt(GAM * R.p / R.r);
+	double al = fabs(unl) + sqrt_dev(GAM * L.p / L.r);
+	double ar = fabs(unr) + sqrt_dev(GAM * R.p / R.r);
 	double alpha = al > ar ? al : ar;
 	double rol = L.r, rul = L.r * L.u, rvl = L.r * L.v;
 	double rel = L.p / (GAM - 1.0) + 0.5 * L.r * (L.u * L.u + L.v * L.v);
@@ -358,13 +380,13 @@ __device__ __forceinline__ Prim ghost_implicit(int type, const double* par, doub
 // divisions of the reference's expressions become products with
 // reciprocals: the results differ from them by rounding only.
 struct RoeSide {
-	double s, is;  // sqrt(rho), 1/sqrt(rho)
+	double s, is;  // sqrt_dev(rho), 1/sqrt_dev(rho)
 };
 
 __device__ __forceinline__ RoeSide roe_side(double r)
 {
 	RoeSide x;
-	x.s = sqrt(r);
+	x.s = sqrt_dev(r);
 	x.is = 1.0 / x.s;
 	return x;
 }
@@ -388,7 +410,7 @@ __device__ __forceinline__ RoeAvg roe_average(const Prim& L, RoeSide sl, const P
 	double r = sl.s * sr.s;
 	double p = (HI - q2 * 0.5) * r * (AGAM / GAM);
 	double por = p * (sl.is * sr.is);  // p / r
-	a.c = sqrt(GAM * por);
+	a.c = sqrt_dev(GAM * por);
 	a.ic = 1.0 / a.c;
 	a.H = por * kH + 0.5 * q2;
 	return a;
@@ -442,7 +464,7 @@ __device__ __forceinline__ void split_face(const RoeAvg& a, double nx, double ny
 __device__ __forceinline__ void flux_hllc(double* f, const Prim& L, const Prim& R, double nx, double ny, double GAM)
 {
 	double unl = L.u * nx + L.v * ny, unr = R.u * nx + R.v * ny;
-	double cl = sqrt(GAM * L.p / L.r), cr = sqrt(GAM * R.p / R.r);
+	double cl = sqrt_dev(GAM * L.p / L.r), cr = sqrt_dev(GAM * R.p / R.r);
 	double El = L.p / (GAM - 1.0) + 0.5 * L.r * (L.u * L.u + L.v * L.v);
 	double Er = R.p / (GAM - 1.0) + 0.5 * R.r * (R.u * R.u + R.v * R.v);
 	double sl = fmin(unl - cl, unr - cr);
@@ -486,11 +508,11 @@ __device__ __forceinline__ void flux_roe(double* f, const Prim& L, const Prim& R
 	double El = L.p / (GAM - 1.0) + 0.5 * L.r * (L.u * L.u + L.v * L.v);
 	double Er = R.p / (GAM - 1.0) + 0.5 * R.r * (R.u * R.u + R.v * R.v);
 	double Hl = (El + L.p) / L.r, Hr = (Er + R.p) / R.r;
-	double sl = sqrt(L.r), sr = sqrt(R.r), w = 1.0 / (sl + sr);
+	double sl = sqrt_dev(L.r), sr = sqrt_dev(R.r), w = 1.0 / (sl + sr);
 	double u = (sl * L.u + sr * R.u) * w, v = (sl * L.v + sr * R.v) * w;
 	double H = (sl * Hl + sr * Hr) * w;
 	double q2 = u * u + v * v;
-	double c = sqrt((GAM - 1.0) * (H - 0.5 * q2));
+	double c = sqrt_dev((GAM - 1.0) * (H - 0.5 * q2));
 	double un = u * nx + v * ny, ut = -u * ny + v * nx;
 	double dr = R.r - L.r, dp = R.p - L.p;
 	double dun = unr - unl;
