@@ -315,6 +315,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 	for (int q = 0; q < 4; q++) wi[q] *= si;
 	double pb[4] = {0, 0, 0, 0};
 	double gb[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	double gi[8];  // this cell's gradient record, used on every face
+	ld8(b.G, i, gi);
 	CellTop tp = load_top(b, i);
 	for (int k = 0; k < 3; k++) {
 		int f = tp.f[k];
@@ -337,17 +339,18 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 		} else {
 			for (int q = 0; q < 4; q++) fb[q] = -wi[q] * hl;
 		}
-		int c1 = side ? j : i, c2 = side ? i : j;
-		double d1x = b.edl[edl_index(e, 0, gp)], d1y = b.edl[edl_index(e, 0, gp) + 1];
-		Prim L = side ? pj : pi, R;
-		recon(L, b, c1, d1x, d1y);
-		double d2x = 0.0, d2y = 0.0;
+		// this cell's and the neighbour's states at the Gauss point (the
+		// reconstruction of each side, as the edge pass forms it)
+		double dx = b.edl[edl_index(e, side, gp)], dy = b.edl[edl_index(e, side, gp) + 1];
+		Prim own = pi, L, R;
+		recon_g(own, gi, dx, dy);
 		if (j >= 0) {
-			d2x = b.edl[edl_index(e, 1, gp)];
-			d2y = b.edl[edl_index(e, 1, gp) + 1];
-			R = side ? pi : pj;
-			recon(R, b, c2, d2x, d2y);
+			Prim oth = pj;
+			recon(oth, b, j, b.edl[edl_index(e, 1 - side, gp)], b.edl[edl_index(e, 1 - side, gp) + 1]);
+			L = side ? oth : own;
+			R = side ? own : oth;
 		} else {
+			L = own;
 			R = ghost_of<MM>(b, ph, e, i, L, pi);
 		}
 		double Lb[4], Rb[4];
@@ -363,7 +366,6 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 				a.Ebc[(size_t)(4 * gp + q) * ne + e] += dpar[q];
 			}
 		}
-		double dx = side ? d2x : d1x, dy = side ? d2y : d1y;
 		for (int q = 0; q < 4; q++) {
 			double o = side ? Rb[q] : Lb[q];
 			pb[q] += o;
@@ -428,7 +430,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 		lda4(a.Pb, b.nc, i, pb);
 		double gb[8];
 		lda8(a.Gb, b.nc, i, gb);
-		double si = b.S[i];
+		double isi = LSQ ? 0.0 : 1.0 / b.S[i];
 		CellTop tp = load_top(b, i);
 		for (int k = 0; k < 3; k++) {
 			int f = tp.f[k];
@@ -445,7 +447,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 				double l = edge_geo(b, e)[2], nx = edge_geo(b, e)[0], ny = edge_geo(b, e)[1];
 				double sg = side ? -1.0 : 1.0;
 				for (int q = 0; q < 4; q++) {
-					nbb[q] = sg * 0.5 * l * (nx * gb[2 * q] + ny * gb[2 * q + 1]) / si;
+					nbb[q] = sg * 0.5 * l * (nx * gb[2 * q] + ny * gb[2 * q + 1]) * isi;
 					pb[q] += nbb[q];
 				}
 			}
@@ -460,8 +462,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 				} else {
 					double l = edge_geo(b, e)[2], nx = edge_geo(b, e)[0], ny = edge_geo(b, e)[1];
 					double sg = side ? 1.0 : -1.0;  // the neighbour sits on the other side
-					double sj = b.S[j];
-					for (int q = 0; q < 4; q++) pb[q] += sg * 0.5 * l * (nx * gj[2 * q] + ny * gj[2 * q + 1]) / sj;
+					double isj = 1.0 / b.S[j];
+					for (int q = 0; q < 4; q++) pb[q] += sg * 0.5 * l * (nx * gj[2 * q] + ny * gj[2 * q + 1]) * isj;
 				}
 			} else {
 				// ghost of the cell-centre state across a boundary edge
