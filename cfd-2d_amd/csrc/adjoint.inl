@@ -677,10 +677,10 @@ void record_steps(cfd2d_ctx* c, AdjWork* A, int nsteps)
 			c->b.P = A->Pst + t * n4;
 			c->b.Po = A->Pst + (t + 1) * n4;
 			c->b.G = A->Gst + t * n8;
-			c->ph.store_g = 1;  // a fused edge pass stores the stage's gradients too
 			launch_gradient_edge<MM>(c);
-			c->ph.store_g = 0;
+			c->ph.store_g = 1;  // the update pass stores the stage's gradient records
 			launch_update<MM>(c, S == 3 ? modes3[st] : modes2[st]);
+			c->ph.store_g = 0;
 		}
 		c->b.P = A->Pst + t * n4;
 		c->b.Po = c->b.P;

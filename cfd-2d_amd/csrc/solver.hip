@@ -76,7 +76,7 @@ struct Phys {
 	double gam_flux, igm1, cfl;  // igm1 = 1 / (gam_flux - 1)
 	double ro_min, ro_max, p_min, p_max, u_max;
 	int max_newton, cfl_mode, steady;
-	int store_g;  // the fused edge pass also stores the gradient records (adjoint recording)
+	int store_g;  // the update pass also stores the gradient records of its input state (adjoint recording)
 };
 
 // Block index remapped so that consecutive index ranges stay on one XCD
@@ -435,12 +435,6 @@ __global__ __launch_bounds__(BLOCK) void k_edge_flux(Bufs b, Phys ph)
 		Prim nb[3];
 		cell_gradient<MM, GR == 1 || GR == 3>(b, ph, cs, ps, face, nb, gs);
 		if (GR >= 3) minmod_limit(b, cs, face, ps, nb, gs);
-		// when the records are wanted, each cell's is stored by the lane that
-		// formed it on the cell's first face (lowest edge index): once
-		if (ph.store_g && (face[0] >> 1) == e && !(gp == 1 && !inner)) {
-			double* o = static_cast<double*>(__builtin_assume_aligned(b.G + 8 * (size_t)cs, 32));
-			for (int q = 0; q < 8; q++) o[q] = gs[q];
-		}
 	} else {
 		ps = load_prim(b, cs);
 		ld8(b.G, cs, gs);
@@ -571,14 +565,31 @@ __device__ __forceinline__ void st_state(double* p, double v)
 // the bound flags (clamped values are not written back, 439-446).  The stage
 // reads its input from b.U and the step-start state from b.U0 and writes
 // b.Un; launch_update points them so that the step-start state is never
-// copied (the reference's U_old copy, 318-321).  NT: see ld_state.
-template <bool MM, int MODE, bool NT>
+// copied (the reference's U_old copy, 318-321).  NT: see ld_state.  REC
+// (adjoint recording, where the stage's input primitives b.P and its output
+// b.Po are separate slots): the pass first forms the gradient records of
+// its input state, with the gradient pass's code, and stores them to b.G
+// through LDS -- the edge pass formed the same values but could only store
+// them one scattered record per cell.
+template <bool MM, int MODE, bool NT, bool REC>
 __global__ __launch_bounds__(BLOCK) void k_update(Bufs b, Phys ph)
 {
-	__shared__ double lds[BLOCK * 5];
+	__shared__ double lds[BLOCK * (REC ? 9 : 5)];
 	int lb = xcd_block();
 	int i = lb * BLOCK + threadIdx.x;
 	int flagged = 0;
+	if (REC) {
+		double g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+		if (i < b.nc) {
+			Prim pi;
+			int face[3];
+			Prim nb[3];
+			if (b.lsqw) cell_gradient<MM, true>(b, ph, i, pi, face, nb, g);
+			else cell_gradient<MM, false>(b, ph, i, pi, face, nb, g);
+		}
+		store_records<8>(b.G, lb * BLOCK, min(BLOCK, b.nc - lb * BLOCK), g, lds);
+		__syncthreads();  // lds is reused below
+	}
 	double pv[4] = {0.0, 0.0, 0.0, 0.0};  // primitives written for this cell
 	if (i < b.nc) {
 		size_t n = b.nc;
@@ -943,14 +954,14 @@ void launch_gradient_edge(cfd2d_ctx* c)
 	}
 }
 
-template <bool MM, bool NT>
+template <bool MM, bool NT, bool REC>
 void launch_update_nt(cfd2d_ctx* c, int mode, const Bufs& b)
 {
 	switch (mode) {
-	case UPD_FIRST: LAUNCH(c, CFD2D_K_UPDATE, (k_update<MM, UPD_FIRST, NT>), c->nc, b, c->ph); break;
-	case UPD_RK2_LAST: LAUNCH(c, CFD2D_K_UPDATE, (k_update<MM, UPD_RK2_LAST, NT>), c->nc, b, c->ph); break;
-	case UPD_RK3_MID: LAUNCH(c, CFD2D_K_UPDATE, (k_update<MM, UPD_RK3_MID, NT>), c->nc, b, c->ph); break;
-	default: LAUNCH(c, CFD2D_K_UPDATE, (k_update<MM, UPD_RK3_LAST, NT>), c->nc, b, c->ph); break;
+	case UPD_FIRST: LAUNCH(c, CFD2D_K_UPDATE, (k_update<MM, UPD_FIRST, NT, REC>), c->nc, b, c->ph); break;
+	case UPD_RK2_LAST: LAUNCH(c, CFD2D_K_UPDATE, (k_update<MM, UPD_RK2_LAST, NT, REC>), c->nc, b, c->ph); break;
+	case UPD_RK3_MID: LAUNCH(c, CFD2D_K_UPDATE, (k_update<MM, UPD_RK3_MID, NT, REC>), c->nc, b, c->ph); break;
+	default: LAUNCH(c, CFD2D_K_UPDATE, (k_update<MM, UPD_RK3_LAST, NT, REC>), c->nc, b, c->ph); break;
 	}
 }
 
@@ -973,8 +984,9 @@ void launch_update(cfd2d_ctx* c, int mode)
 	b.U = mode == UPD_FIRST ? state : stage;
 	b.U0 = state;
 	b.Un = last ? state : stage;
-	if (stream_state(c)) launch_update_nt<MM, true>(c, mode, b);
-	else launch_update_nt<MM, false>(c, mode, b);
+	if (c->ph.store_g) launch_update_nt<MM, false, true>(c, mode, b);  // adjoint recording
+	else if (stream_state(c)) launch_update_nt<MM, true, false>(c, mode, b);
+	else launch_update_nt<MM, false, false>(c, mode, b);
 }
 
 template <bool MM>
