@@ -26,8 +26,9 @@
 // on scheduling.
 //
 // Storage: the forward kernels write each stage's P and G into per-stage
-// slots (the update writes the next stage's P), so recording costs no extra
-// pass.  The state is checkpointed every ckpt steps; the last segment is
+// slots (the update pass writes the next stage's P and, while recording,
+// forms and stores the stage's gradient records), so recording costs no
+// extra pass.  The state is checkpointed every ckpt steps; the last segment is
 // recorded during the forward pass itself and every earlier segment is
 // recomputed once from its checkpoint.  With ckpt >= nsteps nothing is
 // recomputed.
