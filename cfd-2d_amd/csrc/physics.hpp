@@ -201,7 +201,10 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 	double RCE = RE * CE;
 	double DU = UB - UE;
 	hit = 0;
-	if (DU < -2.0 * (CB + CE) / AGAM) {
+	// the vacuum test DU < -2 (CB + CE) / AGAM, multiplied through by AGAM > 0
+	// (no division; the comparison can only differ within a rounding of the
+	// threshold)
+	if (DU * AGAM < -2.0 * (CB + CE)) {
 		RF = 0.0; RS = 0.0; EF = 0.0; ES = 0.0;
 		SBL = UB - CB;
 		SFL = UB + 2.0 * CB / AGAM;
