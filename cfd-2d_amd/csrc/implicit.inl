@@ -116,8 +116,8 @@ __device__ __forceinline__ void store_strided(double* base, size_t stride, int c
 	for (int d = 2 * t; d < nd; d += 2 * BLOCK) {
 		int c = d / W, q = d % W;  // W even: d and d + 1 lie in one record
 		double* o = static_cast<double*>(__builtin_assume_aligned(base + (size_t)(c0 + c) * stride + q, 16));
-		o[0] = lds[c * PAD + q];
-		o[1] = lds[c * PAD + q + 1];
+		__builtin_nontemporal_store(lds[c * PAD + q], o);  // written once, read by the host copy
+		__builtin_nontemporal_store(lds[c * PAD + q + 1], o + 1);
 	}
 	__syncthreads();
 }
