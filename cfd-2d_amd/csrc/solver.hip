@@ -166,7 +166,8 @@ __device__ __forceinline__ int4 load_faces(const Bufs& b, size_t i)
 // registers (lane t: cell c0 + t) through LDS, so that each store instruction
 // covers contiguous memory instead of one 16-byte piece every W * 8 bytes.
 // Every thread of the block calls it; lds holds BLOCK * (W + 1) doubles.
-template <int W>
+// NT: non-temporal stores, for records read back only much later.
+template <int W, bool NT = false>
 __device__ __forceinline__ void store_records(double* rec, int c0, int nvalid, const double* v, double* lds)
 {
 	constexpr int PAD = W + 1;  // odd row length spreads the lanes over the LDS banks
@@ -178,8 +179,13 @@ __device__ __forceinline__ void store_records(double* rec, int c0, int nvalid, c
 	for (int d = 2 * t; d < nd; d += 2 * BLOCK) {
 		int c = d / W, q = d % W;  // W even: d and d + 1 lie in one record
 		double* o = static_cast<double*>(__builtin_assume_aligned(dst + d, 16));
-		o[0] = lds[c * PAD + q];
-		o[1] = lds[c * PAD + q + 1];
+		if (NT) {
+			__builtin_nontemporal_store(lds[c * PAD + q], o);
+			__builtin_nontemporal_store(lds[c * PAD + q + 1], o + 1);
+		} else {
+			o[0] = lds[c * PAD + q];
+			o[1] = lds[c * PAD + q + 1];
+		}
 	}
 }
 
@@ -587,7 +593,8 @@ __global__ __launch_bounds__(BLOCK) void k_update(Bufs b, Phys ph)
 			if (b.lsqw) cell_gradient<MM, true>(b, ph, i, pi, face, nb, g);
 			else cell_gradient<MM, false>(b, ph, i, pi, face, nb, g);
 		}
-		store_records<8>(b.G, lb * BLOCK, min(BLOCK, b.nc - lb * BLOCK), g, lds);
+		// read only by the reverse sweep, many stages later: kept out of the caches
+		store_records<8, true>(b.G, lb * BLOCK, min(BLOCK, b.nc - lb * BLOCK), g, lds);
 		__syncthreads();  // lds is reused below
 	}
 	double pv[4] = {0.0, 0.0, 0.0, 0.0};  // primitives written for this cell
