@@ -2,8 +2,8 @@
 // and the context half of the C ABI.
 //
 // Data layout (structure of arrays, FP64, cells in the reference's order):
-//   cells  U[4][nc] conserved, U0[4][nc] step start (structure of arrays:
-//          streamed), P[nc][4] primitives (r, p, u, v) of U (Po: where the
+//   cells  U[4][nc] conserved state, U0[4][nc] the RK stage buffer
+//          (launch_update; structure of arrays: streamed), P[nc][4] primitives (r, p, u, v) of U (Po: where the
 //          update writes them, = P except while the adjoint records stages)
 //          and G[nc][8] gradients (rx ry px py ux uy vx vy) as one 32 B / 64 B
 //          record per cell, because they are gathered by neighbours: one
