@@ -77,7 +77,7 @@ def _xml_escape(s):
 
 def task_xml(mesh_path, *, regions, boundaries, materials=None, steady=0, tau=1.0, tmax=1.0e9,
              cfl=0.5, step_max=1000, file_output_step=1000000, log_output_step=1000000,
-             limits=None, mesh_type="salome_unv", extensions=None, method="FVM_TVD"):
+             limits=None, mesh_type="salome_unv", extensions=None, method="FVM_TVD", initial_file=None):
     """Text of a task.xml in the reference's FVM_TVD schema.
 
     regions: list of dict(name, material=0, Vx, Vy, T, P, cell_type=0)
@@ -85,6 +85,9 @@ def task_xml(mesh_path, *, regions, boundaries, materials=None, steady=0, tau=1.
                 pair/shift for the periodic extension)
     extensions: optional dict of extension control nodes written as
                 <NAME value="..."/> inside <control> (FLUX, GRADIENT, LIMITER, ...)
+    initial_file: optional <initial file="..."/> (extension): raw little-endian
+                float64 conserved state, double[4][cells] in the mesh file's
+                cell order, replacing the regions' uniform states
     """
     if materials is None:
         materials = [dict(name="air", M=8.314472, Cp=3.5, K=0.0, ML=0.0)]
@@ -148,6 +151,8 @@ def task_xml(mesh_path, *, regions, boundaries, materials=None, steady=0, tau=1.
     w('    <name value="%s"/>\n' % _xml_escape(mesh_path))
     w('    <filesType value="%s"/>\n' % mesh_type)
     w("  </mesh>\n")
+    if initial_file is not None:
+        w('  <initial file="%s"/>\n' % _xml_escape(initial_file))
     w("</task>\n")
     return s.getvalue()
 

@@ -1199,7 +1199,8 @@ int cfd2d_ctx_create(const cfd2d_mesh* mesh, const cfd2d_params* p, int n_bc, co
 	c->ph.p_min = p->p_min;
 	c->ph.p_max = p->p_max;
 	c->ph.u_max = p->u_max;
-	c->ph.max_newton = p->max_newton;
+	// the device Newton loop always ends: no cap given means the library's
+	c->ph.max_newton = p->max_newton > 0 ? p->max_newton : CFD2D_NEWTON_CAP;
 	c->ph.cfl_mode = p->cfl_mode;
 	c->ph.steady = p->steady;
 

@@ -46,6 +46,7 @@ enum { CFD2D_CFL_REFERENCE = 0, CFD2D_CFL_SPECTRAL = 1 };
  * (src/bnd_cond.cpp:50-51); PRESSURE_OUTLET is an extension */
 enum { CFD2D_BND_INLET = 1, CFD2D_BND_OUTLET = 2, CFD2D_BND_WALL = 3, CFD2D_BND_PRESSURE_OUTLET = 4 };
 enum { CFD2D_MAXMAT = 8 };
+enum { CFD2D_NEWTON_CAP = 500 };  /* Newton iterations when cfd2d_params.max_newton <= 0 */
 
 /* kernels reported by cfd2d_timing_read */
 enum {
@@ -67,7 +68,8 @@ typedef struct cfd2d_params {
 	int viscous;       /* 1 = Navier-Stokes viscous flux (extension) */
 	int steady;        /* <STEADY>: 1 = local time step recomputed every step */
 	int cfl_mode;      /* CFD2D_CFL_* */
-	int max_newton;    /* cap on rim_orig's Newton loop (0 = none, as the reference) */
+	int max_newton;    /* cap on rim_orig's Newton loop; <= 0 selects CFD2D_NEWTON_CAP (the reference's
+	                      loop has no cap, but a device loop must end; hits raise CFD2D_ERR_NEWTON) */
 	double cfl;        /* <CFL> */
 	double tau;        /* <TAU>: upper bound of the unsteady time step */
 	double gam_flux;   /* gamma used by the flux: 1.4 in the reference (fvm_tvd.cpp:345) */

@@ -644,7 +644,8 @@ static void flux_godunov(double* f, const prm* pL, const prm* pR, double nx, dou
 	f[3] = (RI * (EI + 0.5 * (UI * UI + VI * VI)) + PI) * UN;
 }
 
-/* FLUX=LAX as FVM_TVD_IMPLICIT::calcFlux defines it (src/methods/fvm_tvd_implicit.cpp:939-960) */
+/* FLUX=LAX as FVM_TVD_IMPLICIT::calcFlux defines it (src/methods/fvm_tvd_implicit.cpp:939-960);
+ * pinned by tests/test_oracle_pins.py::test_lax_* (closed forms) */
 static void flux_lax(double* f, const prm* pL, const prm* pR, double nx, double ny, double GAM)
 {
 	double unl = pL->u * nx + pL->v * ny;
@@ -919,8 +920,10 @@ static void grad_lsq(or_solver* s)
  * (ghosts on boundaries); at each of the cell's six Gauss points,
  * D = grad . d and psi_k = minmod(1, (phi_max - phi)/D) for D > 0,
  * minmod(1, (phi_min - phi)/D) for D < 0, 1 for D = 0; the gradient is scaled
- * by min_k psi_k.  Keeps every reconstructed value inside the neighbour
- * bounds (tests/test_oracle_ext.py::test_minmod_bounds).
+ * by min_k psi_k, the largest scaling in [0, 1] that keeps every
+ * reconstructed value inside the neighbour bounds
+ * (tests/test_oracle_pins.py::test_minmod_is_the_largest_bounded_scaling,
+ * tests/test_oracle_ext.py::test_minmod_bounds).
  */
 static void limit_minmod(or_solver* s)
 {
@@ -990,7 +993,8 @@ static void grad_T(const or_solver* s, int i, double* tx, double* ty)
  * boundary); the face velocity is the mean of the reconstructed left/right
  * velocities; mu = material ML, conductivity = material K, or mu*Cp/Pr when K
  * is zero.  Returns the viscous part of the normal flux, which is subtracted
- * from the inviscid flux.  Pinned by tests/test_oracle_ext.py::test_viscous_*.
+ * from the inviscid flux.  Pinned by tests/test_oracle_ext.py::test_viscous_*
+ * (stress) and tests/test_oracle_pins.py::test_heat_conduction_* (conduction).
  */
 static void viscous_flux(const or_solver* s, int e, const prm* pL, const prm* pR, double* fv)
 {

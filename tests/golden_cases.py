@@ -38,7 +38,7 @@ def case(name, **over):
     lim = s.get("limits") or dict(ro_min=1.0e-10, ro_max=1.0e10, p_min=1.0e-10, p_max=1.0e10, u_max=1.0e10)
     kw = dict(regions=regions, boundaries=bnds, steady=bool(s["steady"]), cfl=s["cfl"], tau=1.0,
               ro_min=lim["ro_min"], ro_max=lim["ro_max"], p_min=lim["p_min"], p_max=lim["p_max"],
-              u_max=lim["u_max"], max_newton=0)
+              u_max=lim["u_max"], max_newton=500)
     if s.get("materials"):
         kw["materials"] = [Material(name=x["name"], M=x["M"], Cp=x["Cp"], K=x["K"], ML=x["ML"])
                            for x in s["materials"]]

@@ -47,3 +47,46 @@ def test_driver_writes_vtk_matching_the_oracle():
     s.run(steps)
     ref = s.state()[0]
     assert np.max(np.abs(rho - ref)) <= 1e-11 * np.max(np.abs(ref))
+
+
+def _run_driver(d, steps, **task):
+    exe = os.path.join(REPO, "cfd-2d_amd", "cfd2d", "cfd2d_run")
+    regions = [dict(name="cells_left", Vx=0.0, Vy=0.0, T=1.0, P=1.0),
+               dict(name="cells_right", Vx=0.0, Vy=0.0, T=1.0, P=0.5)]
+    bnds = [dict(name="edge_bottom", type="BOUND_WALL_SLIP"), dict(name="edge_top", type="BOUND_WALL_SLIP"),
+            dict(name="edge_left", type="BOUND_OUTLET"), dict(name="edge_right", type="BOUND_OUTLET")]
+    with open(os.path.join(d, "task.xml"), "w") as f:
+        f.write(unv.task_xml("mesh.unv", regions=regions, boundaries=bnds, step_max=steps,
+                             file_output_step=steps, log_output_step=steps, **task))
+    r = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return open(os.path.join(d, "res_%010d.vtk" % steps)).read()
+
+
+def test_driver_initial_file_matches_the_oracle():
+    """<task><initial file=...> (a per-cell state in the file's cell order,
+    raw float64 [4][cells]) replaces the regions' uniform states: the
+    driver's density after 12 steps equals the oracle's from the same state."""
+    nx, ny, steps = 40, 6, 12
+    c = cases.shock_tube(nx, ny, seed=1, mild=True)
+    m = c.mesh
+    xc = m.centroids()
+    rho = 1.0 + 0.2 * np.sin(2 * np.pi * xc[:, 0]) * np.cos(3 * np.pi * xc[:, 1] / 0.2)
+    u = 0.1 * np.sin(np.pi * xc[:, 0])
+    v = 0.02 * meshgen.uniform(7, 1, np.arange(m.n_cells))
+    p = 1.0 + 0.3 * np.cos(2 * np.pi * xc[:, 0])
+    from cfd2d.config import prim_to_cons
+    U0 = prim_to_cons(rho, u, v, p, 1.4)
+    with tempfile.TemporaryDirectory() as d:
+        unv.write_unv(m, os.path.join(d, "mesh.unv"))
+        np.ascontiguousarray(U0, "<f8").tofile(os.path.join(d, "u0.bin"))
+        text = _run_driver(d, steps, initial_file="u0.bin")
+    dens = _block(text, "SCALARS Density float 1\nLOOKUP_TABLE default\n", m.n_cells)
+    plain = oracle.OracleSolver(c)  # from the regions' states
+    plain.run(steps)
+    assert np.max(np.abs(dens - plain.state()[0])) > 1e-2
+    c.U0 = U0
+    s = oracle.OracleSolver(c)
+    s.run(steps)
+    ref = s.state()[0]
+    assert np.max(np.abs(dens - ref)) <= 1e-11 * np.max(np.abs(ref))
