@@ -49,6 +49,7 @@ SIGNATURES = {
     "cfd2d_mesh_set_cell_groups": (C.c_int, [C.c_void_p, _ip]),
     "cfd2d_mesh_periodic": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double]),
     "cfd2d_mesh_concat": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "cfd2d_mesh_renumber": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
     "cfd2d_mesh_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cfd2d_mesh_nodes": (C.c_int, [C.c_void_p, _dp]),
     "cfd2d_mesh_group_count": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
@@ -83,6 +84,8 @@ SIGNATURES = {
     "cfd2d_adjoint_objectives": (C.c_int, [C.c_void_p, _dp]),
     "cfd2d_implicit_assemble": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p]),
+    "cfd2d_riemann": (C.c_int, [C.c_int, _dp, C.c_double, C.c_int, _dp, C.c_void_p, C.POINTER(C.c_longlong)]),
+    "cfd2d_flux": (C.c_int, [C.c_int, C.c_int, _dp, C.c_double, C.c_int, _dp, C.POINTER(C.c_longlong)]),
     "cfd2d_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
     "cfd2d_host_free": (C.c_int, [C.c_void_p]),
     "cfd2d_timing": (C.c_int, [C.c_void_p, C.c_int]),
@@ -124,6 +127,34 @@ def device_count():
     n = C.c_int(0)
     rc = lib().cfd2d_device_count(C.byref(n))
     return n.value if rc == 0 else 0
+
+
+def riemann(rows, gam=1.4, max_newton=0):
+    """rim_orig on the device for rows (n, 8) = (rho, p, u_n, u_t) left then
+    right; returns (out (n, 5) = RI, EI, PI, UI, VI; Newton iterations (n,);
+    rows that hit the cap)."""
+    rows = np.ascontiguousarray(rows, np.float64).reshape(-1, 8)
+    n = rows.shape[0]
+    out = np.zeros((n, 5))
+    its = np.zeros(n, np.int32)
+    hits = C.c_longlong()
+    check(lib().cfd2d_riemann(n, rows.ravel(), float(gam), int(max_newton), out.ravel(),
+                              its.ctypes.data_as(C.c_void_p), C.byref(hits)))
+    return out, its, hits.value
+
+
+FLUX_KINDS = {"GODUNOV": 0, "LAX": 1, "HLLC": 2, "ROE": 3}
+
+
+def flux(kind, rows, gam=1.4, max_newton=0):
+    """A numerical flux on the device for rows (n, 10) = (rL, pL, uL, vL, rR,
+    pR, uR, vR, nx, ny); returns ((n, 4) fluxes, rows that hit the Newton cap)."""
+    k = FLUX_KINDS[kind] if isinstance(kind, str) else int(kind)
+    rows = np.ascontiguousarray(rows, np.float64).reshape(-1, 10)
+    out = np.zeros((rows.shape[0], 4))
+    hits = C.c_longlong()
+    check(lib().cfd2d_flux(k, rows.shape[0], rows.ravel(), float(gam), int(max_newton), out.ravel(), C.byref(hits)))
+    return out, hits.value
 
 
 class Mesh:
@@ -168,6 +199,17 @@ class Mesh:
         h = C.c_void_p()
         check(lib().cfd2d_mesh_concat(len(meshes), arr, C.byref(h)))
         return cls(h)
+
+    ORDERS = {"none": 0, "hilbert": 1, "shuffle": 2}
+
+    def renumber(self, order="hilbert", seed=0):
+        """(the mesh with its cells in another order, permutation p: new cell
+        k = old cell p[k]); cfd2d_mesh_renumber."""
+        p = np.zeros(self.nc, np.int32)
+        h = C.c_void_p()
+        check(lib().cfd2d_mesh_renumber(self._h, self.ORDERS[order], int(seed), p.ctypes.data_as(C.c_void_p),
+                                        C.byref(h)))
+        return Mesh(h), p
 
     def group_names(self, kind=0):
         n = C.c_int()

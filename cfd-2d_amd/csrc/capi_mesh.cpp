@@ -1,4 +1,5 @@
 // C ABI: errors and host meshes.
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <string>
@@ -110,6 +111,23 @@ int cfd2d_mesh_periodic(cfd2d_mesh* m, int ga, int gb, double sx, double sy)
 	std::string err;
 	int rc = cfd2d::pair_periodic(m->m, ga, gb, sx, sy, err);
 	return rc ? fail(CFD2D_ERR_PERIODIC, err) : CFD2D_OK;
+}
+
+int cfd2d_mesh_renumber(const cfd2d_mesh* m, int order, int seed, int* cell_perm, cfd2d_mesh** out)
+{
+	if (!m || !out) return fail(CFD2D_ERR_ARG, "cfd2d_mesh_renumber: bad arguments");
+	auto* h = new (std::nothrow) cfd2d_mesh;
+	if (!h) return fail(CFD2D_ERR_ARG, "out of host memory");
+	std::string err;
+	std::vector<int> perm;
+	int rc = cfd2d::renumber_mesh(m->m, order, (unsigned long long)(unsigned)seed, perm, h->m, err);
+	if (rc) {
+		delete h;
+		return fail(rc == cfd2d::MESH_ERR_PERIODIC ? CFD2D_ERR_PERIODIC : CFD2D_ERR_MESH, err);
+	}
+	if (cell_perm) std::copy(perm.begin(), perm.end(), cell_perm);
+	*out = h;
+	return CFD2D_OK;
 }
 
 int cfd2d_mesh_concat(int n, const cfd2d_mesh* const* parts, cfd2d_mesh** out)

@@ -194,6 +194,11 @@ int build_mesh_ex(int nn, const double* xy, int nc, const int* tri, int nb, cons
 	m.cell_case.assign(nc, 0);
 	m.case_cell_offset = {0, nc};
 	m.case_edge_offset = {0, ne};
+	m.bnodes.assign(bedges, bedges + 2 * (size_t)nb);
+	m.bgroup.assign(nb, -1);
+	if (bgroup) m.bgroup.assign(bgroup, bgroup + nb);
+	m.missing_group = missing_group;
+	m.periodic.clear();
 	return rc;
 }
 
@@ -528,6 +533,118 @@ int pair_periodic(Mesh& m, int ga, int gb, double sx, double sy, std::string& er
 		std::sort(ce, ce + 3);
 	}
 	if (m.case_edge_offset.size() == 2) m.case_edge_offset[1] = ne2;
+	m.periodic.push_back(Mesh::Periodic{ga, gb, sx, sy});
+	return MESH_OK;
+}
+
+// ---------------------------------------------------------------- renumbering
+
+/*
+ * Extension: the cell order decides how far apart in memory the cells and
+ * edges that one kernel gathers together lie (the mesh builder numbers edges
+ * in cell-discovery order).  Meshes from external generators come in the
+ * generator's order; the reference partitions its dual graph with METIS for
+ * MPI (src/methods/decomp.cpp:86-104).  For locality on one device the
+ * cells are sorted here along a Hilbert curve through their centroids
+ * (quantised to a 2^16 x 2^16 grid over their bounding box; ties keep the
+ * old order) and the nodes are numbered by first use in that order, as
+ * cfd2d/renumber.py does for generator meshes.  ORDER_SHUFFLE sorts by a
+ * seeded SplitMix64 hash of the old index instead, to measure what an
+ * unordered mesh costs.
+ */
+unsigned long long hilbert_index(unsigned long long x, unsigned long long y, int bits)
+{
+	unsigned long long n = 1ull << bits, d = 0;
+	for (unsigned long long s = n >> 1; s > 0; s >>= 1) {
+		unsigned long long rx = (x & s) ? 1 : 0, ry = (y & s) ? 1 : 0;
+		d += s * s * ((3 * rx) ^ ry);
+		if (!ry) {  // rotate the quadrant so the curve stays continuous
+			if (rx) {
+				x = n - 1 - x;
+				y = n - 1 - y;
+			}
+			std::swap(x, y);
+		}
+	}
+	return d;
+}
+
+namespace {
+
+unsigned long long splitmix64(unsigned long long z)
+{
+	z += 0x9E3779B97F4A7C15ull;
+	z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+	z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+	return z ^ (z >> 31);
+}
+
+}  // namespace
+
+int renumber_mesh(const Mesh& in, int order, unsigned long long seed, std::vector<int>& perm, Mesh& out,
+                  std::string& err)
+{
+	if (in.case_cell_offset.size() > 2) {
+		err = "renumbering a concatenated mesh: renumber its parts before concatenating";
+		return MESH_ERR_INPUT;
+	}
+	if (order < ORDER_NONE || order > ORDER_SHUFFLE) {
+		err = "unknown cell order";
+		return MESH_ERR_INPUT;
+	}
+	int nc = in.nc, nn = in.nn;
+	std::vector<unsigned long long> key(nc, 0);
+	if (order == ORDER_HILBERT) {
+		double lo[2] = {in.cx[0], in.cy[0]}, hi[2] = {in.cx[0], in.cy[0]};
+		for (int i = 0; i < nc; i++) {
+			lo[0] = std::min(lo[0], in.cx[i]);
+			hi[0] = std::max(hi[0], in.cx[i]);
+			lo[1] = std::min(lo[1], in.cy[i]);
+			hi[1] = std::max(hi[1], in.cy[i]);
+		}
+		double span[2] = {hi[0] > lo[0] ? hi[0] - lo[0] : 1.0, hi[1] > lo[1] ? hi[1] - lo[1] : 1.0};
+		for (int i = 0; i < nc; i++) {
+			long long qx = std::min((long long)((in.cx[i] - lo[0]) / span[0] * 65535.0), 65535LL);
+			long long qy = std::min((long long)((in.cy[i] - lo[1]) / span[1] * 65535.0), 65535LL);
+			key[i] = hilbert_index((unsigned long long)qx, (unsigned long long)qy, 16);
+		}
+	} else if (order == ORDER_SHUFFLE) {
+		unsigned long long base = splitmix64(seed);
+		for (int i = 0; i < nc; i++) key[i] = splitmix64(base + (unsigned long long)i);
+	}
+	perm.resize(nc);
+	for (int i = 0; i < nc; i++) perm[i] = i;
+	std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+	// nodes by first use in the new cell order (unused nodes keep their order, last)
+	std::vector<int> nid(nn, -1);
+	int w = 0;
+	for (int k = 0; k < nc; k++)
+		for (int q = 0; q < 3; q++) {
+			int v = in.cnodes[3 * (size_t)perm[k] + q];
+			if (nid[v] < 0) nid[v] = w++;
+		}
+	for (int v = 0; v < nn; v++)
+		if (nid[v] < 0) nid[v] = w++;
+	std::vector<double> xy(2 * (size_t)nn);
+	for (int v = 0; v < nn; v++) {
+		xy[2 * (size_t)nid[v]] = in.xy[2 * (size_t)v];
+		xy[2 * (size_t)nid[v] + 1] = in.xy[2 * (size_t)v + 1];
+	}
+	std::vector<int> tri(3 * (size_t)nc), bn(in.bnodes.size());
+	for (int k = 0; k < nc; k++)
+		for (int q = 0; q < 3; q++) tri[3 * (size_t)k + q] = nid[in.cnodes[3 * (size_t)perm[k] + q]];
+	for (size_t k = 0; k < bn.size(); k++) bn[k] = nid[in.bnodes[k]];
+	int nb = (int)(bn.size() / 2);
+	out = Mesh();
+	int rc = build_mesh_ex(nn, xy.data(), nc, tri.data(), nb, bn.data(), in.bgroup.data(), out, err,
+	                       in.missing_group);
+	if (rc) return rc;
+	out.edge_groups = in.edge_groups;
+	out.cell_groups = in.cell_groups;
+	out.numeric_groups = in.numeric_groups;
+	for (int k = 0; k < nc; k++) out.cell_group[k] = in.cell_group[perm[k]];
+	for (const Mesh::Periodic& p : in.periodic)
+		if ((rc = pair_periodic(out, p.ga, p.gb, p.sx, p.sy, err))) return rc;
 	return MESH_OK;
 }
 

@@ -31,7 +31,18 @@ struct Mesh {
 	std::vector<int> case_cell_offset;  // n_cases + 1 entries
 	std::vector<int> case_edge_offset;  // n_cases + 1 entries
 	bool numeric_groups = false;     // groups named by Triangle boundary markers / region attributes
+	// what the builder was given, kept so the mesh can be rebuilt in another cell order
+	std::vector<int> bnodes, bgroup; // boundary elements: 2 nodes each, group
+	int missing_group = -2;          // group of a boundary edge with no element (-2: an error)
+	struct Periodic {
+		int ga, gb;
+		double sx, sy;
+	};
+	std::vector<Periodic> periodic;  // pairings applied, in order
 };
+
+// cell orders for renumber_mesh
+enum { ORDER_NONE = 0, ORDER_HILBERT = 1, ORDER_SHUFFLE = 2 };
 
 // Error codes (the reference's Exception types, src/global.h:241-248, where one exists)
 enum {
@@ -49,5 +60,12 @@ int read_unv(const std::string& path, Mesh& m, std::string& err);
 int read_triangle(const std::string& base, Mesh& m, std::string& err);
 int pair_periodic(Mesh& m, int group_a, int group_b, double sx, double sy, std::string& err);
 int concat_meshes(const std::vector<const Mesh*>& parts, Mesh& out, std::string& err);
+// Hilbert-curve position of an integer point of [0, 2^bits)^2
+unsigned long long hilbert_index(unsigned long long x, unsigned long long y, int bits);
+// The cell permutation of an order (perm[k] = old index of new cell k) and
+// the mesh rebuilt in it (nodes numbered by first use, groups and periodic
+// pairings carried over); one case only.
+int renumber_mesh(const Mesh& in, int order, unsigned long long seed, std::vector<int>& perm, Mesh& out,
+                  std::string& err);
 
 }  // namespace cfd2d

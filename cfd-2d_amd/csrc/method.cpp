@@ -137,6 +137,11 @@ void FVM_TVD::parse(const char* xmlFileName)
 	if (const char* v = opt_value(control, "VISCOUS")) prm.viscous = atoi(v);
 	if (const char* v = opt_value(control, "PRANDTL")) prm.prandtl = atof(v);
 	if (const char* v = opt_value(control, "MAX_NEWTON")) prm.max_newton = atoi(v);
+	if (const char* v = opt_value(control, "RENUMBER")) {
+		static const char* n[] = {"NONE", "HILBERT"};
+		static const int c[] = {CFD2D_ORDER_NONE, CFD2D_ORDER_HILBERT};
+		renumberOrder = lookup(v, n, c, 2, "RENUMBER");
+	}
 
 	XmlNode* lim = need(task, "limits");
 	limitRmin = attr_d(need(lim, "ro"), "min");
@@ -253,6 +258,16 @@ void FVM_TVD::build()
 		check(cfd2d_mesh_periodic(msh, ga, gb, b.shift[0], b.shift[1]));
 	}
 	check(cfd2d_mesh_counts(msh, nullptr, &nc, &ne));
+	// the device steps the cells in Hilbert order (neighbours close in
+	// memory); output and the initial-state file keep the file's order
+	perm.clear();
+	if (renumberOrder != CFD2D_ORDER_NONE) {
+		cfd2d_mesh* r = nullptr;
+		perm.assign(nc, 0);
+		check(cfd2d_mesh_renumber(msh, renumberOrder, 0, perm.data(), &r));
+		fileMsh = msh;
+		msh = r;
+	}
 	// boundary objects, then group -> object (by name for UNV groups, by the
 	// numeric edge type for Triangle markers; fvm_tvd.cpp:132-174)
 	std::vector<int> bt;
@@ -288,23 +303,29 @@ void FVM_TVD::build()
 				break;
 			}
 	std::vector<double> U(4 * (size_t)nc);
-	cellMat.assign(nc, 0);
+	std::vector<int> mat(nc, 0);  // device order
 	for (int i = 0; i < nc; i++) {
 		int r = cg[i] >= 0 ? regOf[cg[i]] : -1;
 		if (r < 0) throw Exception("ERROR: unknown cell name '" + (cg[i] >= 0 ? cgn[cg[i]] : std::string("?")) + "'...", 1);
 		const RegionSpec& R = regions[r];
-		cellMat[i] = R.matId;
+		mat[i] = R.matId;
 		double u[4];
 		region_cons(R, materials[R.matId], u);
 		for (int q = 0; q < 4; q++) U[(size_t)q * nc + i] = u[q];
 	}
-	if (!initialFile.empty()) {
+	if (!initialFile.empty()) {  // double[4][cells] in the file's cell order
 		std::ifstream f(initialFile, std::ios::binary);
 		if (!f) throw Exception("File '" + initialFile + "' opening error.", Exception::FILE_OPENING_ERROR);
-		f.read((char*)U.data(), U.size() * sizeof(double));
-		if ((size_t)f.gcount() != U.size() * sizeof(double))
+		std::vector<double> Uf(U.size());
+		f.read((char*)Uf.data(), Uf.size() * sizeof(double));
+		if ((size_t)f.gcount() != Uf.size() * sizeof(double))
 			throw Exception("initial state file is shorter than 4*cells doubles", 1);
+		for (int q = 0; q < 4; q++)
+			for (int k = 0; k < nc; k++)
+				U[(size_t)q * nc + k] = Uf[(size_t)q * nc + (perm.empty() ? k : perm[k])];
 	}
+	cellMat.assign(nc, 0);
+	for (int k = 0; k < nc; k++) cellMat[perm.empty() ? k : perm[k]] = mat[k];
 	prm.steady = STEADY ? 1 : 0;
 	prm.cfl = CFL;
 	prm.tau = TAU;
@@ -322,7 +343,7 @@ void FVM_TVD::build()
 		prm.K[k] = materials[k].K;
 		prm.ML[k] = materials[k].ML;
 	}
-	check(cfd2d_ctx_create(msh, &prm, (int)bt.size(), bt.data(), bp.data(), gbc.data(), cellMat.data(), &ctx));
+	check(cfd2d_ctx_create(msh, &prm, (int)bt.size(), bt.data(), bp.data(), gbc.data(), mat.data(), &ctx));
 	check(cfd2d_set_state(ctx, U.data()));
 	check(cfd2d_calc_tau(ctx));
 	check(cfd2d_get_tau(ctx, &TAU, nullptr));
@@ -372,8 +393,10 @@ void FVM_TVD::done()
 	if (writer.joinable()) writer.join();  // an error it held was reported by run()
 	if (ctx) cfd2d_ctx_destroy(ctx);
 	if (msh) cfd2d_mesh_destroy(msh);
+	if (fileMsh) cfd2d_mesh_destroy(fileMsh);
 	ctx = nullptr;
 	msh = nullptr;
+	fileMsh = nullptr;
 }
 
 // One output step copied to the host: the state and the per-cell time step.
@@ -384,13 +407,14 @@ struct FVM_TVD::Snapshot {
 
 std::shared_ptr<FVM_TVD::Snapshot> FVM_TVD::snapshot(int step)
 {
+	const cfd2d_mesh* om = fileMsh ? fileMsh : msh;  // files keep the mesh file's cell order
 	if (outCells.empty()) {  // the mesh part of every file, read once
 		int nn = 0, ne = 0;
-		check(cfd2d_mesh_counts(msh, &nn, nullptr, &ne));
+		check(cfd2d_mesh_counts(om, &nn, nullptr, &ne));
 		outCells.assign(3 * (size_t)nc, 0);
-		check(cfd2d_mesh_export(msh, nullptr, outCells.data(), nullptr, nullptr, nullptr, nullptr, nullptr));
+		check(cfd2d_mesh_export(om, nullptr, outCells.data(), nullptr, nullptr, nullptr, nullptr, nullptr));
 		outXY.assign(2 * (size_t)nn, 0.0);
-		check(cfd2d_mesh_nodes(msh, outXY.data()));
+		check(cfd2d_mesh_nodes(om, outXY.data()));
 		outNodes = nn;
 	}
 	auto s = std::make_shared<Snapshot>();
@@ -399,6 +423,15 @@ std::shared_ptr<FVM_TVD::Snapshot> FVM_TVD::snapshot(int step)
 	s->ctau.resize(nc);
 	check(cfd2d_get_state(ctx, s->U.data()));
 	check(cfd2d_get_tau(ctx, nullptr, s->ctau.data()));
+	if (!perm.empty()) {  // device order -> file order
+		std::vector<double> U(s->U.size()), ct(nc);
+		for (int k = 0; k < nc; k++) {
+			for (int q = 0; q < 4; q++) U[(size_t)q * nc + perm[k]] = s->U[(size_t)q * nc + k];
+			ct[perm[k]] = s->ctau[k];
+		}
+		s->U.swap(U);
+		s->ctau.swap(ct);
+	}
 	return s;
 }
 

@@ -104,10 +104,13 @@ private:
 	std::vector<RegionSpec> regions;
 	std::vector<BoundarySpec> boundaries;
 	std::string meshName, meshType, initialFile;
-	cfd2d_mesh* msh = nullptr;
+	cfd2d_mesh* msh = nullptr;      // the mesh the device steps (cells renumbered unless RENUMBER is NONE)
+	cfd2d_mesh* fileMsh = nullptr;  // the mesh in the file's cell order (when renumbered)
+	int renumberOrder = CFD2D_ORDER_HILBERT;
+	std::vector<int> perm;          // device cell k is file cell perm[k] (empty: the same order)
 	cfd2d_ctx* ctx = nullptr;
 	int nc = 0;
-	std::vector<int> cellMat;
+	std::vector<int> cellMat;       // material of each cell in the file's order
 	// output: mesh part of the VTK files (read once) and the writer thread
 	std::vector<double> outXY;
 	std::vector<int> outCells;

@@ -100,7 +100,8 @@ __device__ __forceinline__ Prim ghost_state(int type, const double* par, double 
 // rim_orig (src/global.cpp:232-405): exact Riemann solver, Newton on p*
 // from the acoustic guess with the P >= 1e-5 clamp, stopping when
 // |DU - F1 - F2| <= 1e-5 is seen before the update; powers as
-// exp(log(x) k).  max_iter > 0 caps the loop and reports it in *hit.
+// exp(log(x) k).  max_iter > 0 caps the loop and reports it in hit; iters
+// (if given) receives the Newton iterations taken (0 on the vacuum branch).
 // Divisions by loop invariants (1/RB, 1/RE, 1/PB, 1/PE, the gamma constants)
 // are formed once and multiplied; the shock branches share one reciprocal of
 // Z * PK for F and F'; the post-loop shock state uses 1/(R (U - D)) for both
@@ -183,7 +184,7 @@ __device__ __forceinline__ double pow_iagam(double x, const RimConst& K)
 
 __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& UI, double& VI, double RB,
                                     double PB, double UB, double VB, double RE, double PE, double UE, double VE,
-                                    const RimConst& K, int max_iter, int& hit)
+                                    const RimConst& K, int max_iter, int& hit, int* iters = nullptr)
 {
 	const double GAM = K.GAM, AGAM = K.AGAM, DGAM = K.DGAM, HGAM = K.HGAM, FGAM = K.FGAM, OGAM = K.OGAM;
 	const double QGAM = K.QGAM, PGAM = K.PGAM, SGAM = K.SGAM, TGAM = K.TGAM;
@@ -201,6 +202,7 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 	double RCE = RE * CE;
 	double DU = UB - UE;
 	hit = 0;
+	if (iters) *iters = 0;
 	// the vacuum test DU < -2 (CB + CE) / AGAM, multiplied through by AGAM > 0
 	// (no division; the comparison can only differ within a rounding of the
 	// threshold)
@@ -249,6 +251,7 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 				break;
 			}
 		}
+		if (iters) *iters = it;
 		// the reference forms both rarefaction heads before branching; each is
 		// only read on its rarefaction side, so it is formed there
 		if (PB > P) {

@@ -1700,3 +1700,4 @@ int cfd2d_ctx_stream(cfd2d_ctx* c, void** s)
 
 #include "adjoint.inl"
 #include "implicit.inl"
+#include "pointwise.inl"

@@ -97,6 +97,18 @@ int  cfd2d_mesh_set_groups(cfd2d_mesh* m, int kind /*0 edge, 1 cell*/, int n, co
 int  cfd2d_mesh_set_cell_groups(cfd2d_mesh* m, const int* cell_group);
 int  cfd2d_mesh_periodic(cfd2d_mesh* m, int group_a, int group_b, double sx, double sy);
 int  cfd2d_mesh_concat(int n, const cfd2d_mesh* const* parts, cfd2d_mesh** out);
+/* A new mesh with the cells of m in another order and the nodes numbered by
+ * first use in it; groups, cell groups and periodic pairings carry over.
+ * order CFD2D_ORDER_HILBERT sorts the cells along a Hilbert curve through
+ * their centroids (locality: neighbouring cells and their edges close in
+ * memory), CFD2D_ORDER_SHUFFLE by a seeded hash (to measure an unordered
+ * mesh), CFD2D_ORDER_NONE keeps the order.  cell_perm int[n_cells] (may be
+ * NULL): new cell k is old cell cell_perm[k].  One case only (renumber the
+ * parts of a batch before cfd2d_mesh_concat).  The role of the reference's
+ * METIS dual-graph partition (src/methods/decomp.cpp:86-104), here for
+ * locality on one device. */
+enum { CFD2D_ORDER_NONE = 0, CFD2D_ORDER_HILBERT = 1, CFD2D_ORDER_SHUFFLE = 2 };
+int  cfd2d_mesh_renumber(const cfd2d_mesh* m, int order, int seed, int* cell_perm, cfd2d_mesh** out);
 int  cfd2d_mesh_counts(const cfd2d_mesh* m, int* n_nodes, int* n_cells, int* n_edges);
 int  cfd2d_mesh_nodes(const cfd2d_mesh* m, double* xy);   /* double[2][n_nodes] interleaved x,y */
 int  cfd2d_mesh_group_count(const cfd2d_mesh* m, int kind, int* n);
@@ -216,6 +228,28 @@ int  cfd2d_adjoint_objectives(cfd2d_ctx* c, double* J_case);
  * context's gradient and edge-flux buffers as scratch. */
 int  cfd2d_implicit_assemble(cfd2d_ctx* c, int steady, double cfl, double tau, double* blocks, int* cols,
                              double* rhs, double* ctau);
+/* ---- the Riemann solver and the numerical fluxes on given state pairs ----
+ * Parity checks of the device physics row by row, outside any mesh (states
+ * no mesh case reaches: vacuum, strong shocks, Toro's tests).  Host arrays;
+ * the rows are staged through device memory allocated for the call.
+ * max_newton as cfd2d_params.max_newton (<= 0: CFD2D_NEWTON_CAP);
+ * *cap_hits (may be NULL) = rows whose Newton loop hit the cap.
+ *
+ * rim_orig (src/global.cpp:232-405, as FVM_TVD::calcFlux calls it,
+ * fvm_tvd.cpp:602-622): rows double[n][8] = (rho, p, u_n, u_t) of the left
+ * then the right side; out double[n][5] = (rho, e, p, u_n, u_t) sampled at
+ * x/t = 0 (RI, EI, PI, UI, VI); iters int[n] (may be NULL) = Newton
+ * iterations of each row (0 on the vacuum branch); gam = GAM. */
+int  cfd2d_riemann(int n, const double* rows, double gam, int max_newton, double* out, int* iters,
+                   long long* cap_hits);
+/* The numerical flux kind (CFD2D_FLUX_*: GODUNOV = FVM_TVD::calcFlux,
+ * fvm_tvd.cpp:602-622; LAX = FVM_TVD_IMPLICIT::calcFlux,
+ * fvm_tvd_implicit.cpp:939-960; HLLC and ROE extensions) of n state pairs:
+ * rows double[n][10] = (rL, pL, uL, vL, rR, pR, uR, vR, nx, ny), out
+ * double[n][4] = (mass, x-momentum, y-momentum, energy) flux through the
+ * unit normal. */
+int  cfd2d_flux(int kind, int n, const double* rows, double gam, int max_newton, double* out, long long* cap_hits);
+
 /* Page-locked host memory (hipHostMalloc): output buffers of
  * cfd2d_implicit_assemble placed here are copied at the link's full rate.
  * No reference counterpart (the reference's matrix lives in host memory). */
