@@ -89,10 +89,12 @@ def host_cpu():
     return dict(model=model, usable_cores=usable, logical_cpus=os.cpu_count())
 
 
-def reference_cpu(steps=100):
+def reference_cpu(steps=100, opt="O2"):
     """The reference's own FVM_TVD (compiled from its sources into
-    oracle/_ref) on the 8,000-cell shock tube, where it can run."""
-    exe = os.path.join(REPO, "oracle", "_ref", "ref_harness")
+    oracle/_ref) on the 8,000-cell shock tube, where it can run; opt "O0" is
+    the build as its CMakeLists.txt ships it (no optimisation flags,
+    CMakeLists.txt:4-6), "O2" the optimised build."""
+    exe = os.path.join(REPO, "oracle", "_ref", "ref_harness" if opt == "O2" else "ref_harness_O0")
     if not os.path.exists(exe):
         return None
     from cfd2d import meshgen, unv
@@ -114,8 +116,8 @@ def reference_cpu(steps=100):
     line = [l for l in out.splitlines() if l.startswith("{")][-1]
     r = json.loads(line)
     return dict(value=r["cell_updates_per_s"], unit="cell-updates/s", cores=1, kind="reference",
-                sample="%d steps of the reference FVM_TVD (GG, exact Godunov, RK2) on the 8,000-cell tube, g++ -O2"
-                % steps)
+                sample="%d steps of the reference FVM_TVD (GG, exact Godunov, RK2) on the 8,000-cell tube, g++ -%s%s"
+                % (steps, opt, " (as shipped)" if opt == "O0" else ""))
 
 
 def batched_vs_serial(n_cases, steps, seed=7):
@@ -214,7 +216,9 @@ def batched_adjoint(n_cases, steps, seed=11):
     limiter), J_k = y pressure force on case k's bump after `steps` steps.
     One cfd2d_adjoint call for all cases (automatic interval, arena from an
     untimed first call) against the batched forward run of the same steps;
-    the returned gradients include dJ_k/dMach_k and dJ_k/dangle_k."""
+    the returned gradients include dJ_k/dMach_k and dJ_k/dangle_k.  Then one
+    cfd2d_adjoint_geometry call, which also returns every case's dJ_k/d(node
+    coordinates), of which the bump nodes' y derivatives are reported."""
     from cfd2d import batch
     from cfd2d import case as cases
     from cfd2d import meshgen, sensitivity
@@ -239,14 +243,31 @@ def batched_adjoint(n_cases, steps, seed=11):
     t0 = time.perf_counter()
     res = b.adjoint(steps, mask, 0)
     t_adj = time.perf_counter() - t0
+    b.ctx.set_state(U0)
+    b.ctx.adjoint_geometry(steps, mask, 0)  # sizes the arena for the geometry accumulators
+    b.ctx.set_state(U0)
+    b.ctx.sync()
+    t0 = time.perf_counter()
+    gres = b.adjoint_geometry(steps, mask, 0)
+    t_geo = time.perf_counter() - t0
     b.close()
     finite = all(np.isfinite(gU).all() and np.isfinite(gb).all() for _, gU, gb in res)
+    finite_geo = all(np.isfinite(gx).all() for _, _, _, gx in gres)
     grads = [sensitivity.mach_angle_gradient(c, gU, gb) for c, (_, gU, gb) in zip(cs, res)]
+    bump_dy = []
+    for c, (_, _, _, gx) in zip(cs, gres):
+        m = c.mesh
+        nodes = np.unique(m.bedges[m.bedge_group == m.group_index("bump")])
+        bump_dy.append(gx[nodes, 1])
     cells = sum(c.mesh.n_cells for c in cs)
     return dict(cases=n_cases, cells=cells, steps=steps, forward_s=round(t_fwd, 4), adjoint_s=round(t_adj, 4),
                 first_call_s=round(t_first, 4), ratio=round(t_adj / t_fwd, 3),
                 adjoint_cell_steps_per_s=cells * steps / t_adj, finite=bool(finite),
-                dJ_dMach_case0=grads[0][0], dJ_dangle_case0=grads[0][1])
+                dJ_dMach_case0=grads[0][0], dJ_dangle_case0=grads[0][1],
+                geometry=dict(adjoint_s=round(t_geo, 4), ratio=round(t_geo / t_fwd, 3), finite=bool(finite_geo),
+                              bump_nodes=int(sum(len(v) for v in bump_dy)),
+                              dJ_dy_bump_case0_l2=float(np.linalg.norm(bump_dy[0])),
+                              dJ_dy_bump_case0_max=float(np.abs(bump_dy[0]).max())))
 
 
 def renumbering(n, steps=10):
@@ -453,6 +474,9 @@ def main():
         ref = reference_cpu()
         if ref:
             out["cpu_reference"] = ref
+        ref0 = reference_cpu(steps=30, opt="O0")
+        if ref0:
+            out["cpu_reference_as_shipped"] = ref0
     print(json.dumps(out), flush=True)
 
 

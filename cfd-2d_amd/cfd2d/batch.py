@@ -82,6 +82,16 @@ class BatchSolver:
         return [(Jc[k], gU[:, self.cell_off[k]:self.cell_off[k + 1]], gb[self.bc_off[k]:self.bc_off[k + 1]])
                 for k in range(len(self.cases))]
 
+    def adjoint_geometry(self, nsteps, edge_mask, ckpt_every=0):
+        """adjoint() plus each case's dJ_k/d(its node coordinates) as
+        (n_nodes_k, 2) (cfd2d_adjoint_geometry on the concatenated mesh, whose
+        nodes are the cases' nodes in order)."""
+        _, gU, gb, gx = self.ctx.adjoint_geometry(nsteps, edge_mask, ckpt_every)
+        Jc = self.ctx.adjoint_objectives(len(self.cases))
+        no = np.concatenate([[0], np.cumsum([c.mesh.n_nodes for c in self.cases])])
+        return [(Jc[k], gU[:, self.cell_off[k]:self.cell_off[k + 1]], gb[self.bc_off[k]:self.bc_off[k + 1]],
+                 gx[no[k]:no[k + 1]]) for k in range(len(self.cases))]
+
     def wall_force_y(self, edge_mask):
         """Per case: sum over its masked edges of p(c1) * l * n_y on the current
         state (one download of the state for all cases)."""

@@ -58,6 +58,7 @@ SIGNATURES = {
     "cfd2d_mesh_case_offsets": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_void_p, C.c_void_p]),
     "cfd2d_mesh_export": (C.c_int, [C.c_void_p, _dp, _ip, _ip, _dp, _ip, _ip, _dp]),
     "cfd2d_mesh_destroy": (None, [C.c_void_p]),
+    "cfd2d_mesh_geometry_vjp": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, _dp]),
     "cfd2d_ctx_create": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int, _ip, _dp, _ip, _ip,
                                    C.POINTER(C.c_void_p)]),
     "cfd2d_ctx_destroy": (None, [C.c_void_p]),
@@ -82,6 +83,9 @@ SIGNATURES = {
     "cfd2d_adjoint": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip, C.POINTER(C.c_double), C.c_void_p,
                                 C.c_void_p]),
     "cfd2d_adjoint_objectives": (C.c_int, [C.c_void_p, _dp]),
+    "cfd2d_adjoint_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "cfd2d_adjoint_geometry": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, _ip, C.POINTER(C.c_double),
+                                         C.c_void_p, C.c_void_p, _dp]),
     "cfd2d_implicit_assemble": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p]),
     "cfd2d_riemann": (C.c_int, [C.c_int, _dp, C.c_double, C.c_int, _dp, C.c_void_p, C.POINTER(C.c_longlong)]),
@@ -95,6 +99,9 @@ SIGNATURES = {
     "cfd2d_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                  C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "cfd2d_timing_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
+    "cfd2d_edge_shape": (C.c_int, [C.c_void_p, C.c_int]),
+    "cfd2d_tuning": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                               C.POINTER(C.c_int)]),
     "cfd2d_ctx_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cfd2d_ctx_stream": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
 }
@@ -367,6 +374,23 @@ class Context:
                                   C.byref(J), gU.ctypes.data_as(C.c_void_p), gb.ctypes.data_as(C.c_void_p)))
         return J.value, gU.reshape(4, self.nc), gb.reshape(-1, 4)[:self.n_bc]
 
+    def adjoint_geometry(self, nsteps, edge_mask, ckpt_every=20):
+        """adjoint() plus dJ/d(node coordinates) of the context's mesh as
+        (n_nodes, 2); returns (J, dJ/dU0, dJ/dpar, dJ/dxy)."""
+        J = C.c_double()
+        gU = np.zeros(4 * self.nc)
+        gb = np.zeros(4 * max(self.n_bc, 1))
+        gx = np.zeros(2 * self.mesh.nn)
+        check(lib().cfd2d_adjoint_geometry(self._h, self.mesh._h, int(nsteps), int(ckpt_every),
+                                           np.ascontiguousarray(edge_mask, np.int32), C.byref(J),
+                                           gU.ctypes.data_as(C.c_void_p), gb.ctypes.data_as(C.c_void_p), gx))
+        return J.value, gU.reshape(4, self.nc), gb.reshape(-1, 4)[:self.n_bc], gx.reshape(-1, 2)
+
+    def adjoint_graph(self, enable=True):
+        """Record the adjoint's passes as graphs and replay them (default) or
+        launch every kernel directly; the same results."""
+        check(lib().cfd2d_adjoint_graph(self._h, 1 if enable else 0))
+
     def adjoint_objectives(self, n_cases):
         """Per-case objective of the last adjoint call (concatenated meshes)."""
         out = np.zeros(n_cases)
@@ -421,10 +445,22 @@ class Context:
 
     def autotune(self, trial_steps=3):
         """Times the step variants on the current state (restored afterwards)
-        and keeps the fastest; returns dict(fuse, graph, stream, ms_per_step)."""
+        and keeps the fastest; returns dict(fuse, graph, stream, edge_shape,
+        ms_per_step)."""
         f, g, st, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
         check(lib().cfd2d_autotune(self._h, int(trial_steps), C.byref(f), C.byref(g), C.byref(st), C.byref(ms)))
-        return dict(fuse=f.value, graph=g.value, stream=st.value, ms_per_step=ms.value)
+        out = self.tuning()
+        out.update(fuse=f.value, graph=g.value, stream=st.value, ms_per_step=ms.value)
+        return out
+
+    def edge_shape(self, shape):
+        """Launch shape of the edge pass (0..3, cfd2d_edge_shape); same results."""
+        check(lib().cfd2d_edge_shape(self._h, int(shape)))
+
+    def tuning(self):
+        f, g, st, sh = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().cfd2d_tuning(self._h, C.byref(f), C.byref(g), C.byref(st), C.byref(sh)))
+        return dict(fuse=f.value, graph=g.value, stream=st.value, edge_shape=sh.value)
 
     def timing(self, enable=True, kernels=None):
         """Per-kernel HIP-event timing on or off (counters reset); kernels =

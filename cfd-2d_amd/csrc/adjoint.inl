@@ -54,15 +54,25 @@ struct AdjBufs {
 	const int* cslot;   // [3][nc] slot of this cell in its neighbour's face list
 	const int* bedges;  // boundary edges in ascending order
 	int nbedges;
+	// geometry adjoints, summed over every stage (GEO kernels only; each slot
+	// is added to by one thread per kernel):
+	double* Ge;  // [GE_N][ne] per edge: n-bar of the flux at Gauss points 1, 2
+	             // (x, y each), the Gauss-point offset adjoints of side s at
+	             // Gauss point g (x, y) at 4 + 4 s + 2 g, l-bar at GE_L
+	double* Gc;  // [GC_N][nc] per cell: S-bar, then per face k the adjoint of
+	             // the Green-Gauss weight n l (x, y) or of the LSQ weight (x, y)
 };
+
+constexpr int GE_L = 12, GE_N = 13, GC_N = 7;
 
 // d(ghost)/d(inside L, centre C, parameters) applied to gb (r, p, u, v
 // order): the reverse of ghost_state.  Every output component is assigned on
 // every path (the callers add them where they belong).
 __device__ __forceinline__ void ghost_vjp(int type, const double* par, double nx, double ny, const Prim& L,
                                           const Prim& C, const MatV& m, const double* gb, double* dL, double* dC,
-                                          double* dpar)
+                                          double* dpar, double* dn = nullptr)
 {
+	if (dn) dn[0] = dn[1] = 0.0;  // only the walls' mirror depends on the normal
 	if (type == BND_INLET) {
 		// u, v, T, p from the parameters; r = P M / (T gR)
 		double T = par[2], P = par[3];
@@ -107,6 +117,12 @@ __device__ __forceinline__ void ghost_vjp(int type, const double* par, double nx
 		double Unb = -2.0 * (nx * gb[2] + ny * gb[3]);
 		dL[2] = gb[2] + Unb * nx;
 		dL[3] = gb[3] + Unb * ny;
+		if (dn) {
+			// u' = u - 2 n (u.n): d/dn of (u', v') applied to gb
+			double Un = L.u * nx + L.v * ny;
+			dn[0] = -2.0 * (gb[2] * Un + (nx * gb[2] + ny * gb[3]) * L.u);
+			dn[1] = -2.0 * (gb[3] * Un + (nx * gb[2] + ny * gb[3]) * L.v);
+		}
 	}
 	dC[0] = -Tb * C.p / (C.r * den);
 	dC[1] = Tb / den;
@@ -119,7 +135,7 @@ __device__ __forceinline__ void ghost_vjp(int type, const double* par, double nx
 // Written for few divisions: 1/r per side, and dc/dp = c/(2p),
 // dc/dr = -c/(2r) on the side that sets the wave speed.
 __device__ __forceinline__ void lax_vjp(const Prim& L, const Prim& R, double nx, double ny, double G, double igm1,
-                                        const double* fb, double* Lb, double* Rb)
+                                        const double* fb, double* Lb, double* Rb, double* nb = nullptr)
 {
 	double unl = L.u * nx + L.v * ny, unr = R.u * nx + R.v * ny;
 	double irl = 1.0 / L.r, irr = 1.0 / R.r;
@@ -155,6 +171,10 @@ __device__ __forceinline__ void lax_vjp(const Prim& L, const Prim& R, double nx,
 		Rpb += psb;
 		Rrb += rsb;
 	}
+	if (nb) {  // the normal enters through u.n of both sides and the pressure terms
+		nb[0] = unlb * L.u + unrb * R.u + h1 * (R.p + L.p);
+		nb[1] = unlb * L.v + unrb * R.v + h2 * (R.p + L.p);
+	}
 	Lrb += rolb + rulb * L.u + rvlb * L.v + relb * kl;
 	Lub += rulb * L.r + relb * L.r * L.u + unlb * nx;
 	Lvb += rvlb * L.r + relb * L.r * L.v + unlb * ny;
@@ -184,10 +204,166 @@ __device__ __forceinline__ D1 dsqrt(D1 a)
 	return D1{s, a.d / (2.0 * s)};
 }
 
-// HLLC (the same formulas as flux_hllc) on one-direction duals
-__device__ __forceinline__ void hllc_d1(D1* f, const D1* L, const D1* R, double nx, double ny, double G)
+__device__ __forceinline__ D1 operator-(D1 a, double b) { return D1{a.v - b, a.d}; }
+__device__ __forceinline__ D1 operator-(double a, D1 b) { return D1{a - b.v, -b.d}; }
+__device__ __forceinline__ D1 operator+(double a, D1 b) { return D1{a + b.v, b.d}; }
+__device__ __forceinline__ D1 operator*(D1 a, double b) { return D1{a.v * b, a.d * b}; }
+__device__ __forceinline__ D1 operator/(double a, D1 b) { return D1{a / b.v, -a * b.d / (b.v * b.v)}; }
+__device__ __forceinline__ D1 dsel(bool c, D1 a, D1 b) { return c ? a : b; }
+
+// side_root and the fan power on duals: the value as the forward forms it,
+// the derivative of t^(1/k) (k = 1/OGAM on a rarefaction, 2 on a shock)
+__device__ __forceinline__ D1 dside_root(D1 t, bool rare, const RimConst& K)
 {
-	D1 unl = L[2] * D1{nx, 0} + L[3] * D1{ny, 0}, unr = R[2] * D1{nx, 0} + R[3] * D1{ny, 0};
+	double y = side_root(t.v, rare, K);
+	return D1{y, (rare ? K.OGAM : 0.5) * y / t.v * t.d};
+}
+
+__device__ __forceinline__ D1 dpow_iagam(D1 x, const RimConst& K)
+{
+	double y = pow_iagam(x.v, K);
+	return D1{y, K.IAGAM * y / x.v * x.d};
+}
+
+// rim (physics.hpp) on one-direction duals: the same operations, branches
+// and Newton iterations as the forward evaluation of the same inputs (every
+// test reads the value parts), so the tangent is that of the discrete map
+// the forward step evaluates, iteration count and stop rule included.
+__device__ __forceinline__ void rim_d1(D1& RI, D1& EI, D1& PI, D1& UI, D1& VI, D1 RB, D1 PB, D1 UB, D1 VB, D1 RE,
+                                       D1 PE, D1 UE, D1 VE, const RimConst& K, int max_iter)
+{
+	const double eps = 1.0e-5;
+	D1 zero{0.0, 0.0};
+	D1 US = zero, UF = zero, RF, RS, EF, ES, SBL, SFL, SSL, SEL;
+	D1 iB = 1.0 / (RB * PB), iE = 1.0 / (RE * PE);
+	D1 IRB = PB * iB, IPB = RB * iB, IRE = PE * iE, IPE = RE * iE;
+	D1 CB = dsqrt(K.GAM * PB * IRB), CE = dsqrt(K.GAM * PE * IRE);
+	D1 RCB = RB * CB, RCE = RE * CE;
+	D1 EB = CB * CB * K.ISGAM, EE = CE * CE * K.ISGAM;
+	D1 DU = UB - UE;
+	if (DU.v * K.AGAM < -2.0 * (CB.v + CE.v)) {
+		RF = zero; RS = zero; EF = zero; ES = zero;
+		SBL = UB - CB;
+		SFL = UB + 2.0 * CB / K.AGAM;
+		SSL = UE - 2.0 * CE / K.AGAM;
+		SEL = UE + CE;
+	} else {
+		D1 P = (PB * RCE + PE * RCB + DU * RCB * RCE) / (RCB + RCE);
+		int it = 0;
+		for (;;) {
+			if (P.v < eps) P = D1{eps, 0.0};
+			D1 F[2], FS[2];
+			for (int sd = 0; sd < 2; sd++) {
+				D1 PK = sd ? PE : PB, IPK = sd ? IPE : IPB, C = sd ? CE : CB, RC = sd ? RCE : RCB;
+				D1 x = P * IPK;
+				bool rare = PK.v > P.v;
+				D1 t = dsel(rare, x, K.PGAM * x + K.OGAM);
+				D1 y = dside_root(t, rare, K);
+				D1 r = 1.0 / dsel(rare, K.GAM * P, RC * y * t);
+				F[sd] = dsel(rare, K.DGAM * (C * y - C), (P - PK) * t * r);
+				FS[sd] = dsel(rare, C * y * r, (K.QGAM * x + K.FGAM) * r * K.IRGAM);
+			}
+			D1 res = DU - F[0] - F[1];
+			P = P + res / (FS[0] + FS[1]);
+			++it;
+			if (!(fabs(res.v) > eps)) break;
+			if (max_iter > 0 && it >= max_iter) break;
+		}
+		for (int sd = 0; sd < 2; sd++) {
+			D1 PK = sd ? PE : PB, IPK = sd ? IPE : IPB, IRK = sd ? IRE : IRB, C = sd ? CE : CB;
+			D1 RK = sd ? RE : RB, UK = sd ? UE : UB;
+			double sg = sd ? 1.0 : -1.0;
+			bool rare = PK.v > P.v;
+			D1 t = dsel(rare, P * IPK, (K.TGAM * P + K.HGAM * PK) * IRK);
+			D1 y = dside_root(t, rare, K);
+			D1 Z = C * y;
+			D1 Er = Z * Z * K.ISGAM;
+			D1 D = UK + sg * y;
+			D1 M = (-sg) * (RK * y);
+			D1 Q = PK - P + M * ((-sg) * y);
+			D1 r = 1.0 / dsel(rare, K.AGAM * Er, Q * M);
+			D1 Rs, Es, Us, head, tail;
+			if (rare) {
+				Es = Er;
+				Us = UK + (sg * K.DGAM) * (Z - C);
+				Rs = P * r;
+				head = UK + sg * C;
+				tail = Us + sg * Z;
+			} else {
+				D1 IM = Q * r;
+				Rs = M * M * M * r;
+				Us = D + Q * IM;
+				Es = P * Q * IM * IM * K.IAGAM;
+				head = D;
+				tail = D;
+			}
+			if (sd) {
+				RS = Rs; ES = Es; US = Us; SEL = head; SSL = tail;
+			} else {
+				RF = Rs; EF = Es; UF = Us; SBL = head; SFL = tail;
+			}
+		}
+	}
+	if (SEL.v <= 0.0) {
+		RI = RE; EI = EE; UI = UE; VI = VE;
+	} else if (SBL.v >= 0.0) {
+		RI = RB; EI = EB; UI = UB; VI = VB;
+	} else if ((SSL.v >= 0.0) && (SFL.v <= 0.0)) {
+		if (US.v >= 0.0) {
+			RI = RF; EI = EF; UI = UF; VI = VB;
+		} else {
+			RI = RS; EI = ES; UI = US; VI = VE;
+		}
+	} else if (SFL.v > 0.0) {
+		UI = (UB + K.DGGAM * dsqrt(EB)) * K.IDGAM1;
+		VI = VB;
+		EI = (UI * UI) * K.ISGAM;
+		RI = RB * dpow_iagam(EI / EB, K);
+	} else {
+		UI = (UE - K.DGGAM * dsqrt(EE)) * K.IDGAM1;
+		VI = VE;
+		EI = (UI * UI) * K.ISGAM;
+		RI = RE * dpow_iagam(EI / EE, K);
+	}
+	PI = K.AGAM * EI * RI;
+}
+
+// The exact Godunov flux's vector-Jacobian product by forward mode: one dual
+// evaluation of flux_godunov per input direction (the eight states, and the
+// normal when nb is given), each dotted with fb.
+__device__ __forceinline__ void godunov_vjp(const Prim& L, const Prim& R, double nx, double ny, const RimConst& K,
+                                            int max_iter, const double* fb, double* Lb, double* Rb,
+                                            double* nb = nullptr)
+{
+	const double lv[4] = {L.r, L.p, L.u, L.v}, rv[4] = {R.r, R.p, R.u, R.v};
+	int nd = nb ? 10 : 8;
+	for (int j = 0; j < nd; j++) {
+		D1 l[4], r[4];
+		for (int q = 0; q < 4; q++) {
+			l[q] = D1{lv[q], (j == q) ? 1.0 : 0.0};
+			r[q] = D1{rv[q], (j == 4 + q) ? 1.0 : 0.0};
+		}
+		D1 Nx{nx, j == 8 ? 1.0 : 0.0}, Ny{ny, j == 9 ? 1.0 : 0.0};
+		D1 unl = l[2] * Nx + l[3] * Ny, unr = r[2] * Nx + r[3] * Ny;
+		D1 utl = l[2] * Ny - l[3] * Nx, utr = r[2] * Ny - r[3] * Nx;
+		D1 RI, EI, PI, UN, UT;
+		rim_d1(RI, EI, PI, UN, UT, l[0], l[1], unl, utl, r[0], r[1], unr, utr, K, max_iter);
+		D1 UI = UN * Nx + UT * Ny, VI = UN * Ny - UT * Nx;
+		D1 f0 = RI * UN;
+		D1 f1 = f0 * UI + PI * Nx;
+		D1 f2 = f0 * VI + PI * Ny;
+		D1 f3 = (RI * (EI + 0.5 * (UI * UI + VI * VI)) + PI) * UN;
+		double s = fb[0] * f0.d + fb[1] * f1.d + fb[2] * f2.d + fb[3] * f3.d;
+		if (j < 4) Lb[j] = s;
+		else if (j < 8) Rb[j - 4] = s;
+		else nb[j - 8] = s;
+	}
+}
+
+// HLLC (the same formulas as flux_hllc) on one-direction duals
+__device__ __forceinline__ void hllc_d1(D1* f, const D1* L, const D1* R, D1 nx, D1 ny, double G)
+{
+	D1 unl = L[2] * nx + L[3] * ny, unr = R[2] * nx + R[3] * ny;
 	D1 cl = dsqrt(G * L[1] / L[0]), cr = dsqrt(G * R[1] / R[0]);
 	D1 El = L[1] / (G - 1.0) + 0.5 * L[0] * (L[2] * L[2] + L[3] * L[3]);
 	D1 Er = R[1] / (G - 1.0) + 0.5 * R[0] * (R[2] * R[2] + R[3] * R[3]);
@@ -196,15 +372,15 @@ __device__ __forceinline__ void hllc_d1(D1* f, const D1* L, const D1* R, double 
 	D1 sr = b1.v > b2.v ? b1 : b2;
 	if (sl.v >= 0.0) {
 		f[0] = L[0] * unl;
-		f[1] = L[0] * L[2] * unl + L[1] * D1{nx, 0};
-		f[2] = L[0] * L[3] * unl + L[1] * D1{ny, 0};
+		f[1] = L[0] * L[2] * unl + L[1] * nx;
+		f[2] = L[0] * L[3] * unl + L[1] * ny;
 		f[3] = (El + L[1]) * unl;
 		return;
 	}
 	if (sr.v <= 0.0) {
 		f[0] = R[0] * unr;
-		f[1] = R[0] * R[2] * unr + R[1] * D1{nx, 0};
-		f[2] = R[0] * R[3] * unr + R[1] * D1{ny, 0};
+		f[1] = R[0] * R[2] * unr + R[1] * nx;
+		f[2] = R[0] * R[3] * unr + R[1] * ny;
 		f[3] = (Er + R[1]) * unr;
 		return;
 	}
@@ -214,9 +390,9 @@ __device__ __forceinline__ void hllc_d1(D1* f, const D1* L, const D1* R, double 
 	const D1* K = lft ? L : R;
 	D1 sk = lft ? sl : sr, unk = lft ? unl : unr, Ek = lft ? El : Er;
 	D1 k = (lft ? ml : mr) / (sk - ss);
-	D1 us0 = k, us1 = k * (K[2] + (ss - unk) * D1{nx, 0}), us2 = k * (K[3] + (ss - unk) * D1{ny, 0});
+	D1 us0 = k, us1 = k * (K[2] + (ss - unk) * nx), us2 = k * (K[3] + (ss - unk) * ny);
 	D1 us3 = k * (Ek / K[0] + (ss - unk) * (ss + K[1] / (K[0] * (sk - unk))));
-	D1 fk0 = K[0] * unk, fk1 = K[0] * K[2] * unk + K[1] * D1{nx, 0}, fk2 = K[0] * K[3] * unk + K[1] * D1{ny, 0};
+	D1 fk0 = K[0] * unk, fk1 = K[0] * K[2] * unk + K[1] * nx, fk2 = K[0] * K[3] * unk + K[1] * ny;
 	D1 fk3 = (Ek + K[1]) * unk;
 	f[0] = fk0 + sk * (us0 - K[0]);
 	f[1] = fk1 + sk * (us1 - K[0] * K[2]);
@@ -224,20 +400,23 @@ __device__ __forceinline__ void hllc_d1(D1* f, const D1* L, const D1* R, double 
 	f[3] = fk3 + sk * (us3 - Ek);
 }
 
+// nb (if given): the adjoint of the normal, two more forward-mode directions
 __device__ __forceinline__ void hllc_vjp(const Prim& L, const Prim& R, double nx, double ny, double G, const double* fb,
-                                         double* Lb, double* Rb)
+                                         double* Lb, double* Rb, double* nb = nullptr)
 {
 	const double lv[4] = {L.r, L.p, L.u, L.v}, rv[4] = {R.r, R.p, R.u, R.v};
-	for (int j = 0; j < 8; j++) {
+	int nd = nb ? 10 : 8;
+	for (int j = 0; j < nd; j++) {
 		D1 l[4], r[4], f[4];
 		for (int q = 0; q < 4; q++) {
 			l[q] = D1{lv[q], (j == q) ? 1.0 : 0.0};
 			r[q] = D1{rv[q], (j == 4 + q) ? 1.0 : 0.0};
 		}
-		hllc_d1(f, l, r, nx, ny, G);
+		hllc_d1(f, l, r, D1{nx, j == 8 ? 1.0 : 0.0}, D1{ny, j == 9 ? 1.0 : 0.0}, G);
 		double s = fb[0] * f[0].d + fb[1] * f[1].d + fb[2] * f[2].d + fb[3] * f[3].d;
 		if (j < 4) Lb[j] = s;
-		else Rb[j - 4] = s;
+		else if (j < 8) Rb[j - 4] = s;
+		else nb[j - 8] = s;
 	}
 }
 
@@ -245,7 +424,7 @@ __device__ __forceinline__ void hllc_vjp(const Prim& L, const Prim& R, double nx
 // with Rb = scale * tau/S * W; each lane takes one Gauss point: the flux VJP,
 // the ghost VJP on a boundary, then its (r, p, u, v) adjoints on both sides.
 // Each lane adds its boundary-parameter adjoints into its own Ebc slot.
-template <bool MM, int FLUX>
+template <bool MM, int FLUX, bool GEO>
 __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, double scale)
 {
 	int t = xcd_block() * BLOCK + threadIdx.x;
@@ -275,23 +454,54 @@ __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, 
 	} else {
 		R = ghost_of<MM>(b, ph, e, c1, L, p1);
 	}
-	double Lb[4], Rb[4];
-	if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb);
-	else lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb);
+	double Lb[4], Rb[4], nb[2] = {0.0, 0.0};
+	if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb, GEO ? nb : nullptr);
+	else if (FLUX == CFD2D_FLUX_GODUNOV) godunov_vjp(L, R, nx, ny, ph.rk, ph.max_newton, fb, Lb, Rb, GEO ? nb : nullptr);
+	else lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb, GEO ? nb : nullptr);
+	double Rsb[4];  // adjoint of c2's reconstructed state (inner edges)
 	if (c2 < 0) {
 		// the ghost's inputs: the reconstructed L, the cell centre, the parameters
 		int bc = -c2 - 1;
-		double dL[4], dC[4], dpar[4];
-		ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], nx, ny, L, p1, mat_of<MM>(b, ph, c1), Rb, dL, dC, dpar);
+		double dL[4], dC[4], dpar[4], dn[2];
+		ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], nx, ny, L, p1, mat_of<MM>(b, ph, c1), Rb, dL, dC, dpar, dn);
 		for (int q = 0; q < 4; q++) {
 			Lb[q] += dL[q];
+			Rsb[q] = 0.0;
 			Rb[q] = dC[q];
 			a.Ebc[(size_t)(4 * gp + q) * ne + e] += dpar[q];
 		}
+		nb[0] += dn[0];
+		nb[1] += dn[1];
+	} else {
+		for (int q = 0; q < 4; q++) Rsb[q] = Rb[q];
 	}
 	for (int q = 0; q < 4; q++) {
 		a.Ea[(size_t)(4 * gp + q) * ne + e] = Lb[q];
 		a.Ea[(size_t)(8 + 4 * gp + q) * ne + e] = Rb[q];
+	}
+	if (GEO) {
+		// this lane's Gauss point: the normal, and each side's offset (the
+		// reconstruction phi + g . d gives d-bar = sum_q phi-bar_q g_q)
+		double g1[8], g2[8];
+		ld8(b.G, c1, g1);
+		double d1x = 0.0, d1y = 0.0, d2x = 0.0, d2y = 0.0;
+		for (int q = 0; q < 4; q++) {
+			d1x += Lb[q] * g1[2 * q];
+			d1y += Lb[q] * g1[2 * q + 1];
+		}
+		if (c2 >= 0) {
+			ld8(b.G, c2, g2);
+			for (int q = 0; q < 4; q++) {
+				d2x += Rsb[q] * g2[2 * q];
+				d2y += Rsb[q] * g2[2 * q + 1];
+			}
+		}
+		a.Ge[(size_t)(2 * gp) * ne + e] += nb[0];
+		a.Ge[(size_t)(2 * gp + 1) * ne + e] += nb[1];
+		a.Ge[(size_t)(4 + 2 * gp) * ne + e] += d1x;
+		a.Ge[(size_t)(5 + 2 * gp) * ne + e] += d1y;
+		a.Ge[(size_t)(8 + 2 * gp) * ne + e] += d2x;
+		a.Ge[(size_t)(9 + 2 * gp) * ne + e] += d2y;
 	}
 }
 
@@ -302,7 +512,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, 
 // (P, G) adjoints go straight to Pb, Gb.  Two lanes per cell, one Gauss point
 // each, summed by a lane shuffle.  A cell owns its boundary faces and adds
 // their parameter adjoints into Ebc (one slot per Gauss point).
-template <bool MM, int FLUX>
+template <bool MM, int FLUX, bool GEO>
 __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, double scale)
 {
 	int t = xcd_block() * BLOCK + threadIdx.x;
@@ -354,24 +564,38 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 			L = own;
 			R = ghost_of<MM>(b, ph, e, i, L, pi);
 		}
-		double Lb[4], Rb[4];
-		if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb);
-		else lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb);
+		double Lb[4], Rb[4], nb[2] = {0.0, 0.0};
+		if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb, GEO ? nb : nullptr);
+		else lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb, GEO ? nb : nullptr);
 		if (j < 0) {
 			int bc = -edge_cells(b, e).y - 1;
-			double dL[4], dC[4], dpar[4];
-			ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], nx, ny, L, pi, mat_of<MM>(b, ph, i), Rb, dL, dC, dpar);
+			double dL[4], dC[4], dpar[4], dn[2];
+			ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], nx, ny, L, pi, mat_of<MM>(b, ph, i), Rb, dL, dC, dpar, dn);
 			for (int q = 0; q < 4; q++) {
 				Lb[q] += dL[q];
 				pb[q] += dC[q];
 				a.Ebc[(size_t)(4 * gp + q) * ne + e] += dpar[q];
 			}
+			nb[0] += dn[0];
+			nb[1] += dn[1];
 		}
+		double ddx = 0.0, ddy = 0.0;  // adjoint of this cell's offset to this Gauss point
 		for (int q = 0; q < 4; q++) {
 			double o = side ? Rb[q] : Lb[q];
 			pb[q] += o;
 			gb[2 * q] += o * dx;
 			gb[2 * q + 1] += o * dy;
+			ddx += o * gi[2 * q];
+			ddy += o * gi[2 * q + 1];
+		}
+		if (GEO) {
+			// the face's normal is c1's to account for (both cells evaluate it)
+			if (side == 0) {
+				a.Ge[(size_t)(2 * gp) * ne + e] += nb[0];
+				a.Ge[(size_t)(2 * gp + 1) * ne + e] += nb[1];
+			}
+			a.Ge[(size_t)(4 + 4 * side + 2 * gp) * ne + e] += ddx;
+			a.Ge[(size_t)(5 + 4 * side + 2 * gp) * ne + e] += ddy;
 		}
 	}
 	for (int q = 0; q < 4; q++) pb[q] += __shfl_xor(pb[q], 1);
@@ -416,7 +640,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_cell(Bufs b, AdjBufs a)
 // Reverse of the gradient operator and of the primitive conversion, then the
 // RK bookkeeping: Ws = b W + (dP/dU)^T Pb, and acc0 per MODE
 // (0: Acc = a W, 1: Acc += a W, 2: Ws += Acc, the step's first stage).
-template <bool MM, bool LSQ, int MODE>
+template <bool MM, bool LSQ, int MODE, bool GEO>
 __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, double wa, double wb)
 {
 	__shared__ double lds_w[BLOCK * 5], lds_a[BLOCK * 5];
@@ -432,6 +656,29 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 		double gb[8];
 		lda8(a.Gb, b.nc, i, gb);
 		double isi = LSQ ? 0.0 : 1.0 / b.S[i];
+		double w[4];
+		ld4(a.Wn, i, w);
+		double Sb = 0.0, rbi[4];  // GEO: the cell's area adjoint; wb tau/S W (the stage's residual seed)
+		if (GEO) {
+			// the stage residual R_i (the forward fluxes of this stage, gathered
+			// as the update pass does): U' = ... + wb (tau/S) R, so
+			// d/dS (tau/S) = -(tau/S)/S gives S-bar -= wb (tau/S) (W . R) / S
+			double R[4];
+			gather_residual(b, i, R);
+			double si = wb * b.dtS[i], wr = 0.0;
+			for (int q = 0; q < 4; q++) {
+				wr += w[q] * R[q];
+				rbi[q] = si * w[q];
+			}
+			Sb -= si * wr / b.S[i];
+			if (!LSQ) {  // Green-Gauss: G = (sum over faces) / S
+				double g[8];
+				ld8(b.G, i, g);
+				double gg = 0.0;
+				for (int q = 0; q < 8; q++) gg += gb[q] * g[q];
+				Sb -= gg * isi;
+			}
+		}
 		CellTop tp = load_top(b, i);
 		for (int k = 0; k < 3; k++) {
 			int f = tp.f[k];
@@ -452,6 +699,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 					pb[q] += nbb[q];
 				}
 			}
+			Prim pk = pc;  // GEO: the value this face contributed (neighbour or ghost)
 			if (j >= 0) {
 				// the neighbour's gradient depends on this cell's value
 				int s = a.cslot[k * n + i];
@@ -466,17 +714,53 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 					double isj = 1.0 / b.S[j];
 					for (int q = 0; q < 4; q++) pb[q] += sg * 0.5 * l * (nx * gj[2 * q] + ny * gj[2 * q + 1]) * isj;
 				}
+				if (GEO) pk = load_prim(b, j);
 			} else {
 				// ghost of the cell-centre state across a boundary edge
 				int bc = -edge_cells(b, e).y - 1;
-				double dL[4], dC[4], dpar[4];
-				ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], edge_geo(b, e)[0], edge_geo(b, e)[1], pc, pc, m, nbb, dL, dC, dpar);
+				double dL[4], dC[4], dpar[4], dn[2];
+				ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], edge_geo(b, e)[0], edge_geo(b, e)[1], pc, pc, m, nbb, dL, dC,
+				          dpar, GEO ? dn : nullptr);
 				for (int q = 0; q < 4; q++) {
 					pb[q] += dL[q] + dC[q];
 					a.Ebc[(size_t)q * ne + e] += dpar[q];
 				}
+				if (GEO) {
+					a.Ge[e] += dn[0];  // a boundary face is its cell's own (side 0)
+					a.Ge[ne + e] += dn[1];
+					pk = ghost_of<MM>(b, ph, e, i, pc, pc);
+				}
+			}
+			if (GEO) {
+				// the face's weight: Green-Gauss (sign n l) takes (phi_i + phi_k)/2 / S,
+				// least squares w_k takes (phi_k - phi_i)
+				double vi[4] = {pc.r, pc.p, pc.u, pc.v}, vk[4] = {pk.r, pk.p, pk.u, pk.v};
+				double wbx = 0.0, wby = 0.0;
+				for (int q = 0; q < 4; q++) {
+					double t = LSQ ? vk[q] - vi[q] : 0.5 * (vi[q] + vk[q]) * isi;
+					wbx += gb[2 * q] * t;
+					wby += gb[2 * q + 1] * t;
+				}
+				a.Gc[(size_t)(1 + 2 * k) * n + i] += wbx;
+				a.Gc[(size_t)(2 + 2 * k) * n + i] += wby;
+				if (side == 0) {
+					// the edge length: F_e = (f_1 + f_2) l/2 enters R_c1 with -, R_c2 with +
+					double F[4];
+					ld4(b.F, e, F);
+					double rbj[4] = {0.0, 0.0, 0.0, 0.0};
+					if (j >= 0) {
+						double wj[4];
+						ld4(a.Wn, j, wj);
+						double sj = wb * b.dtS[j];
+						for (int q = 0; q < 4; q++) rbj[q] = sj * wj[q];
+					}
+					double lb = 0.0;
+					for (int q = 0; q < 4; q++) lb += F[q] * (rbj[q] - rbi[q]);
+					a.Ge[(size_t)GE_L * ne + e] += lb / edge_geo(b, e)[2];
+				}
 			}
 		}
+		if (GEO) a.Gc[i] += Sb;
 		// P = (r, p, u, v)(U): the transposed Jacobian
 		double r = pc.r, u = pc.u, v = pc.v, g1 = m.gam - 1.0;
 		double ub[4];
@@ -484,8 +768,6 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 		ub[1] = pb[2] / r - pb[1] * g1 * u;
 		ub[2] = pb[3] / r - pb[1] * g1 * v;
 		ub[3] = pb[1] * g1;
-		double w[4];
-		ld4(a.Wn, i, w);
 		for (int q = 0; q < 4; q++) ws[q] = wb * w[q] + ub[q];
 		if (MODE == 0) {
 			for (int q = 0; q < 4; q++) acc[q] = wa * w[q];
@@ -587,7 +869,7 @@ struct Carve {
 };
 
 void carve_work(AdjWork& W, Carve& cv, int nck, size_t nrec, size_t nc, size_t ne, int nbc, bool edge_buffer,
-                size_t K)
+                size_t K, bool geo)
 {
 	size_t n4 = 4 * nc, n8 = 8 * nc;
 	W.ckpt.assign(nck - 1, nullptr);
@@ -603,6 +885,8 @@ void carve_work(AdjWork& W, Carve& cv, int nck, size_t nrec, size_t nc, size_t n
 	W.a.Ea = edge_buffer ? cv.take<double>((size_t)EA * ne) : nullptr;
 	W.a.Ebc = cv.take<double>(8 * ne);
 	W.a.bcbar = cv.take<double>(4 * (size_t)(nbc > 0 ? nbc : 1));
+	W.a.Ge = geo ? cv.take<double>(GE_N * ne) : nullptr;
+	W.a.Gc = geo ? cv.take<double>(GC_N * nc) : nullptr;
 	W.seed_cells = cv.take<int>(K);
 	W.seed_w = cv.take<double>(K);
 	W.seed_p = cv.take<double>(K);
@@ -692,31 +976,45 @@ void record_steps(cfd2d_ctx* c, AdjWork* A, int nsteps)
 	c->b.G = G0;
 }
 
-template <bool MM, bool LSQ>
+template <bool MM, bool LSQ, bool GEO>
 void launch_adj_grad(cfd2d_ctx* c, AdjWork* A, double wa, double wb, int mode)
 {
 	dim3 g(grid_of(c->nc)), blk(BLOCK);
-	if (mode == 0) hipLaunchKernelGGL((k_adj_grad<MM, LSQ, 0>), g, blk, 0, c->st, c->b, c->ph, A->a, wa, wb);
-	else if (mode == 1) hipLaunchKernelGGL((k_adj_grad<MM, LSQ, 1>), g, blk, 0, c->st, c->b, c->ph, A->a, wa, wb);
-	else hipLaunchKernelGGL((k_adj_grad<MM, LSQ, 2>), g, blk, 0, c->st, c->b, c->ph, A->a, wa, wb);
+	if (mode == 0) hipLaunchKernelGGL((k_adj_grad<MM, LSQ, 0, GEO>), g, blk, 0, c->st, c->b, c->ph, A->a, wa, wb);
+	else if (mode == 1) hipLaunchKernelGGL((k_adj_grad<MM, LSQ, 1, GEO>), g, blk, 0, c->st, c->b, c->ph, A->a, wa, wb);
+	else hipLaunchKernelGGL((k_adj_grad<MM, LSQ, 2, GEO>), g, blk, 0, c->st, c->b, c->ph, A->a, wa, wb);
 }
 
-template <bool MM>
+// GEO: geometry adjoints too; the stage's forward fluxes are formed again
+// first (edge pass on the recorded P and G, into the context's flux buffer)
+// for the length and area terms
+template <bool MM, bool GEO>
 void launch_adj_stage(cfd2d_ctx* c, AdjWork* A, double wa, double wb, int mode)
 {
-	if (c->p.flux == CFD2D_FLUX_HLLC) {
-		// VALU-heavy VJP: once per edge, through the edge buffer
-		hipLaunchKernelGGL((k_adj_edge<MM, CFD2D_FLUX_HLLC>), dim3(grid_of(2 * c->ne)), dim3(BLOCK), 0, c->st, c->b,
-		                   c->ph, A->a, wb);
+	if (GEO) launch_edge<MM>(c);
+	if (c->p.flux == CFD2D_FLUX_HLLC || c->p.flux == CFD2D_FLUX_GODUNOV) {
+		// VALU-heavy VJPs (forward-mode directions): once per edge, through the edge buffer
+		if (c->p.flux == CFD2D_FLUX_HLLC)
+			hipLaunchKernelGGL((k_adj_edge<MM, CFD2D_FLUX_HLLC, GEO>), dim3(grid_of(2 * c->ne)), dim3(BLOCK), 0,
+			                   c->st, c->b, c->ph, A->a, wb);
+		else
+			hipLaunchKernelGGL((k_adj_edge<MM, CFD2D_FLUX_GODUNOV, GEO>), dim3(grid_of(2 * c->ne)), dim3(BLOCK), 0,
+			                   c->st, c->b, c->ph, A->a, wb);
 		hipLaunchKernelGGL(k_adj_cell, dim3(grid_of(c->nc)), dim3(BLOCK), 0, c->st, c->b, A->a);
 	} else {
 		// cheap VJP: evaluated from both sides, no edge buffer
-		hipLaunchKernelGGL((k_adj_face<MM, CFD2D_FLUX_LAX>), dim3(grid_of(2 * c->nc)), dim3(BLOCK), 0, c->st, c->b,
-		                   c->ph, A->a, wb);
+		hipLaunchKernelGGL((k_adj_face<MM, CFD2D_FLUX_LAX, GEO>), dim3(grid_of(2 * c->nc)), dim3(BLOCK), 0, c->st,
+		                   c->b, c->ph, A->a, wb);
 	}
-	if (c->p.gradient == CFD2D_GRAD_LSQ) launch_adj_grad<MM, true>(c, A, wa, wb, mode);
-	else launch_adj_grad<MM, false>(c, A, wa, wb, mode);
+	if (c->p.gradient == CFD2D_GRAD_LSQ) launch_adj_grad<MM, true, GEO>(c, A, wa, wb, mode);
+	else launch_adj_grad<MM, false, GEO>(c, A, wa, wb, mode);
 }
+
+// what a geometry-sensitivity call brings back from the reverse sweep
+struct GeoOut {
+	std::vector<double> Ge, Gc;  // [GE_N][ne], [GC_N][nc]
+	std::vector<double> lny;     // per edge: the objective's p(c1) where masked (adjoint of l n_y)
+};
 
 // steps per segment when the caller leaves it to the library: the largest
 // interval whose stage records fit in half of the free device memory (the
@@ -732,8 +1030,58 @@ int auto_interval(cfd2d_ctx* c, int nsteps)
 	return k >= nsteps ? nsteps : (int)k;
 }
 
+__global__ __launch_bounds__(BLOCK) void k_copy(double* dst, const double* src, size_t count)
+{
+	size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+	if (i < count) dst[i] = src[i];
+}
+
+__global__ void k_zero_int(int* x) { *x = 0; }
+
+// device-to-device copy as a kernel (recorded into the adjoint's graphs)
+void launch_copy(cfd2d_ctx* c, double* dst, const double* src, size_t count)
+{
+	hipLaunchKernelGGL(k_copy, dim3((unsigned)((count + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, c->st, dst, src, count);
+}
+
+// Runs enq (work on the context stream only) recorded as a graph and
+// replayed: recorded again only when key changes (sizes, checkpoint
+// interval, objective size, the arena's address, the step variant), so a
+// repeated call with the same shape is one graph launch per pass.  With
+// graphs off the work is enqueued directly.
+template <typename F>
+int adj_pass(cfd2d_ctx* c, cfd2d_ctx::AdjGraph& G, const std::vector<long long>& key, F enq)
+{
+	if (!c->adj_use_graph) {
+		enq();
+		return 0;
+	}
+	if (!G.x || G.key != key) {
+		drop_adj_graph(G);
+		c->capturing = true;
+		hipError_t e = hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal);
+		if (e != hipSuccess) {
+			c->capturing = false;
+			return hip_fail(e, "hipStreamBeginCapture");
+		}
+		enq();
+		e = hipStreamEndCapture(c->st, &G.g);
+		c->capturing = false;
+		if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+		e = hipGraphInstantiate(&G.x, G.g, nullptr, nullptr, 0);
+		if (e != hipSuccess) {
+			drop_adj_graph(G);
+			return hip_fail(e, "hipGraphInstantiate");
+		}
+		G.key = key;
+	}
+	HIPCHK(hipGraphLaunch(G.x, c->st));
+	return 0;
+}
+
 template <bool MM>
-int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, double* dJdU0, double* dJdbc)
+int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, double* dJdU0, double* dJdbc,
+                GeoOut* geo = nullptr)
 {
 	int nc = c->nc, ne = c->ne, S = c->p.rk_stages;
 	size_t n4 = 4 * (size_t)nc, n8 = 8 * (size_t)nc, bytes = n4 * sizeof(double);
@@ -762,12 +1110,12 @@ int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, do
 	int K = (int)cells.size();
 	AdjWork work;
 	AdjWork* A = &work;
-	bool edge_buffer = c->p.flux == CFD2D_FLUX_HLLC;
+	bool edge_buffer = c->p.flux == CFD2D_FLUX_HLLC || c->p.flux == CFD2D_FLUX_GODUNOV;
 	Carve sizing{nullptr};
-	carve_work(work, sizing, nck, nrec, nc, ne, c->nbc, edge_buffer, K);
+	carve_work(work, sizing, nck, nrec, nc, ne, c->nbc, edge_buffer, K, geo != nullptr);
 	if ((rc = ensure_arena(c, sizing.off))) return rc;
 	Carve cv{(char*)c->adj_ws};
-	carve_work(work, cv, nck, nrec, nc, ne, c->nbc, edge_buffer, K);
+	carve_work(work, cv, nck, nrec, nc, ne, c->nbc, edge_buffer, K, geo != nullptr);
 	AdjBufs& a = A->a;
 	a.cslot = c->adj_cslot;
 	a.bedges = c->adj_bedges;
@@ -776,25 +1124,35 @@ int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, do
 		HIPCHK(hipMemcpyAsync(A->seed_cells, cells.data(), K * sizeof(int), hipMemcpyHostToDevice, c->st));
 		HIPCHK(hipMemcpyAsync(A->seed_w, wsum.data(), K * sizeof(double), hipMemcpyHostToDevice, c->st));
 	}
-	launch_fill(c, a.Ebc, 8 * (size_t)ne);
-	HIPCHK(hipMemsetAsync(c->b.anyflag + 1, 0, sizeof(int), c->st));
-
-	// forward pass: checkpoints, then the last segment recorded in place
+	// what the recorded passes depend on besides the context itself
+	std::vector<long long> key = {nsteps, ck, K, geo != nullptr, (long long)(size_t)c->adj_ws, c->fuse_mode,
+	                              c->stream_mode, c->edge_shape, c->nbc, c->adj_nbedges};
 	int last0 = (nck - 1) * ck;
-	for (int s = 0; s < last0; s++) {
-		if (s % ck == 0) HIPCHK(hipMemcpyAsync(A->ckpt[s / ck], c->b.U, bytes, hipMemcpyDeviceToDevice, c->st));
-		step_once(c);
-	}
-	record_steps<MM>(c, A, nsteps - last0);
-	HIPCHK(hipMemcpyAsync(A->Ufin, c->b.U, bytes, hipMemcpyDeviceToDevice, c->st));
-	// seed dJ/dU at the final state (its primitives are the last P slot)
-	launch_fill(c, a.Wn, n4);
-	if (K > 0) {
-		Bufs bf = c->b;
-		bf.P = A->Pst + (size_t)(nsteps - last0) * S * n4;
-		hipLaunchKernelGGL((k_seed<MM>), dim3(grid_of(K)), dim3(BLOCK), 0, c->st, bf, c->ph, a, A->seed_cells,
-		                   A->seed_w, K, A->seed_p);
-	}
+
+	// forward pass: checkpoints, then the last segment recorded in place, and
+	// the seed dJ/dU at the final state (its primitives are the last P slot)
+	auto forward = [&]() {
+		launch_fill(c, a.Ebc, 8 * (size_t)ne);
+		if (geo) {
+			launch_fill(c, a.Ge, GE_N * (size_t)ne);
+			launch_fill(c, a.Gc, GC_N * (size_t)nc);
+		}
+		hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, c->st, c->b.anyflag + 1);
+		for (int s = 0; s < last0; s++) {
+			if (s % ck == 0) launch_copy(c, A->ckpt[s / ck], c->b.U, n4);
+			step_once(c);
+		}
+		record_steps<MM>(c, A, nsteps - last0);
+		launch_copy(c, A->Ufin, c->b.U, n4);
+		launch_fill(c, a.Wn, n4);
+		if (K > 0) {
+			Bufs bf = c->b;
+			bf.P = A->Pst + (size_t)(nsteps - last0) * S * n4;
+			hipLaunchKernelGGL((k_seed<MM>), dim3(grid_of(K)), dim3(BLOCK), 0, c->st, bf, c->ph, a, A->seed_cells,
+			                   A->seed_w, K, A->seed_p);
+		}
+	};
+	if ((rc = adj_pass(c, c->adj_fwd, key, forward))) return rc;
 	std::vector<double> pk(K > 0 ? K : 1);
 	int lim = 0;
 	if (K > 0) HIPCHK(hipMemcpyAsync(pk.data(), A->seed_p, K * sizeof(double), hipMemcpyDeviceToHost, c->st));
@@ -817,47 +1175,70 @@ int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, do
 			c->adj_Jcase[cs] += t;
 		}
 		if (J) *J = Jv;
+		if (geo) {
+			geo->lny.assign(ne, 0.0);
+			for (int e = 0; e < ne; e++)
+				if (mask[e])
+					geo->lny[e] = pk[std::lower_bound(cells.begin(), cells.end(), c->hc1[e]) - cells.begin()];
+		}
 	}
-	// reverse sweep, segment by segment (the last one is already recorded)
+	// reverse sweep, segment by segment (the last one is already recorded),
+	// the boundary-parameter sums, dJ/dU0 as [4][nc] (into Ws), and the
+	// context left at the final state
 	const double wa2[2] = {0.0, 0.5}, wb2[2] = {1.0, 0.5};
 	const double wa3[3] = {0.0, 0.75, 1.0 / 3.0}, wb3[3] = {1.0, 0.25, 2.0 / 3.0};
-	double *P0 = c->b.P, *G0 = c->b.G;
-	for (int g = nck - 1; g >= 0; g--) {
-		int s0 = g * ck, len = std::min(nsteps, s0 + ck) - s0;
-		if (g != nck - 1) {
-			HIPCHK(hipMemcpyAsync(c->b.U, A->ckpt[g], bytes, hipMemcpyDeviceToDevice, c->st));
-			record_steps<MM>(c, A, len);
+	// every stage swaps the roles of Wn and Ws: after nsteps S stages dJ/dU0
+	// (as records) is in finW, and the transposed copy goes to finT
+	bool odd = ((long long)nsteps * S) % 2 == 1;
+	double* finW = odd ? a.Ws : a.Wn;
+	double* finT = odd ? a.Wn : a.Ws;
+	auto reverse = [&]() {
+		double *P0 = c->b.P, *G0 = c->b.G;
+		double *Wn = a.Wn, *Ws = a.Ws;
+		for (int g = nck - 1; g >= 0; g--) {
+			int s0 = g * ck, len = std::min(nsteps, s0 + ck) - s0;
+			if (g != nck - 1) {
+				launch_copy(c, c->b.U, A->ckpt[g], n4);
+				record_steps<MM>(c, A, len);
+			}
+			for (int t = len * S - 1; t >= 0; t--) {
+				int st = t % S;
+				c->b.P = A->Pst + (size_t)t * n4;
+				c->b.G = A->Gst + (size_t)t * n8;
+				double wa = S == 3 ? wa3[st] : wa2[st], wb = S == 3 ? wb3[st] : wb2[st];
+				int mode = st == 0 ? 2 : (st == S - 1 ? 0 : 1);
+				if (geo) launch_adj_stage<MM, true>(c, A, wa, wb, mode);
+				else launch_adj_stage<MM, false>(c, A, wa, wb, mode);
+				std::swap(a.Wn, a.Ws);
+			}
+			c->b.P = P0;
+			c->b.G = G0;
 		}
-		for (int t = len * S - 1; t >= 0; t--) {
-			int st = t % S;
-			c->b.P = A->Pst + (size_t)t * n4;
-			c->b.G = A->Gst + (size_t)t * n8;
-			double wa = S == 3 ? wa3[st] : wa2[st], wb = S == 3 ? wb3[st] : wb2[st];
-			int mode = st == 0 ? 2 : (st == S - 1 ? 0 : 1);
-			launch_adj_stage<MM>(c, A, wa, wb, mode);
-			std::swap(a.Wn, a.Ws);
-		}
-		c->b.P = P0;
-		c->b.G = G0;
-	}
-	if (c->nbc > 0 && a.nbedges > 0)
-		hipLaunchKernelGGL(k_bc_reduce, dim3(1), dim3(BLOCK), 0, c->st, c->b, a, c->nbc);
-	if (dJdU0) {
-		// per-cell records to the API's [4][nc], transposed on the device
-		// (into Ws, free now) and copied once
-		hipLaunchKernelGGL(k_records_to_arrays, dim3(grid_of(nc)), dim3(BLOCK), 0, c->st, (const double*)a.Wn, a.Ws,
+		if (c->nbc > 0 && a.nbedges > 0)
+			hipLaunchKernelGGL(k_bc_reduce, dim3(1), dim3(BLOCK), 0, c->st, c->b, a, c->nbc);
+		// per-cell records to the API's [4][nc], transposed on the device into
+		// the other W buffer (free now)
+		hipLaunchKernelGGL(k_records_to_arrays, dim3(grid_of(nc)), dim3(BLOCK), 0, c->st, (const double*)finW, finT,
 		                   nc);
-		HIPCHK(hipMemcpyAsync(dJdU0, a.Ws, bytes, hipMemcpyDeviceToHost, c->st));
-	}
+		launch_copy(c, c->b.U, A->Ufin, n4);
+		launch_prim<MM>(c);
+		a.Wn = Wn;  // the host's view back to the start (a replay swaps nothing on the host)
+		a.Ws = Ws;
+	};
+	if ((rc = adj_pass(c, c->adj_rev, key, reverse))) return rc;
+	if (dJdU0) HIPCHK(hipMemcpyAsync(dJdU0, finT, bytes, hipMemcpyDeviceToHost, c->st));
 	if (dJdbc && c->nbc > 0) {
 		if (a.nbedges > 0)
 			HIPCHK(hipMemcpyAsync(dJdbc, a.bcbar, 4 * (size_t)c->nbc * sizeof(double), hipMemcpyDeviceToHost, c->st));
 		else
 			std::fill(dJdbc, dJdbc + 4 * (size_t)c->nbc, 0.0);
 	}
-	// leave the context at the final state
-	HIPCHK(hipMemcpyAsync(c->b.U, A->Ufin, bytes, hipMemcpyDeviceToDevice, c->st));
-	launch_prim<MM>(c);
+	if (geo) {
+		geo->Ge.resize(GE_N * (size_t)ne);
+		geo->Gc.resize(GC_N * (size_t)nc);
+		HIPCHK(hipMemcpyAsync(geo->Ge.data(), a.Ge, geo->Ge.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+		HIPCHK(hipMemcpyAsync(geo->Gc.data(), a.Gc, geo->Gc.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+	}
 	HIPCHK(hipStreamSynchronize(c->st));
 	return check_launch("adjoint reverse sweep");
 }
@@ -868,15 +1249,42 @@ extern "C" int cfd2d_adjoint(cfd2d_ctx* c, int nsteps, int ckpt_every, const int
                              double* dJdU0, double* dJdbc)
 {
 	if (!c || nsteps < 1 || !edge_mask) return fail(CFD2D_ERR_ARG, "cfd2d_adjoint: bad arguments");
-	if ((c->p.flux != CFD2D_FLUX_LAX && c->p.flux != CFD2D_FLUX_HLLC) || c->p.limiter != CFD2D_LIMITER_NONE ||
-	    c->p.viscous || c->p.steady)
-		return fail(CFD2D_ERR_CONFIG, "adjoint: needs FLUX LAX or HLLC, no limiter, inviscid, unsteady");
+	if (c->p.flux == CFD2D_FLUX_ROE || c->p.limiter != CFD2D_LIMITER_NONE || c->p.viscous || c->p.steady)
+		return fail(CFD2D_ERR_CONFIG, "adjoint: needs FLUX GODUNOV, LAX or HLLC, no limiter, inviscid, unsteady");
 	bool timing = c->timing;
 	c->timing = false;  // the adjoint's launches are not part of the step timing
 	int rc = c->mm ? adjoint_run<true>(c, nsteps, ckpt_every, edge_mask, J, dJdU0, dJdbc)
 	               : adjoint_run<false>(c, nsteps, ckpt_every, edge_mask, J, dJdU0, dJdbc);
 	c->timing = timing;
 	return rc;
+}
+
+extern "C" int cfd2d_adjoint_geometry(cfd2d_ctx* c, const cfd2d_mesh* m, int nsteps, int ckpt_every,
+                                      const int* edge_mask, double* J, double* dJdU0, double* dJdbc, double* dJdxy)
+{
+	if (!c || !m || nsteps < 1 || !edge_mask || !dJdxy) return fail(CFD2D_ERR_ARG, "cfd2d_adjoint_geometry: bad arguments");
+	if (m->m.nc != c->nc || m->m.ne != c->ne)
+		return fail(CFD2D_ERR_ARG, "cfd2d_adjoint_geometry: the mesh is not the context's (cell or edge count)");
+	if (c->p.flux == CFD2D_FLUX_ROE || c->p.limiter != CFD2D_LIMITER_NONE || c->p.viscous || c->p.steady)
+		return fail(CFD2D_ERR_CONFIG, "adjoint: needs FLUX GODUNOV, LAX or HLLC, no limiter, inviscid, unsteady");
+	bool timing = c->timing;
+	c->timing = false;
+	GeoOut geo;
+	int rc = c->mm ? adjoint_run<true>(c, nsteps, ckpt_every, edge_mask, J, dJdU0, dJdbc, &geo)
+	               : adjoint_run<false>(c, nsteps, ckpt_every, edge_mask, J, dJdU0, dJdbc, &geo);
+	c->timing = timing;
+	if (rc) return rc;
+	cfd2d::geometry_vjp(m->m, c->p.gradient == CFD2D_GRAD_LSQ, geo.Ge.data(), geo.Gc.data(), geo.lny.data(), dJdxy);
+	return CFD2D_OK;
+}
+
+extern "C" int cfd2d_adjoint_graph(cfd2d_ctx* c, int enable)
+{
+	if (!c) return fail(CFD2D_ERR_ARG, "null");
+	c->adj_use_graph = enable != 0;
+	drop_adj_graph(c->adj_fwd);
+	drop_adj_graph(c->adj_rev);
+	return CFD2D_OK;
 }
 
 extern "C" int cfd2d_adjoint_objectives(cfd2d_ctx* c, double* J_case)

@@ -130,6 +130,14 @@ int cfd2d_mesh_renumber(const cfd2d_mesh* m, int order, int seed, int* cell_perm
 	return CFD2D_OK;
 }
 
+int cfd2d_mesh_geometry_vjp(const cfd2d_mesh* m, int lsq, const double* Ge, const double* Gc, const double* lny,
+                            double* dxy)
+{
+	if (!m || !dxy) return fail(CFD2D_ERR_ARG, "cfd2d_mesh_geometry_vjp: bad arguments");
+	cfd2d::geometry_vjp(m->m, lsq != 0, Ge, Gc, lny, dxy);
+	return CFD2D_OK;
+}
+
 int cfd2d_mesh_concat(int n, const cfd2d_mesh* const* parts, cfd2d_mesh** out)
 {
 	if (n <= 0 || !parts || !out) return fail(CFD2D_ERR_ARG, "cfd2d_mesh_concat: bad arguments");

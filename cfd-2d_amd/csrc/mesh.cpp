@@ -703,3 +703,181 @@ int concat_meshes(const std::vector<const Mesh*>& parts, Mesh& o, std::string& e
 }
 
 }  // namespace cfd2d
+
+// ---------------------------------------------------------------- geometry adjoint
+
+namespace cfd2d {
+
+/*
+ * Extension: the reverse of the geometry above (centroids, edge normals,
+ * lengths and Gauss points, Heron areas, the explicit method's Green-Gauss
+ * weights n l and least-squares weights A^-1 d_k) -- given the adjoints of
+ * the quantities the device kernels read, the adjoint of every node
+ * coordinate.  Inputs (any pointer may be NULL):
+ *   Ge [13][ne]  per edge: normal adjoint (x, y) at Gauss points 1 and 2
+ *                (summed), the adjoint of side s's offset to Gauss point g
+ *                (x, y) at 4 + 4 s + 2 g, the length adjoint at 12
+ *   Gc [7][nc]   per cell: area adjoint, then per face k the adjoint of its
+ *                Green-Gauss weight sign * n l (x, y) or LSQ weight (x, y)
+ *   lny [ne]     adjoint of l * n_y (the wall-force objective's edge term)
+ * The normal's orientation (out of c1) and the periodic shifts are held
+ * fixed: derivatives are those of the smooth pieces the builder evaluates.
+ */
+int geometry_vjp(const Mesh& m, bool lsq, const double* Ge, const double* Gc, const double* lny, double* dxy)
+{
+	const int nc = m.nc, ne = m.ne, nn = m.nn;
+	const size_t NE = (size_t)ne, NC = (size_t)nc;
+	std::vector<double> nb(2 * NE, 0.0), lb(NE, 0.0), gpb(4 * NE, 0.0), midb(2 * NE, 0.0), cb(2 * NC, 0.0);
+	std::vector<double> Sb(NC, 0.0);
+	for (int e = 0; e < ne; e++) {
+		if (Ge) {
+			nb[2 * e] = Ge[0 * NE + e] + Ge[2 * NE + e];
+			nb[2 * e + 1] = Ge[1 * NE + e] + Ge[3 * NE + e];
+			lb[e] = Ge[12 * NE + e];
+			for (int s = 0; s < 2; s++) {
+				int cell = s ? m.c2[e] : m.c1[e];
+				for (int g = 0; g < 2; g++) {
+					double bx = Ge[(size_t)(4 + 4 * s + 2 * g) * NE + e], by = Ge[(size_t)(5 + 4 * s + 2 * g) * NE + e];
+					if (cell < 0) continue;
+					gpb[4 * e + 2 * g] += bx;  // offset = Gauss point - (centroid + shift)
+					gpb[4 * e + 2 * g + 1] += by;
+					cb[2 * cell] -= bx;
+					cb[2 * cell + 1] -= by;
+				}
+			}
+		}
+		if (lny) {
+			lb[e] += lny[e] * m.ny[e];
+			nb[2 * e + 1] += lny[e] * m.l[e];
+		}
+	}
+	if (Gc) {
+		for (int i = 0; i < nc; i++) {
+			Sb[i] += Gc[i];
+			if (lsq) {
+				// w_k = A^-1 d_k, A = sum d d^T: d-bar_k = u_k + (A-bar + A-bar^T) d_k with
+				// u_k = A^-1 w-bar_k and A-bar = -sum_k u_k w_k^T
+				double d[3][2], mid[3][2];
+				int es[3], side[3];
+				for (int k = 0; k < 3; k++) {
+					int e = m.cedges[3 * (size_t)i + k];
+					es[k] = e;
+					side[k] = m.c1[e] == i ? 0 : 1;
+					int c1 = m.c1[e], c2 = m.c2[e];
+					mid[k][0] = m.gp[6 * (size_t)e];
+					mid[k][1] = m.gp[6 * (size_t)e + 1];
+					if (c2 < 0) {
+						double dn = (mid[k][0] - m.cx[i]) * m.nx[e] + (mid[k][1] - m.cy[i]) * m.ny[e];
+						d[k][0] = 2.0 * dn * m.nx[e];
+						d[k][1] = 2.0 * dn * m.ny[e];
+					} else if (i == c1) {
+						d[k][0] = (m.cx[c2] + m.shx[e]) - m.cx[c1];
+						d[k][1] = (m.cy[c2] + m.shy[e]) - m.cy[c1];
+					} else {
+						d[k][0] = m.cx[c1] - (m.cx[c2] + m.shx[e]);
+						d[k][1] = m.cy[c1] - (m.cy[c2] + m.shy[e]);
+					}
+				}
+				double a11 = 0, a12 = 0, a22 = 0;
+				for (int k = 0; k < 3; k++) {
+					a11 += d[k][0] * d[k][0];
+					a12 += d[k][0] * d[k][1];
+					a22 += d[k][1] * d[k][1];
+				}
+				double det = a11 * a22 - a12 * a12;
+				double w[3][2], u[3][2];
+				double B11 = 0, B12 = 0, B21 = 0, B22 = 0;  // A-bar
+				for (int k = 0; k < 3; k++) {
+					w[k][0] = (a22 * d[k][0] - a12 * d[k][1]) / det;
+					w[k][1] = (a11 * d[k][1] - a12 * d[k][0]) / det;
+					double wbx = Gc[(size_t)(1 + 2 * k) * NC + i], wby = Gc[(size_t)(2 + 2 * k) * NC + i];
+					u[k][0] = (a22 * wbx - a12 * wby) / det;
+					u[k][1] = (a11 * wby - a12 * wbx) / det;
+					B11 -= u[k][0] * w[k][0];
+					B12 -= u[k][0] * w[k][1];
+					B21 -= u[k][1] * w[k][0];
+					B22 -= u[k][1] * w[k][1];
+				}
+				for (int k = 0; k < 3; k++) {
+					double dbx = u[k][0] + (2.0 * B11) * d[k][0] + (B12 + B21) * d[k][1];
+					double dby = u[k][1] + (B12 + B21) * d[k][0] + (2.0 * B22) * d[k][1];
+					int e = es[k];
+					int c1 = m.c1[e], c2 = m.c2[e];
+					if (c2 < 0) {
+						// d = 2 dn n, dn = (mid - c) . n
+						double nx = m.nx[e], ny = m.ny[e];
+						double dn = (mid[k][0] - m.cx[i]) * nx + (mid[k][1] - m.cy[i]) * ny;
+						double dnb = 2.0 * (nx * dbx + ny * dby);
+						nb[2 * e] += 2.0 * dn * dbx + dnb * (mid[k][0] - m.cx[i]);
+						nb[2 * e + 1] += 2.0 * dn * dby + dnb * (mid[k][1] - m.cy[i]);
+						midb[2 * e] += dnb * nx;
+						midb[2 * e + 1] += dnb * ny;
+						cb[2 * i] -= dnb * nx;
+						cb[2 * i + 1] -= dnb * ny;
+					} else {
+						int plus = side[k] == 0 ? c2 : c1, minus = side[k] == 0 ? c1 : c2;
+						cb[2 * plus] += dbx;
+						cb[2 * plus + 1] += dby;
+						cb[2 * minus] -= dbx;
+						cb[2 * minus + 1] -= dby;
+					}
+				}
+			} else {
+				for (int k = 0; k < 3; k++) {
+					int e = m.cedges[3 * (size_t)i + k];
+					double sg = m.c1[e] == i ? 1.0 : -1.0;  // weight = sg n l
+					double wbx = Gc[(size_t)(1 + 2 * k) * NC + i], wby = Gc[(size_t)(2 + 2 * k) * NC + i];
+					nb[2 * e] += sg * m.l[e] * wbx;
+					nb[2 * e + 1] += sg * m.l[e] * wby;
+					lb[e] += sg * (m.nx[e] * wbx + m.ny[e] * wby);
+				}
+			}
+		}
+	}
+	// Heron: S = sqrt(Q), Q = p (p - a)(p - b)(p - c), p = (a + b + c)/2, over the cell's edges
+	for (int i = 0; i < nc; i++) {
+		if (Sb[i] == 0.0) continue;
+		int e[3] = {m.cedges[3 * (size_t)i], m.cedges[3 * (size_t)i + 1], m.cedges[3 * (size_t)i + 2]};
+		double x[3] = {m.l[e[0]], m.l[e[1]], m.l[e[2]]};
+		double p = (x[0] + x[1] + x[2]) / 2.0;
+		double f[3] = {p - x[0], p - x[1], p - x[2]};
+		double S = m.S[i];
+		for (int k = 0; k < 3; k++) {
+			// dQ/dx_k = dQ/dp / 2 - p (product of the other two factors)
+			double dp = f[0] * f[1] * f[2] + p * (f[1] * f[2] + f[0] * f[2] + f[0] * f[1]);
+			double dq = 0.5 * dp - p * (f[(k + 1) % 3] * f[(k + 2) % 3]);
+			lb[e[k]] += Sb[i] * dq / (2.0 * S);
+		}
+	}
+	std::fill(dxy, dxy + 2 * (size_t)nn, 0.0);
+	for (int i = 0; i < nc; i++)
+		for (int q = 0; q < 3; q++) {
+			int v = m.cnodes[3 * (size_t)i + q];
+			dxy[2 * (size_t)v] += cb[2 * i] / 3.0;
+			dxy[2 * (size_t)v + 1] += cb[2 * i + 1] / 3.0;
+		}
+	const double sq3 = 1.0 / sqrt(3.0);
+	for (int e = 0; e < ne; e++) {
+		int a = m.n1[e], b = m.n2[e];
+		double dx = m.xy[2 * (size_t)b] - m.xy[2 * (size_t)a], dy = m.xy[2 * (size_t)b + 1] - m.xy[2 * (size_t)a + 1];
+		double l = m.l[e];
+		// n = sigma (dy, -dx) / l; sigma fixed by the orientation out of c1
+		double hx = dy / l, hy = -dx / l;
+		double sigma = hx * m.nx[e] + hy * m.ny[e] >= 0.0 ? 1.0 : -1.0;
+		double nbx = nb[2 * e], nby = nb[2 * e + 1];
+		double proj = hx * nbx + hy * nby;
+		double vbx = sigma * (nbx - proj * hx) / l, vby = sigma * (nby - proj * hy) / l;
+		double ddx = -vby + lb[e] * dx / l, ddy = vbx + lb[e] * dy / l;
+		// Gauss points mid -/+ (1/sqrt 3) d / 2, mid = (P1 + P2) / 2
+		double mbx = midb[2 * e] + gpb[4 * e] + gpb[4 * e + 2], mby = midb[2 * e + 1] + gpb[4 * e + 1] + gpb[4 * e + 3];
+		ddx += 0.5 * sq3 * (gpb[4 * e + 2] - gpb[4 * e]);
+		ddy += 0.5 * sq3 * (gpb[4 * e + 3] - gpb[4 * e + 1]);
+		dxy[2 * (size_t)a] += 0.5 * mbx - ddx;
+		dxy[2 * (size_t)a + 1] += 0.5 * mby - ddy;
+		dxy[2 * (size_t)b] += 0.5 * mbx + ddx;
+		dxy[2 * (size_t)b + 1] += 0.5 * mby + ddy;
+	}
+	return MESH_OK;
+}
+
+}  // namespace cfd2d

@@ -67,5 +67,8 @@ unsigned long long hilbert_index(unsigned long long x, unsigned long long y, int
 // pairings carried over); one case only.
 int renumber_mesh(const Mesh& in, int order, unsigned long long seed, std::vector<int>& perm, Mesh& out,
                   std::string& err);
+// adjoints of the geometry the kernels read -> adjoints of the node
+// coordinates (dxy: 2 per node); layouts at the definition
+int geometry_vjp(const Mesh& m, bool lsq, const double* Ge, const double* Gc, const double* lny, double* dxy);
 
 }  // namespace cfd2d

@@ -99,13 +99,14 @@ __device__ __forceinline__ Prim ghost_state(int type, const double* par, double 
 
 // rim_orig (src/global.cpp:232-405): exact Riemann solver, Newton on p*
 // from the acoustic guess with the P >= 1e-5 clamp, stopping when
-// |DU - F1 - F2| <= 1e-5 is seen before the update; powers as
-// exp(log(x) k).  max_iter > 0 caps the loop and reports it in hit; iters
-// (if given) receives the Newton iterations taken (0 on the vacuum branch).
-// Divisions by loop invariants (1/RB, 1/RE, 1/PB, 1/PE, the gamma constants)
-// are formed once and multiplied; the shock branches share one reciprocal of
-// Z * PK for F and F'; the post-loop shock state uses 1/(R (U - D)) for both
-// the velocity and the energy.  These change the rounding of single
+// |DU - F1 - F2| <= 1e-5 is seen before the update; the rarefaction/shock
+// branch of each side on PK > P as in the reference.  max_iter > 0 caps the
+// loop and reports it in hit; iters (if given) receives the Newton
+// iterations taken (0 on the vacuum branch).  Restructured for the device:
+// each side's branch is one root and one reciprocal (side_newton,
+// side_star), so lanes that take different branches do not run both; 1/rho
+// and 1/p of a side come from one division; divisions by constants are
+// products with their reciprocals.  These change the rounding of single
 // operations only (results agree with the reference to round-off, not bit
 // for bit); the FP64 division is the costliest operation of this solver.
 // Outputs the state sampled at x/t = 0: density RI, internal energy EI,
@@ -113,7 +114,8 @@ __device__ __forceinline__ Prim ghost_state(int type, const double* par, double 
 // The gamma-dependent constants of rim_orig (src/global.cpp:236-250),
 // formed once on the host with the same expressions instead of per call.
 struct RimConst {
-	double GAM, AGAM, DGAM, GGAM, HGAM, FGAM, OGAM, QGAM, PGAM, RGAM, SGAM, TGAM, DGAM1, IAGAM, DGGAM, ISGAM, IRGAM;
+	double GAM, AGAM, DGAM, GGAM, HGAM, FGAM, OGAM, QGAM, PGAM, RGAM, SGAM, TGAM, DGAM1, IAGAM, DGGAM, ISGAM, IRGAM,
+	    IDGAM1;
 	int gamma14;  // OGAM = 1/7 and 1/AGAM = 5/2 to rounding (GAM = 1.4, the reference's value)
 };
 
@@ -137,41 +139,9 @@ __host__ __device__ inline RimConst rim_const(double GAM)
 	k.DGGAM = k.DGAM * k.GGAM;
 	k.ISGAM = 1.0 / k.SGAM;
 	k.IRGAM = 1.0 / k.RGAM;
+	k.IDGAM1 = 1.0 / k.DGAM1;
 	k.gamma14 = fabs(7.0 * k.OGAM - 1.0) < 1e-14 && fabs(k.IAGAM - 2.5) < 1e-14;
 	return k;
-}
-
-// x^OGAM, the rarefaction power of rim_orig (computed there as
-// exp(log(x) * OGAM)).  For GAM = 1.4, OGAM is 1/7: a hardware single-
-// precision estimate of x^(1/7) refined by two Newton steps on y^7 = x,
-// y <- y + (x / y^6 - y) / 7, which is quadratically convergent from the
-// ~1e-7 estimate to within an ulp or two (about 40 instructions instead of
-// the ~160 of the double-precision log and exp).  1/y^6 comes from the
-// hardware reciprocal with Newton-Raphson refinement (r <- r (2 - y^6 r),
-// once in the first step, whose error the root iteration removes, twice in
-// the second) instead of a full IEEE division.  Other gammas, and x outside
-// [1e-30, 1e30], take exp(log(x) * OGAM).
-__device__ __forceinline__ double pow_ogam(double x, const RimConst& K)
-{
-	if (K.gamma14 && x > 1e-30 && x < 1e30) {
-		float xf = (float)x;
-		double y = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf(xf) * (1.0f / 7.0f));
-		{
-			double y2 = y * y;
-			double y6 = y2 * y2 * y2;
-			double r = __builtin_amdgcn_rcp(y6);
-			r = r * (2.0 - y6 * r);
-			y = y + (x * r - y) * (1.0 / 7.0);
-			y2 = y * y;
-			y6 = y2 * y2 * y2;
-			r = __builtin_amdgcn_rcp(y6);
-			r = r * (2.0 - y6 * r);
-			r = r * (2.0 - y6 * r);
-			y = y + (x * r - y) * (1.0 / 7.0);
-		}
-		return y;
-	}
-	return exp(log(x) * K.OGAM);
 }
 
 // x^(1/AGAM) of the fan density (rim_orig: exp(log(x) * (1/AGAM))); x^(5/2)
@@ -182,24 +152,116 @@ __device__ __forceinline__ double pow_iagam(double x, const RimConst& K)
 	return exp(log(x) * K.IAGAM);
 }
 
+// The root rim_orig takes on one side: t^OGAM on a rarefaction (t = P/PK;
+// x^(1/7) for GAM = 1.4) and sqrt(t) on a shock.  For GAM = 1.4 both are one
+// computation, so the lanes of a wave that take different sides of the
+// reference's branch still run the same instructions: a single-precision
+// seed t^(1/k) (k = 7 or 2) refined by two Newton steps on y^k = t with
+// refined hardware reciprocals of y^(k-1) (quadratic convergence from the
+// ~1e-7 seed to an ulp or two; one refinement of the reciprocal in the first
+// step, whose error the second removes, two in the second).  Other gammas and t
+// outside [1e-30, 1e30] (negative t included) take exp(log(t) OGAM) and the
+// square root.
+__device__ __forceinline__ double side_root(double t, bool rare, const RimConst& K)
+{
+	if (K.gamma14 && t > 1e-30 && t < 1e30) {
+		float tf = (float)t;
+		double y = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf(tf) * (rare ? (1.0f / 7.0f) : 0.5f));
+		double ik = rare ? (1.0 / 7.0) : 0.5;
+		double y2 = y * y;
+		double pk = rare ? y2 * y2 * y2 : y;  // y^(k-1)
+		double r = __builtin_amdgcn_rcp(pk);
+		r = r * (2.0 - pk * r);
+		y = y + (t * r - y) * ik;
+		y2 = y * y;
+		pk = rare ? y2 * y2 * y2 : y;
+		r = __builtin_amdgcn_rcp(pk);
+		r = r * (2.0 - pk * r);
+		r = r * (2.0 - pk * r);
+		return y + (t * r - y) * ik;
+	}
+	return rare ? exp(log(t) * K.OGAM) : sqrt_dev(t);
+}
+
+// one side of the Riemann problem: density, pressure, normal velocity, sound
+// speed, rho c, 1/rho, 1/p
+struct RimSide {
+	double R, P, U, C, RC, IR, IP;
+};
+
+// rim_orig's Newton function of one side at P (F1, FS1 or F2, FS2 of
+// src/global.cpp:278-303): rarefaction (PK > P) F = DGAM (Z - C),
+// F' = Z / (GAM P) with Z = C (P/PK)^OGAM; shock F = (P - PK) / Z,
+// F' = (QGAM P/PK + FGAM) / (RGAM Z PKK) with Z = R C sqrt(PKK),
+// PKK = PGAM P/PK + OGAM.  The two sides of the branch share one root and
+// one reciprocal (of GAM P or Z PKK).
+__device__ __forceinline__ void side_newton(const RimSide& s, double P, const RimConst& K, double& F, double& FS)
+{
+	double x = P * s.IP;
+	bool rare = s.P > P;
+	double t = rare ? x : K.PGAM * x + K.OGAM;
+	double y = side_root(t, rare, K);
+	double r = 1.0 / (rare ? K.GAM * P : s.RC * y * t);
+	F = rare ? K.DGAM * (s.C * y - s.C) : (P - s.P) * t * r;
+	FS = rare ? s.C * y * r : (K.QGAM * x + K.FGAM) * r * K.IRGAM;
+}
+
+// The star state (Rs, Es, Us) of one side at the converged P and its wave's
+// head and tail speeds (src/global.cpp:310-349); sg = -1 for the left side
+// (B), +1 for the right (E).  Rarefaction: Z = C (P/PK)^OGAM, E = Z^2/SGAM,
+// U = UK + sg DGAM (Z - C), R = P / (AGAM E), head UK + sg C, tail U + sg Z.
+// Shock: w = sqrt((TGAM P + HGAM PK)/RK), D = UK + sg w, M = RK (UK - D),
+// Q = PK - P + M (UK - D), R = M^2/Q, U = D + Q/M, E = P Q / (M^2 AGAM),
+// head = tail = D.  One reciprocal per side: 1/(AGAM E) or 1/(Q M).
+__device__ __forceinline__ void side_star(const RimSide& s, double P, double sg, const RimConst& K, double& Rs,
+                                          double& Es, double& Us, double& head, double& tail)
+{
+	bool rare = s.P > P;
+	double t = rare ? P * s.IP : (K.TGAM * P + K.HGAM * s.P) * s.IR;
+	double y = side_root(t, rare, K);
+	double Z = s.C * y;
+	double Er = Z * Z * K.ISGAM;
+	double D = s.U + sg * y;
+	double M = -sg * s.R * y;
+	double Q = s.P - P + M * (-sg * y);
+	double r = 1.0 / (rare ? K.AGAM * Er : Q * M);
+	if (rare) {
+		Es = Er;
+		Us = s.U + sg * K.DGAM * (Z - s.C);
+		Rs = P * r;
+		head = s.U + sg * s.C;
+		tail = Us + sg * Z;
+	} else {
+		double IM = Q * r;  // 1/M
+		Rs = M * M * M * r;
+		Us = D + Q * IM;
+		Es = P * Q * IM * IM * K.IAGAM;
+		head = D;
+		tail = D;
+	}
+}
+
 __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& UI, double& VI, double RB,
                                     double PB, double UB, double VB, double RE, double PE, double UE, double VE,
                                     const RimConst& K, int max_iter, int& hit, int* iters = nullptr)
 {
-	const double GAM = K.GAM, AGAM = K.AGAM, DGAM = K.DGAM, HGAM = K.HGAM, FGAM = K.FGAM, OGAM = K.OGAM;
-	const double QGAM = K.QGAM, PGAM = K.PGAM, SGAM = K.SGAM, TGAM = K.TGAM;
+	const double AGAM = K.AGAM;
 	const double eps = 1.0e-5;
 
 	double US = 0.0, UF = 0.0;
 	double RF, RS, EF, ES, SBL, SFL, SSL, SEL;
-	const double IRB = 1.0 / RB, IRE = 1.0 / RE;
-	double CB = sqrt_dev(GAM * PB * IRB);
-	double CE = sqrt_dev(GAM * PE * IRE);
+	// 1/rho and 1/p of a side from one division
+	const double iB = 1.0 / (RB * PB), iE = 1.0 / (RE * PE);
+	RimSide B, E;
+	B.R = RB; B.P = PB; B.U = UB; B.IR = PB * iB; B.IP = RB * iB;
+	E.R = RE; E.P = PE; E.U = UE; E.IR = PE * iE; E.IP = RE * iE;
+	B.C = sqrt_dev(K.GAM * PB * B.IR);
+	E.C = sqrt_dev(K.GAM * PE * E.IR);
+	B.RC = RB * B.C;
+	E.RC = RE * E.C;
+	double CB = B.C, CE = E.C;
 	double EB = CB * CB * K.ISGAM;
 	double EE = CE * CE * K.ISGAM;
-	const double IPB = 1.0 / PB, IPE = 1.0 / PE;
-	double RCB = RB * CB;
-	double RCE = RE * CE;
 	double DU = UB - UE;
 	hit = 0;
 	if (iters) *iters = 0;
@@ -213,35 +275,13 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 		SSL = UE - 2.0 * CE / AGAM;
 		SEL = UE + CE;
 	} else {
-		double P = (PB * RCE + PE * RCB + DU * RCB * RCE) / (RCB + RCE);
+		double P = (PB * E.RC + PE * B.RC + DU * B.RC * E.RC) / (B.RC + E.RC);
 		int it = 0;
 		for (;;) {
 			if (P < eps) P = eps;
 			double F1, FS1, F2, FS2;
-			double PPB = P * IPB;
-			if (PB > P) {
-				double ZFB = CB * pow_ogam(PPB, K);
-				F1 = DGAM * (ZFB - CB);
-				FS1 = ZFB / (GAM * P);
-			} else {
-				double PKB = PGAM * PPB + OGAM;
-				double ZNB = RCB * sqrt_dev(PKB);
-				double IZ = 1.0 / (ZNB * PKB);
-				F1 = (P - PB) * PKB * IZ;
-				FS1 = (QGAM * PPB + FGAM) * IZ * K.IRGAM;
-			}
-			double PPE = P * IPE;
-			if (PE > P) {
-				double ZFE = CE * pow_ogam(PPE, K);
-				F2 = DGAM * (ZFE - CE);
-				FS2 = ZFE / (GAM * P);
-			} else {
-				double PKE = PGAM * PPE + OGAM;
-				double ZNE = RCE * sqrt_dev(PKE);
-				double IZ = 1.0 / (ZNE * PKE);
-				F2 = (P - PE) * PKE * IZ;
-				FS2 = (QGAM * PPE + FGAM) * IZ * K.IRGAM;
-			}
+			side_newton(B, P, K, F1, FS1);
+			side_newton(E, P, K, F2, FS2);
 			double res = DU - F1 - F2;
 			P = P + res / (FS1 + FS2);
 			++it;
@@ -252,46 +292,8 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 			}
 		}
 		if (iters) *iters = it;
-		// the reference forms both rarefaction heads before branching; each is
-		// only read on its rarefaction side, so it is formed there
-		if (PB > P) {
-			double ZFB = CB * pow_ogam(P * IPB, K);
-			EF = ZFB * ZFB * K.ISGAM;
-			UF = UB + DGAM * (CB - ZFB);
-			RF = P / (AGAM * EF);
-			SBL = UB - CB;
-			SFL = UF - ZFB;
-		} else {
-			double D = UB - sqrt_dev((TGAM * P + HGAM * PB) * IRB);
-			double UBD = UB - D;
-			double RUBD = RB * UBD;
-			double Q = PB - P + RUBD * UBD;
-			double IR = 1.0 / RUBD;
-			RF = RUBD * RUBD / Q;
-			UF = D + Q * IR;
-			EF = P * Q * IR * IR * K.IAGAM;
-			SBL = D;
-			SFL = D;
-		}
-		if (PE > P) {
-			double ZFE = CE * pow_ogam(P * IPE, K);
-			ES = ZFE * ZFE * K.ISGAM;
-			US = UE - DGAM * (CE - ZFE);
-			RS = P / (AGAM * ES);
-			SSL = US + ZFE;
-			SEL = UE + CE;
-		} else {
-			double D = UE + sqrt_dev((TGAM * P + HGAM * PE) * IRE);
-			double UED = UE - D;
-			double RUED = RE * UED;
-			double Q = PE - P + RUED * UED;
-			double IR = 1.0 / RUED;
-			RS = RUED * RUED / Q;
-			US = D + Q * IR;
-			ES = P * Q * IR * IR * K.IAGAM;
-			SEL = D;
-			SSL = D;
-		}
+		side_star(B, P, -1.0, K, RF, EF, UF, SBL, SFL);
+		side_star(E, P, 1.0, K, RS, ES, US, SEL, SSL);
 	}
 	if (SEL <= 0.0) {
 		RI = RE; EI = EE; UI = UE; VI = VE;
@@ -304,14 +306,14 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 			RI = RS; EI = ES; UI = US; VI = VE;
 		}
 	} else if (SFL > 0.0) {
-		UI = (UB + K.DGGAM * sqrt_dev(EB)) / K.DGAM1;
+		UI = (UB + K.DGGAM * sqrt_dev(EB)) * K.IDGAM1;
 		VI = VB;
-		EI = (UI * UI) / SGAM;
+		EI = (UI * UI) * K.ISGAM;
 		RI = RB * pow_iagam(EI / EB, K);
 	} else {
-		UI = (UE - K.DGGAM * sqrt_dev(EE)) / K.DGAM1;
+		UI = (UE - K.DGGAM * sqrt_dev(EE)) * K.IDGAM1;
 		VI = VE;
-		EI = (UI * UI) / SGAM;
+		EI = (UI * UI) * K.ISGAM;
 		RI = RE * pow_iagam(EI / EE, K);
 	}
 	PI = AGAM * EI * RI;

@@ -38,6 +38,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -423,10 +424,34 @@ __device__ __forceinline__ void store_edge_flux(double* F, int e, int gp, const 
 // launch disappear; the extra arithmetic runs inside a pass that is bound by
 // FP64 issue anyway, beside its memory traffic.  Results are the same bits as
 // with GR = 0.
-template <bool MM, int FLUX, bool VISC, int GR>
-__global__ __launch_bounds__(BLOCK) void k_edge_flux(Bufs b, Phys ph)
+//
+// SHAPE picks the launch shape the autotuner may try (EdgeShape): threads per
+// workgroup and the waves per SIMD the register allocation must allow.
+template <int SHAPE>
+struct EdgeShape;
+template <>
+struct EdgeShape<0> {  // 256 threads, registers as the compiler allocates them
+	static constexpr int NT = 256, WPE = 1;
+};
+template <>
+struct EdgeShape<1> {  // 256 threads, at most 96 VGPRs: five waves per SIMD
+	static constexpr int NT = 256, WPE = 5;
+};
+template <>
+struct EdgeShape<2> {  // 128 threads
+	static constexpr int NT = 128, WPE = 1;
+};
+template <>
+struct EdgeShape<3> {  // 512 threads
+	static constexpr int NT = 512, WPE = 1;
+};
+constexpr int EDGE_SHAPES = 4;
+
+template <bool MM, int FLUX, bool VISC, int GR, int SHAPE = 0>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, EdgeShape<SHAPE>::NT),
+                          amdgpu_waves_per_eu(EdgeShape<SHAPE>::WPE))) void k_edge_flux(Bufs b, Phys ph)
 {
-	int t = xcd_block() * BLOCK + threadIdx.x;
+	int t = xcd_block() * EdgeShape<SHAPE>::NT + threadIdx.x;
 	int e = t >> 1, gp = t & 1;
 	if (e >= b.ne) return;  // both lanes of a pair leave together
 	int2 ce = edge_cells(b, e);
@@ -775,6 +800,7 @@ struct cfd2d_ctx {
 	int fuse_mode = -1;          // 1: the edge pass forms the gradients, 0: a separate
 	                             // gradient pass, -1: 1 without a limiter, 0 with one
 	bool prefer_graph = false;   // cfd2d_step replays a recorded step (chosen by cfd2d_autotune)
+	int edge_shape = 0;          // launch shape of the edge pass (EdgeShape; chosen by cfd2d_autotune)
 	int stream_mode = -1;        // update pass reads/writes the state non-temporally: 1 yes,
 	                             // 0 no, -1 when the arrays exceed twice the Infinity Cache
 	bool stream_auto = false;    // that size test, made once the arrays are allocated
@@ -785,9 +811,21 @@ struct cfd2d_ctx {
 	int ev_used = 0;
 	double tot_ms[CFD2D_K_COUNT] = {0};
 	long long launches[CFD2D_K_COUNT] = {0};
-	// graph of one step
-	hipGraph_t graph = nullptr;
-	hipGraphExec_t graph_exec = nullptr;
+	// recorded runs of steps (cfd2d_step_graph), by step count
+	struct StepGraph {
+		int steps;
+		hipGraph_t g;
+		hipGraphExec_t x;
+	};
+	std::vector<StepGraph> graphs;
+	// the adjoint's recorded forward and reverse passes (adjoint.inl)
+	struct AdjGraph {
+		std::vector<long long> key;
+		hipGraph_t g = nullptr;
+		hipGraphExec_t x = nullptr;
+	};
+	AdjGraph adj_fwd, adj_rev;
+	bool adj_use_graph = true;
 	bool capturing = false;
 	std::vector<void*> allocs;
 	// adjoint topology, built on the first cfd2d_adjoint call
@@ -908,14 +946,40 @@ void launch_gradient(cfd2d_ctx* c)
 	else LAUNCH(c, CFD2D_K_GRADIENT, (k_gradient<MM, false, false>), c->nc, c->b, c->ph);
 }
 
+template <bool MM, int FLUX, bool VISC, int GR, int SHAPE>
+void launch_edge_s(cfd2d_ctx* c)
+{
+	constexpr int NT = EdgeShape<SHAPE>::NT;
+	int n = 2 * c->ne;
+	rec_start(c, CFD2D_K_EDGE_FLUX);
+	hipLaunchKernelGGL((k_edge_flux<MM, FLUX, VISC, GR, SHAPE>), dim3((n + NT - 1) / NT), dim3(NT), 0, c->st, c->b,
+	                   c->ph);
+	rec_stop(c);
+}
+
+// the launch shapes are instantiated for single-material contexts and the
+// unlimited or stored-gradient edge passes (the variants the autotuner
+// times); the others launch shape 0
+template <bool MM, int FLUX, bool VISC, int GR>
+void launch_edge_f(cfd2d_ctx* c)
+{
+	constexpr bool shapes = !MM && GR <= 2;
+	switch (shapes ? c->edge_shape : 0) {
+	case 1: launch_edge_s<MM, FLUX, VISC, GR, shapes ? 1 : 0>(c); break;
+	case 2: launch_edge_s<MM, FLUX, VISC, GR, shapes ? 2 : 0>(c); break;
+	case 3: launch_edge_s<MM, FLUX, VISC, GR, shapes ? 3 : 0>(c); break;
+	default: launch_edge_s<MM, FLUX, VISC, GR, 0>(c); break;
+	}
+}
+
 template <bool MM, bool VISC, int GR>
 void launch_edge_v(cfd2d_ctx* c)
 {
 	switch (c->p.flux) {
-	case CFD2D_FLUX_LAX: LAUNCH(c, CFD2D_K_EDGE_FLUX, (k_edge_flux<MM, CFD2D_FLUX_LAX, VISC, GR>), 2 * c->ne, c->b, c->ph); break;
-	case CFD2D_FLUX_HLLC: LAUNCH(c, CFD2D_K_EDGE_FLUX, (k_edge_flux<MM, CFD2D_FLUX_HLLC, VISC, GR>), 2 * c->ne, c->b, c->ph); break;
-	case CFD2D_FLUX_ROE: LAUNCH(c, CFD2D_K_EDGE_FLUX, (k_edge_flux<MM, CFD2D_FLUX_ROE, VISC, GR>), 2 * c->ne, c->b, c->ph); break;
-	default: LAUNCH(c, CFD2D_K_EDGE_FLUX, (k_edge_flux<MM, CFD2D_FLUX_GODUNOV, VISC, GR>), 2 * c->ne, c->b, c->ph); break;
+	case CFD2D_FLUX_LAX: launch_edge_f<MM, CFD2D_FLUX_LAX, VISC, GR>(c); break;
+	case CFD2D_FLUX_HLLC: launch_edge_f<MM, CFD2D_FLUX_HLLC, VISC, GR>(c); break;
+	case CFD2D_FLUX_ROE: launch_edge_f<MM, CFD2D_FLUX_ROE, VISC, GR>(c); break;
+	default: launch_edge_f<MM, CFD2D_FLUX_GODUNOV, VISC, GR>(c); break;
 	}
 }
 
@@ -1086,20 +1150,32 @@ void lsq_weights(const Mesh& m, std::vector<double>& w)
 	}
 }
 
+void drop_adj_graph(cfd2d_ctx::AdjGraph& a)
+{
+	if (a.x) (void)hipGraphExecDestroy(a.x);
+	if (a.g) (void)hipGraphDestroy(a.g);
+	a.x = nullptr;
+	a.g = nullptr;
+	a.key.clear();
+}
+
+// the recorded runs hold the step variant and buffers of their recording
 void drop_graph(cfd2d_ctx* c)
 {
-	if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-	if (c->graph) (void)hipGraphDestroy(c->graph);
-	c->graph_exec = nullptr;
-	c->graph = nullptr;
+	for (auto& g : c->graphs) {
+		(void)hipGraphExecDestroy(g.x);
+		(void)hipGraphDestroy(g.g);
+	}
+	c->graphs.clear();
+	drop_adj_graph(c->adj_fwd);
+	drop_adj_graph(c->adj_rev);
 }
 
 void destroy_ctx(cfd2d_ctx* c)
 {
 	if (!c) return;
 	if (c->st) (void)hipStreamSynchronize(c->st);
-	if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-	if (c->graph) (void)hipGraphDestroy(c->graph);
+	drop_graph(c);
 	for (auto e : c->ev0) (void)hipEventDestroy(e);
 	for (auto e : c->ev1) (void)hipEventDestroy(e);
 	for (void* p : c->allocs) (void)hipFree(p);
@@ -1487,19 +1563,43 @@ int cfd2d_step(cfd2d_ctx* c, int nsteps)
 	return check_launch("cfd2d_step");
 }
 
+// nsteps steps as recorded runs of up to GRAPH_STEPS steps each: a run of
+// that many steps is recorded once (its kernels with their arguments) and
+// replayed with one launch, so a pass of up to GRAPH_STEPS steps is one
+// launch; at most GRAPH_KEEP recorded lengths are kept
+constexpr int GRAPH_STEPS = 500, GRAPH_KEEP = 4;
+
 int cfd2d_step_graph(cfd2d_ctx* c, int nsteps)
 {
 	if (!c || nsteps < 0) return fail(CFD2D_ERR_ARG, "bad arguments");
-	if (!c->graph_exec) {
-		c->capturing = true;
-		HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-		step_once(c);
-		hipError_t e = hipStreamEndCapture(c->st, &c->graph);
-		c->capturing = false;
-		if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-		HIPCHK(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+	while (nsteps > 0) {
+		int k = std::min(nsteps, GRAPH_STEPS);
+		hipGraphExec_t x = nullptr;
+		for (auto& g : c->graphs)
+			if (g.steps == k) x = g.x;
+		if (!x) {
+			if ((int)c->graphs.size() >= GRAPH_KEEP) {
+				(void)hipGraphExecDestroy(c->graphs.front().x);
+				(void)hipGraphDestroy(c->graphs.front().g);
+				c->graphs.erase(c->graphs.begin());
+			}
+			hipGraph_t g = nullptr;
+			c->capturing = true;
+			HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+			for (int s = 0; s < k; s++) step_once(c);
+			hipError_t e = hipStreamEndCapture(c->st, &g);
+			c->capturing = false;
+			if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+			e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+			if (e != hipSuccess) {
+				(void)hipGraphDestroy(g);
+				return hip_fail(e, "hipGraphInstantiate");
+			}
+			c->graphs.push_back(cfd2d_ctx::StepGraph{k, g, x});
+		}
+		HIPCHK(hipGraphLaunch(x, c->st));
+		nsteps -= k;
 	}
-	for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(c->graph_exec, c->st));
 	return CFD2D_OK;
 }
 
@@ -1612,15 +1712,25 @@ int cfd2d_autotune(cfd2d_ctx* c, int trial_steps, int* fuse_mode, int* use_graph
 	if (err == hipSuccess) err = hipEventCreate(&e1);
 	c->timing = false;
 	float best = -1.0f;
-	int best_fuse = 0, best_graph = 0, best_stream = 0;
-	for (int v = 0; v < 8 && err == hipSuccess; v++) {
-		int fuse = v & 1, graph = (v >> 1) & 1, strm = v >> 2;
+	int shape0 = c->edge_shape;
+	int best_fuse = 0, best_graph = 0, best_stream = 0, best_shape = 0;
+	// the code-path variants with the default launch shape, then the launch
+	// shapes of the edge pass with the fastest of them
+	for (int v = 0; v < 8 + EDGE_SHAPES - 1 && err == hipSuccess; v++) {
+		int fuse = v & 1, graph = (v >> 1) & 1, strm = (v >> 2) & 1, shape = 0;
+		if (v >= 8) {
+			fuse = best_fuse;
+			graph = best_graph;
+			strm = best_stream;
+			shape = v - 7;
+		}
 		c->fuse_mode = fuse;
 		c->prefer_graph = graph != 0;
 		c->stream_mode = strm;
+		c->edge_shape = shape;
 		drop_graph(c);
 		if ((err = copy_all(false)) != hipSuccess) break;
-		if (cfd2d_step(c, 1) != CFD2D_OK) {  // warm-up (records the graph)
+		if (cfd2d_step(c, graph ? trial_steps : 1) != CFD2D_OK) {  // warm-up (records the timed graph)
 			err = hipErrorUnknown;
 			break;
 		}
@@ -1638,6 +1748,7 @@ int cfd2d_autotune(cfd2d_ctx* c, int trial_steps, int* fuse_mode, int* use_graph
 			best_fuse = fuse;
 			best_graph = graph;
 			best_stream = strm;
+			best_shape = shape;
 		}
 	}
 	if (err == hipSuccess) err = copy_all(false);
@@ -1651,15 +1762,35 @@ int cfd2d_autotune(cfd2d_ctx* c, int trial_steps, int* fuse_mode, int* use_graph
 		c->fuse_mode = fuse0;
 		c->prefer_graph = graph0;
 		c->stream_mode = stream0;
+		c->edge_shape = shape0;
 		return hip_fail(err, "cfd2d_autotune");
 	}
 	c->fuse_mode = best_fuse;
 	c->prefer_graph = best_graph != 0;
 	c->stream_mode = best_stream;
+	c->edge_shape = best_shape;
 	if (fuse_mode) *fuse_mode = best_fuse;
 	if (use_graph) *use_graph = best_graph;
 	if (stream) *stream = best_stream;
 	if (ms_per_step) *ms_per_step = best / trial_steps;
+	return CFD2D_OK;
+}
+
+int cfd2d_edge_shape(cfd2d_ctx* c, int shape)
+{
+	if (!c || shape < 0 || shape >= EDGE_SHAPES) return fail(CFD2D_ERR_ARG, "cfd2d_edge_shape: shape is 0..3");
+	c->edge_shape = shape;
+	drop_graph(c);  // the recorded step holds the previous choice
+	return CFD2D_OK;
+}
+
+int cfd2d_tuning(cfd2d_ctx* c, int* fuse_mode, int* use_graph, int* stream_state, int* edge_shape)
+{
+	if (!c) return fail(CFD2D_ERR_ARG, "null");
+	if (fuse_mode) *fuse_mode = c->fuse_mode;
+	if (use_graph) *use_graph = c->prefer_graph ? 1 : 0;
+	if (stream_state) *stream_state = c->stream_mode;
+	if (edge_shape) *edge_shape = c->edge_shape;
 	return CFD2D_OK;
 }
 

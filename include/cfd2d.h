@@ -120,6 +120,17 @@ int  cfd2d_mesh_case_offsets(const cfd2d_mesh* m, int* n_cases, int* cell_offset
 int  cfd2d_mesh_export(const cfd2d_mesh* m, double* cell_geom, int* cell_nodes, int* cell_edges,
                        double* edge_geom, int* edge_int, int* edge_group, double* edge_shift);
 void cfd2d_mesh_destroy(cfd2d_mesh* m);
+/* The reverse of the mesh geometry (extension): given adjoints of what the
+ * explicit kernels read -- Ge double[13][n_edges] per edge: normal (x, y)
+ * at Gauss points 1 and 2, side s's offset to Gauss point g (x, y) at
+ * 4 + 4 s + 2 g, length at 12; Gc double[7][n_cells] per cell: area, then
+ * per face (ascending edge order) its Green-Gauss weight sign * n l (x, y),
+ * or with lsq != 0 its least-squares weight (x, y); lny double[n_edges]:
+ * adjoint of l * n_y -- returns dxy double[n_nodes][2], the adjoint of every
+ * node coordinate (inputs may be NULL: zero).  cfd2d_adjoint_geometry's
+ * host half; host only. */
+int  cfd2d_mesh_geometry_vjp(const cfd2d_mesh* m, int lsq, const double* Ge, const double* Gc, const double* lny,
+                             double* dxy);
 
 /* ---- solver context (device) ----
  * Replaces the state arrays and passes of FVM_TVD (fvm_tvd.cpp:177-197,
@@ -171,15 +182,25 @@ int  cfd2d_fuse_gradient(cfd2d_ctx* c, int mode);
  * device arrays are over twice the 256 MB Infinity Cache.  The same bits
  * either way.  No reference counterpart. */
 int  cfd2d_stream_state(cfd2d_ctx* c, int mode);
+/* Launch shape of the edge pass (no reference counterpart): 0 = 256 threads
+ * per workgroup with the compiler's register allocation, 1 = 256 threads
+ * held to five waves per SIMD (at most 96 VGPRs), 2 = 128 threads, 3 = 512
+ * threads.  The same results in every shape.  Shapes apply to
+ * single-material contexts' unlimited and stored-gradient edge passes. */
+int  cfd2d_edge_shape(cfd2d_ctx* c, int shape);
 /* Autotuner (no reference counterpart): times trial_steps steps of each
  * variant -- gradients in the edge pass or in their own pass, eager launches
  * or a replayed recorded step, cached or streamed state in the update pass
- * -- on the context's current state, restores that state exactly, and
- * keeps the fastest for cfd2d_step (same results either way).  Reports the
- * choice and its ms per step (any pointer may be NULL).  Uses a temporary
- * device copy of the state (~100 B per cell). */
+ * -- on the context's current state, then the edge pass's launch shapes
+ * (cfd2d_edge_shape) with the fastest of them, restores that state exactly,
+ * and keeps the fastest for cfd2d_step (same results either way).  Reports
+ * the choice and its ms per step (any pointer may be NULL; cfd2d_tuning
+ * reads the launch shape).  Uses a temporary device copy of the state
+ * (~100 B per cell). */
 int  cfd2d_autotune(cfd2d_ctx* c, int trial_steps, int* fuse_mode, int* use_graph, int* stream_state,
                     double* ms_per_step);
+/* the step variant in effect (as cfd2d_autotune or the setters left it) */
+int  cfd2d_tuning(cfd2d_ctx* c, int* fuse_mode, int* use_graph, int* stream_state, int* edge_shape);
 /* which kernels get events while timing is on: bit k = CFD2D_K_* id
  * (default all); fewer events disturb the stream less */
 int  cfd2d_timing_kernels(cfd2d_ctx* c, unsigned mask);
@@ -200,6 +221,26 @@ int  cfd2d_ctx_stream(cfd2d_ctx* c, void** hip_stream);
  * context is left at the final state, as after cfd2d_step(c, nsteps). */
 int  cfd2d_adjoint(cfd2d_ctx* c, int nsteps, int ckpt_every, const int* edge_mask, double* J, double* dJ_dU0,
                    double* dJ_dbc);
+/* cfd2d_adjoint plus dJ_dxy = dJ/d(node coordinates) as double[n_nodes][2]
+ * (x, y) of mesh m, the mesh the context was created from: the reverse sweep
+ * also accumulates the adjoints of the geometry the kernels read (edge
+ * normals and lengths, Gauss-point offsets, cell areas, Green-Gauss or
+ * least-squares weights; the wall-force objective's l n_y), and the host
+ * carries them back through the mesh builder (centroids, normals, Gauss
+ * points, Heron areas, MeshReaderSalomeUnv.cpp:89-255, and the weights) to
+ * the nodes.  The time step is held at its value (as in cfd2d_adjoint) and
+ * so are the normals' orientation and periodic shifts.  Each stage also
+ * forms its forward fluxes again.  Extension, no reference counterpart. */
+int  cfd2d_adjoint_geometry(cfd2d_ctx* c, const cfd2d_mesh* m, int nsteps, int ckpt_every, const int* edge_mask,
+                            double* J, double* dJ_dU0, double* dJ_dbc, double* dJ_dxy);
+/* Whether cfd2d_adjoint records its forward pass (checkpoints, recorded
+ * segment, seed) and its reverse sweep (recomputed segments, every stage's
+ * reverse kernels, the boundary sums) each as a HIP graph and replays it:
+ * enable != 0 (the default) records on the first call of a shape (step
+ * count, interval, objective size) and replays on later calls, one launch
+ * per pass; 0 launches every kernel directly.  The same results either way.
+ * No reference counterpart. */
+int  cfd2d_adjoint_graph(cfd2d_ctx* c, int enable);
 /* The objective of the last cfd2d_adjoint call split by case of a
  * concatenated mesh (double[n_cases]; each case's own masked edges, summed
  * in ascending edge order).  Extension, no reference counterpart. */

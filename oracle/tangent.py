@@ -2,7 +2,7 @@
 
 TEST INFRASTRUCTURE ONLY: the tangent-linear reference the GPU adjoint is
 checked against (<dJ/dU0, v> + <dJ/dpar, w> = dJ[v, w]).  It covers the
-configurations the adjoint supports: LAX or HLLC flux, Green-Gauss or
+configurations the adjoint supports: LAX, HLLC or exact Godunov flux, Green-Gauss or
 least-squares gradients, Heun RK2 or SSP-RK3, inlet / outlet / slip-wall /
 pressure-outlet boundaries, periodic pairing, no limiter, no viscous terms,
 a fixed time step.  Every formula follows the C oracle (fvm_oracle.c), which
@@ -88,6 +88,19 @@ def sqrt(x):
     return np.sqrt(x)
 
 
+def expd(x):
+    if isinstance(x, Dual):
+        e = np.exp(x.v)
+        return Dual(e, e * x.d)
+    return np.exp(x)
+
+
+def logd(x):
+    if isinstance(x, Dual):
+        return Dual(np.log(x.v), x.d / x.v)
+    return np.log(x)
+
+
 def absd(x):
     if isinstance(x, Dual):
         sg = np.where(x.v < 0, -1.0, 1.0)
@@ -123,8 +136,8 @@ class Model:
         case = solver.case
         c = case.config
         g = solver.geometry
-        if c.flux not in ("LAX", "HLLC") or c.limiter != "NONE" or c.viscous:
-            raise ValueError("tangent model: LAX/HLLC, no limiter, inviscid only")
+        if c.flux not in ("LAX", "HLLC", "GODUNOV") or c.limiter != "NONE" or c.viscous:
+            raise ValueError("tangent model: LAX/HLLC/GODUNOV, no limiter, inviscid only")
         self.flux, self.lsq, self.rk = c.flux, c.gradient in ("LSQ", "LEAST_SQUARES"), c.rk_stages
         self.gam_flux = c.gam_flux
         mat = c.materials[0]
@@ -138,6 +151,9 @@ class Model:
         self.nx, self.ny, self.l = g["edge_geom"][:, 0].copy(), g["edge_geom"][:, 1].copy(), g["edge_geom"][:, 2].copy()
         self.gp = g["edge_geom"][:, 3:9].copy()
         self.sh = g["edge_shift"].copy()
+        self._cell_nodes = g["cell_nodes"].copy()
+        self._n1, self._n2 = g["edge_int"][:, 0].copy(), g["edge_int"][:, 1].copy()
+        self.nx0, self.ny0 = self.nx.copy(), self.ny.copy()
         self.ebc = solver.edge_bc.copy()
         bcs = [b for b in c.boundaries if b.type != "BOUND_PERIODIC"]
         self.bctype = np.array([b.code for b in bcs] or [0])
@@ -154,6 +170,72 @@ class Model:
         if self.lsq:
             self._lsq_weights()
         self.dtS = None
+
+    def set_geometry(self, xy, dxy):
+        """Geometry rebuilt from node coordinates xy (n_nodes, 2) carrying the
+        tangent dxy, with the expressions of the mesh builder (centroid
+        ((x0 + x1) + x2) / 3; normal (y2 - y1, x1 - x2) / l, oriented as
+        exported; Gauss points mid -/+ (x2 - x1) / (2 sqrt 3); Heron area over
+        the cell's edges in order; least-squares weights A^-1 d_k): every
+        geometric array becomes a Dual, so a run gives dJ[dxy]."""
+        X = Dual(np.asarray(xy, dtype=np.float64)[:, 0].copy(), np.asarray(dxy, dtype=np.float64)[:, 0].copy())
+        Y = Dual(np.asarray(xy, dtype=np.float64)[:, 1].copy(), np.asarray(dxy, dtype=np.float64)[:, 1].copy())
+        cn = self._cell_nodes
+        self.cx = (X[cn[:, 0]] + X[cn[:, 1]] + X[cn[:, 2]]) / 3.0
+        self.cy = (Y[cn[:, 0]] + Y[cn[:, 1]] + Y[cn[:, 2]]) / 3.0
+        a, b = self._n1, self._n2
+        x1, y1, x2, y2 = X[a], Y[a], X[b], Y[b]
+        rx, ry = y2 - y1, x1 - x2
+        l = sqrt(rx * rx + ry * ry)
+        sg = np.where(val(rx) * self.nx0 + val(ry) * self.ny0 >= 0.0, 1.0, -1.0)
+        self.nx, self.ny, self.l = (rx / l) * sg, (ry / l) * sg, l
+        sq3 = 1.0 / np.sqrt(3.0)
+        mx, my = (x1 + x2) / 2.0, (y1 + y2) / 2.0
+        cols = [mx, my, mx - sq3 * (x2 - x1) / 2.0, my - sq3 * (y2 - y1) / 2.0,
+                mx + sq3 * (x2 - x1) / 2.0, my + sq3 * (y2 - y1) / 2.0]
+        self.gp = Dual(np.stack([c.v for c in cols], axis=1), np.stack([c.d for c in cols], axis=1))
+        ea, eb, ec = (self.l[self.ce[:, k]] for k in range(3))
+        pp = (ea + eb + ec) / 2.0
+        self.S = sqrt(pp * (pp - ea) * (pp - eb) * (pp - ec))
+        self._offsets()
+        if self.lsq:
+            self._lsq_weights_dual()
+
+    def _offsets(self):
+        cc2 = np.where(self.inner, self.c2, self.c1)
+        gp = self.gp
+        self.d1 = [stack([gp[:, 2 + 2 * k] - self.cx[self.c1], gp[:, 3 + 2 * k] - self.cy[self.c1]]) for k in range(2)]
+        self.d2 = [stack([gp[:, 2 + 2 * k] - (self.cx[cc2] + self.sh[:, 0]),
+                          gp[:, 3 + 2 * k] - (self.cy[cc2] + self.sh[:, 1])]) for k in range(2)]
+
+    def _lsq_weights_dual(self):
+        nc = self.nc
+        me = np.arange(nc)
+        dx, dy = [], []
+        for k in range(3):
+            e = self.ce[:, k]
+            c1, c2 = self.c1[e], self.c2[e]
+            bnd = c2 < 0
+            mx, my = self.gp[e, 0], self.gp[e, 1]
+            nx, ny = self.nx[e], self.ny[e]
+            dn = (mx - self.cx) * nx + (my - self.cy) * ny
+            c2s = np.where(bnd, me, c2)
+            ax = (self.cx[c2s] + self.sh[e, 0]) - self.cx[c1]
+            ay = (self.cy[c2s] + self.sh[e, 1]) - self.cy[c1]
+            bx = self.cx[c1] - (self.cx[c2s] + self.sh[e, 0])
+            by = self.cy[c1] - (self.cy[c2s] + self.sh[e, 1])
+            is1 = me == c1
+            dx.append(where(bnd, 2.0 * dn * nx, where(is1, ax, bx)))
+            dy.append(where(bnd, 2.0 * dn * ny, where(is1, ay, by)))
+        a11 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]
+        a12 = dx[0] * dy[0] + dx[1] * dy[1] + dx[2] * dy[2]
+        a22 = dy[0] * dy[0] + dy[1] * dy[1] + dy[2] * dy[2]
+        det = a11 * a22 - a12 * a12
+        wx = [(a22 * dx[k] - a12 * dy[k]) / det for k in range(3)]
+        wy = [(a11 * dy[k] - a12 * dx[k]) / det for k in range(3)]
+        v = np.stack([np.stack([wx[k].v, wy[k].v], axis=1) for k in range(3)], axis=1)
+        d = np.stack([np.stack([wx[k].d, wy[k].d], axis=1) for k in range(3)], axis=1)
+        self.w = Dual(v, d)
 
     def _lsq_weights(self):
         nc = self.nc
@@ -251,8 +333,112 @@ class Model:
             out.append((gx, gy))
         return out
 
+    def godunov(self, L, R, nx, ny):
+        """FVM_TVD::calcFlux (src/methods/fvm_tvd.cpp:602-622) with rim_orig
+        (src/global.cpp:232-405) as the C oracle restates it, row-vectorised:
+        every row iterates Newton until its own stop test holds (rows that
+        stopped keep their P), branches per row on the values, powers as
+        exp(log(x) k)."""
+        G = self.gam_flux
+        AGAM = G - 1.0
+        DGAM, GGAM, HGAM, FGAM = 2.0 / AGAM, np.sqrt(G * AGAM), AGAM / 2.0, 3.0 * G - 1.0
+        OGAM, QGAM = AGAM / (2.0 * G), G + 1.0
+        PGAM, RGAM, SGAM, TGAM = QGAM / (2.0 * G), 4.0 * G, G * AGAM, QGAM / 2.0
+        eps = 1.0e-5
+        unl = L[2] * nx + L[3] * ny
+        unr = R[2] * nx + R[3] * ny
+        utl = L[2] * ny - L[3] * nx
+        utr = R[2] * ny - R[3] * nx
+        RB, PB, UB, VB = L[0], L[1], unl, utl
+        RE, PE, UE, VE = R[0], R[1], unr, utr
+        CB = sqrt(G * PB / RB)
+        CE = sqrt(G * PE / RE)
+        EB = CB * CB / SGAM
+        EE = CE * CE / SGAM
+        RCB, RCE = RB * CB, RE * CE
+        DU = UB - UE
+        vac = val(DU) < val(-2.0 * (CB + CE) / AGAM)
+        P = (PB * RCE + PE * RCB + DU * RCB * RCE) / (RCB + RCE)
+        active = ~vac
+        it = 0
+        while active.any():
+            P = where(active & (val(P) < eps), eps, P)
+            PPB, PPE = P / PB, P / PE
+            rb, re_ = val(PB) > val(P), val(PE) > val(P)
+            ZFB = CB * expd(logd(where(rb, PPB, 1.0)) * OGAM)
+            PKB = PGAM * PPB + OGAM
+            ZNB = RCB * sqrt(where(rb, 1.0, PKB))
+            F1 = where(rb, DGAM * (ZFB - CB), (P - PB) / ZNB)
+            FS1 = where(rb, ZFB / (G * P), (QGAM * PPB + FGAM) / (RGAM * ZNB * PKB))
+            ZFE = CE * expd(logd(where(re_, PPE, 1.0)) * OGAM)
+            PKE = PGAM * PPE + OGAM
+            ZNE = RCE * sqrt(where(re_, 1.0, PKE))
+            F2 = where(re_, DGAM * (ZFE - CE), (P - PE) / ZNE)
+            FS2 = where(re_, ZFE / (G * P), (QGAM * PPE + FGAM) / (RGAM * ZNE * PKE))
+            res = DU - F1 - F2
+            P = where(active, P + res / (FS1 + FS2), P)
+            active = active & (np.abs(val(res)) > eps)
+            it += 1
+            if it > 500:
+                raise RuntimeError("tangent model: rim_orig did not converge in 500 iterations")
+        # the star states (the vacuum rows take the vacuum branch below)
+        Ps = where(vac, 1.0, P)
+        rb, re_ = val(PB) > val(Ps), val(PE) > val(Ps)
+        ZFB = CB * expd(logd(where(rb, Ps / PB, 1.0)) * OGAM)
+        ZFE = CE * expd(logd(where(re_, Ps / PE, 1.0)) * OGAM)
+        EFr = ZFB * ZFB / SGAM
+        UFr = UB + DGAM * (CB - ZFB)
+        RFr = Ps / (AGAM * EFr)
+        Db = UB - sqrt(where(rb, 1.0, (TGAM * Ps + HGAM * PB) / RB))
+        RUBD = RB * (UB - Db)
+        RFs = RUBD * RUBD / (PB - Ps + RUBD * (UB - Db))
+        UFs = Db + RUBD / RFs
+        EFs = Ps / (AGAM * RFs)
+        ESr = ZFE * ZFE / SGAM
+        USr = UE - DGAM * (CE - ZFE)
+        RSr = Ps / (AGAM * ESr)
+        De = UE + sqrt(where(re_, 1.0, (TGAM * Ps + HGAM * PE) / RE))
+        RUED = RE * (UE - De)
+        RSs = RUED * RUED / (PE - Ps + RUED * (UE - De))
+        USs = De + RUED / RSs
+        ESs = Ps / (AGAM * RSs)
+        zero = where(vac, 0.0, 0.0 * Ps)
+        RF = where(vac, zero, where(rb, RFr, RFs))
+        EF = where(vac, zero, where(rb, EFr, EFs))
+        UF = where(vac, zero, where(rb, UFr, UFs))
+        SBL = where(vac, UB - CB, where(rb, UB - CB, Db))
+        SFL = where(vac, UB + 2.0 * CB / AGAM, where(rb, UFr - ZFB, Db))
+        RS = where(vac, zero, where(re_, RSr, RSs))
+        ES = where(vac, zero, where(re_, ESr, ESs))
+        US = where(vac, zero, where(re_, USr, USs))
+        SEL = where(vac, UE + CE, where(re_, UE + CE, De))
+        SSL = where(vac, UE - 2.0 * CE / AGAM, where(re_, USr + ZFE, De))
+        # sampling at x/t = 0
+        sel, sbl, ssl, sfl, us = val(SEL) <= 0.0, val(SBL) >= 0.0, val(SSL) >= 0.0, val(SFL) <= 0.0, val(US) >= 0.0
+        UIl = (UB + DGAM * GGAM * sqrt(EB)) / (1 + DGAM)
+        EIl = (UIl * UIl) / SGAM
+        RIl = RB * expd(logd(EIl / EB) * (1 / AGAM))
+        UIr = (UE - DGAM * GGAM * sqrt(EE)) / (1 + DGAM)
+        EIr = (UIr * UIr) / SGAM
+        RIr = RE * expd(logd(EIr / EE) * (1 / AGAM))
+        fanl = val(SFL) > 0.0
+        star = ssl & sfl
+        pick = lambda e, b, f, s, fl, fr: where(sel, e, where(sbl, b, where(star, where(us, f, s), where(fanl, fl, fr))))
+        RI = pick(RE, RB, RF, RS, RIl, RIr)
+        EI = pick(EE, EB, EF, ES, EIl, EIr)
+        UI = pick(UE, UB, UF, US, UIl, UIr)
+        VI = pick(VE, VB, VB, VE, VB, VE)
+        PI = AGAM * EI * RI
+        UN, UT = UI, VI
+        ux = UN * nx + UT * ny
+        vy = UN * ny - UT * nx
+        f0 = RI * UN
+        return [f0, f0 * ux + PI * nx, f0 * vy + PI * ny, (RI * (EI + 0.5 * (ux * ux + vy * vy)) + PI) * UN]
+
     def numflux(self, L, R, nx, ny):
         G = self.gam_flux
+        if self.flux == "GODUNOV":
+            return self.godunov(L, R, nx, ny)
         unl = L[2] * nx + L[3] * ny
         unr = R[2] * nx + R[3] * ny
         if self.flux == "LAX":
@@ -382,6 +568,19 @@ def tangent(solver, U0, dU0, dpar, nsteps, tau, mask):
     par = [Dual(m.bcpar0[:, k].copy(), np.asarray(dpar, dtype=np.float64)[:, k].copy()) for k in range(4)]
     U = [Dual(U0[q].copy(), dU0[q].copy()) for q in range(4)]
     U = m.run(U, par, nsteps, tau)
+    J = m.objective(U, mask)
+    return float(J.v), float(J.d)
+
+
+def tangent_geometry(solver, xy, dxy, nsteps, tau, mask, U0=None):
+    """(J, dJ) for node coordinates xy (n_nodes, 2) moving along dxy, the
+    state U0 (default the case's) and boundary parameters held, after
+    nsteps steps of fixed time step tau (the geometry rebuilt as Duals)."""
+    m = Model(solver)
+    m.set_geometry(xy, dxy)
+    par = [m.bcpar0[:, k].copy() for k in range(4)]
+    U0 = solver.case.initial_state() if U0 is None else U0
+    U = m.run([Dual(U0[q].copy()) for q in range(4)], par, nsteps, tau)
     J = m.objective(U, mask)
     return float(J.v), float(J.d)
 

@@ -38,7 +38,7 @@ def directions(s, seed=5):
 
 @pytest.mark.parametrize("make,flux,gradient,rk", [
     (bump, "LAX", "GREEN_GAUSS", 2), (bump, "HLLC", "LSQ", 3), (bump, "LAX", "LSQ", 2),
-    (pulse, "HLLC", "GREEN_GAUSS", 3)])
+    (pulse, "HLLC", "GREEN_GAUSS", 3), (bump, "GODUNOV", "GREEN_GAUSS", 2), (pulse, "GODUNOV", "LSQ", 3)])
 def test_model_forward_equals_oracle(make, flux, gradient, rk):
     c = make(flux, gradient, rk)
     s = oracle.OracleSolver(c)
@@ -51,7 +51,8 @@ def test_model_forward_equals_oracle(make, flux, gradient, rk):
     assert np.max(np.abs(U - ref) / scale) <= 1e-14
 
 
-@pytest.mark.parametrize("flux,gradient,rk,tol", [("LAX", "GREEN_GAUSS", 2, 5e-5), ("HLLC", "LSQ", 3, 2e-6)])
+@pytest.mark.parametrize("flux,gradient,rk,tol", [("LAX", "GREEN_GAUSS", 2, 5e-5), ("HLLC", "LSQ", 3, 2e-6),
+                                                  ("GODUNOV", "GREEN_GAUSS", 2, 2e-6), ("GODUNOV", "LSQ", 3, 1e-5)])
 def test_tangent_matches_finite_differences(flux, gradient, rk, tol):
     """dJ[v, w] against (J(x + h d) - J(x - h d)) / 2h with the oracle's forward
     map; LAX's max() wave speed switches inside the stencil, which bounds the
@@ -88,3 +89,46 @@ def test_tangent_objective_is_the_wall_force():
     s.run(3)
     _, fy = s.wall_force(mask)
     assert abs(J - fy) <= 1e-13 * abs(fy)
+
+
+def bump_nodes(c):
+    m = c.mesh
+    return np.unique(m.bedges[m.bedge_group == m.group_index("bump")])
+
+
+@pytest.mark.parametrize("flux,gradient,rk,tol", [("LAX", "GREEN_GAUSS", 2, 5e-5), ("HLLC", "LSQ", 3, 2e-6),
+                                                  ("HLLC", "GREEN_GAUSS", 2, 2e-6), ("GODUNOV", "GREEN_GAUSS", 2, 2e-6)])
+def test_geometry_tangent_matches_finite_differences(flux, gradient, rk, tol):
+    """dJ[dxy] of the tangent model with the geometry rebuilt as Duals from the
+    nodes, against central differences of the C oracle run on meshes with
+    the nodes moved (bump nodes in y, a few interior nodes in x and y; the
+    time step held): pins the model's geometric derivatives (normals,
+    lengths, Gauss points, centroids, areas, weights, the objective's l n_y)."""
+    import dataclasses
+    c = bump(flux, gradient, rk)
+    s = oracle.OracleSolver(c)
+    tau, _ = s.tau()
+    mask = bump_mask(s)
+    n = 15
+    xy = c.mesh.xy
+    rng = np.random.default_rng(3)
+    dxy = np.zeros_like(xy)
+    bn = bump_nodes(c)
+    dxy[bn, 1] = rng.standard_normal(len(bn)) * 0.01
+    inner = rng.choice(np.setdiff1d(np.arange(len(xy)), np.unique(c.mesh.bedges)), 12, replace=False)
+    dxy[inner] = rng.standard_normal((12, 2)) * 0.01
+    J, dJ = tangent.tangent_geometry(s, xy, dxy, n, tau, mask)
+
+    def Jof(h):
+        cc = dataclasses.replace(c, mesh=dataclasses.replace(c.mesh, xy=xy + h * dxy))
+        o = oracle.OracleSolver(cc)
+        o.set_tau(tau)
+        o.run(n)
+        return o.wall_force(mask)[1]
+
+    J0 = Jof(0.0)
+    assert abs(J - J0) <= 1e-13 * abs(J0)
+    h = 1e-5
+    fd = (Jof(h) - Jof(-h)) / (2 * h)
+    assert abs(dJ) > 1e-6 * abs(J)
+    assert abs(dJ - fd) <= tol * abs(fd), (dJ, fd)

@@ -175,6 +175,7 @@ def test_autotune_restores_state_and_keeps_results(name, opts):
     U, F = a.state(), a.flags()
     choice = a.autotune(trial_steps=2)
     assert choice["fuse"] in (0, 1) and choice["graph"] in (0, 1) and choice["stream"] in (0, 1)
+    assert choice["edge_shape"] in range(4)
     assert choice["ms_per_step"] > 0
     assert np.array_equal(a.state(), U) and np.array_equal(a.flags(), F)
     assert a.status() == b.status()
@@ -300,3 +301,22 @@ def test_batched_equals_individual():
     for a, ref in zip(states, singles):
         assert np.array_equal(a, ref)
     b.close()
+
+
+@pytest.mark.parametrize("opts", [dict(), dict(gradient="LSQ", rk_stages=3, viscous=True)], ids=str)
+def test_edge_launch_shapes_are_bitwise(opts):
+    """Every launch shape of the edge pass (threads per workgroup, waves per
+    SIMD) gives the same bits, fused or stored gradients."""
+    case = gc.case("inlet", **opts)
+    outs = []
+    for shape in range(4):
+        for fuse in (True, False):
+            s = native.GpuSolver(case)
+            s.edge_shape(shape)
+            s.fuse_gradient(fuse)
+            s.run(15)
+            outs.append(s.state())
+            assert s.tuning()["edge_shape"] == shape
+            s.close()
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)
