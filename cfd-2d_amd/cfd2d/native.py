@@ -90,6 +90,7 @@ SIGNATURES = {
                                           C.c_void_p, C.c_void_p]),
     "cfd2d_riemann": (C.c_int, [C.c_int, _dp, C.c_double, C.c_int, _dp, C.c_void_p, C.POINTER(C.c_longlong)]),
     "cfd2d_flux": (C.c_int, [C.c_int, C.c_int, _dp, C.c_double, C.c_int, _dp, C.POINTER(C.c_longlong)]),
+    "cfd2d_flux_noalloc": (C.c_int, [C.c_int, C.c_int, _dp, C.c_double, C.c_int, _dp, C.POINTER(C.c_longlong)]),
     "cfd2d_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
     "cfd2d_host_free": (C.c_int, [C.c_void_p]),
     "cfd2d_timing": (C.c_int, [C.c_void_p, C.c_int]),
@@ -151,16 +152,20 @@ def riemann(rows, gam=1.4, max_newton=0):
 
 
 FLUX_KINDS = {"GODUNOV": 0, "LAX": 1, "HLLC": 2, "ROE": 3}
+NOALLOC_MAX_ROWS = 65536  # CFD2D_NOALLOC_MAX_ROWS
 
 
-def flux(kind, rows, gam=1.4, max_newton=0):
+def flux(kind, rows, gam=1.4, max_newton=0, noalloc=False):
     """A numerical flux on the device for rows (n, 10) = (rL, pL, uL, vL, rR,
-    pR, uR, vR, nx, ny); returns ((n, 4) fluxes, rows that hit the Newton cap)."""
+    pR, uR, vR, nx, ny); returns ((n, 4) fluxes, rows that hit the Newton cap).
+    noalloc=True calls cfd2d_flux_noalloc (no device allocation, at most
+    CFD2D_NOALLOC_MAX_ROWS rows)."""
     k = FLUX_KINDS[kind] if isinstance(kind, str) else int(kind)
     rows = np.ascontiguousarray(rows, np.float64).reshape(-1, 10)
     out = np.zeros((rows.shape[0], 4))
     hits = C.c_longlong()
-    check(lib().cfd2d_flux(k, rows.shape[0], rows.ravel(), float(gam), int(max_newton), out.ravel(), C.byref(hits)))
+    f = lib().cfd2d_flux_noalloc if noalloc else lib().cfd2d_flux
+    check(f(k, rows.shape[0], rows.ravel(), float(gam), int(max_newton), out.ravel(), C.byref(hits)))
     return out, hits.value
 
 

@@ -119,7 +119,56 @@ int pw_alloc_run(int kind, int n, const double* rows, double gam, int max_newton
 	return CFD2D_OK;
 }
 
+// The staging block of cfd2d_flux_noalloc: a fixed array in device memory
+// (each device holds its own copy for the life of the process), sized for
+// CFD2D_NOALLOC_MAX_ROWS flux rows.  Host code must not pass the array's name
+// to a kernel or a copy (on the host it names a stand-in, not device
+// memory): noalloc_scratch() returns the calling device's address of it,
+// looked up once per device with hipGetSymbolAddress.
+// (pw_layout of a flux at the maximum: 10 + 4 doubles per row, half a
+// double of iteration counts, the counter)
+constexpr size_t NOALLOC_DOUBLES = 29 * (size_t)CFD2D_NOALLOC_MAX_ROWS / 2 + 64;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+int noalloc_scratch(double** out)
+{
+	int dev = -1;
+	HIPCHK(hipGetDevice(&dev));
+	if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(CFD2D_ERR_ARG, "cfd2d_flux_noalloc: device index too large");
+	std::lock_guard<std::mutex> lock(g_scratch_mu);
+	if (!g_scratch_ptr[dev]) {
+		void* ptr = nullptr;
+		HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+		if (!ptr) return fail(CFD2D_ERR_HIP, "cfd2d_flux_noalloc: no device address for the scratch array");
+		g_scratch_ptr[dev] = static_cast<double*>(ptr);
+	}
+	*out = g_scratch_ptr[dev];
+	return 0;
+}
+
 }  // namespace
+
+extern "C" int cfd2d_flux_noalloc(int kind, int n, const double* rows, double gam, int max_newton, double* out,
+                                  long long* cap_hits)
+{
+	int rc = pw_check(kind, n, rows, gam, out);
+	if (rc) return rc;
+	if (kind == PW_RIM || n > CFD2D_NOALLOC_MAX_ROWS)
+		return fail(CFD2D_ERR_ARG, "cfd2d_flux_noalloc: at most " + std::to_string(CFD2D_NOALLOC_MAX_ROWS) +
+		                               " rows per call (got " + std::to_string(n) + ")");
+	static_assert(CFD2D_NOALLOC_MAX_ROWS % 4 == 0 &&
+	                  (10 + 4) * (size_t)CFD2D_NOALLOC_MAX_ROWS + CFD2D_NOALLOC_MAX_ROWS / 2 + 4 <= NOALLOC_DOUBLES,
+	              "scratch smaller than the staging layout of the maximum row count");
+	if (pw_layout(kind, (size_t)n).total > NOALLOC_DOUBLES)
+		return fail(CFD2D_ERR_ARG, "cfd2d_flux_noalloc: staging layout exceeds the scratch array");
+	double* st = nullptr;
+	if ((rc = noalloc_scratch(&st))) return rc;
+	return pw_run(kind, n, rows, gam, max_newton, out, nullptr, cap_hits, st);
+}
 
 extern "C" int cfd2d_riemann(int n, const double* rows, double gam, int max_newton, double* out, int* iters,
                              long long* cap_hits)

@@ -290,6 +290,17 @@ int  cfd2d_riemann(int n, const double* rows, double gam, int max_newton, double
  * double[n][4] = (mass, x-momentum, y-momentum, energy) flux through the
  * unit normal. */
 int  cfd2d_flux(int kind, int n, const double* rows, double gam, int max_newton, double* out, long long* cap_hits);
+/* cfd2d_flux without any device allocation, for callers that cannot
+ * allocate (tight serving loops, code that must not call hipMalloc).  No
+ * counterpart in the reference.  The rows are staged through a fixed device
+ * array of the library (7.6 MB per device, held until the process
+ * ends) instead of memory allocated for the call: n <= CFD2D_NOALLOC_MAX_ROWS
+ * (larger n returns CFD2D_ERR_ARG).  Every call on one device shares that
+ * array, so calls that overlap on one device (from several threads) race;
+ * the caller serialises them.  Same arguments and results as cfd2d_flux. */
+enum { CFD2D_NOALLOC_MAX_ROWS = 65536 };
+int  cfd2d_flux_noalloc(int kind, int n, const double* rows, double gam, int max_newton, double* out,
+                        long long* cap_hits);
 
 /* Page-locked host memory (hipHostMalloc): output buffers of
  * cfd2d_implicit_assemble placed here are copied at the link's full rate.
