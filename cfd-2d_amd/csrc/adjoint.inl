@@ -235,7 +235,9 @@ __device__ __forceinline__ void rim_d1(D1& RI, D1& EI, D1& PI, D1& UI, D1& VI, D
 	const double eps = 1.0e-5;
 	D1 zero{0.0, 0.0};
 	D1 US = zero, UF = zero, RF, RS, EF, ES, SBL, SFL, SSL, SEL;
-	D1 iB = 1.0 / (RB * PB), iE = 1.0 / (RE * PE);
+	D1 rpB = RB * PB, rpE = RE * PE;
+	D1 i4 = 1.0 / (rpB * rpE);
+	D1 iB = rpE * i4, iE = rpB * i4;
 	D1 IRB = PB * iB, IPB = RB * iB, IRE = PE * iE, IPE = RE * iE;
 	D1 CB = dsqrt(K.GAM * PB * IRB), CE = dsqrt(K.GAM * PE * IRE);
 	D1 RCB = RB * CB, RCE = RE * CE;
@@ -252,16 +254,22 @@ __device__ __forceinline__ void rim_d1(D1& RI, D1& EI, D1& PI, D1& UI, D1& VI, D
 		int it = 0;
 		for (;;) {
 			if (P.v < eps) P = D1{eps, 0.0};
-			D1 F[2], FS[2];
+			D1 F[2], FS[2], x[2], t[2], y[2], den[2];
+			bool rare[2];
 			for (int sd = 0; sd < 2; sd++) {
-				D1 PK = sd ? PE : PB, IPK = sd ? IPE : IPB, C = sd ? CE : CB, RC = sd ? RCE : RCB;
-				D1 x = P * IPK;
-				bool rare = PK.v > P.v;
-				D1 t = dsel(rare, x, K.PGAM * x + K.OGAM);
-				D1 y = dside_root(t, rare, K);
-				D1 r = 1.0 / dsel(rare, K.GAM * P, RC * y * t);
-				F[sd] = dsel(rare, K.DGAM * (C * y - C), (P - PK) * t * r);
-				FS[sd] = dsel(rare, C * y * r, (K.QGAM * x + K.FGAM) * r * K.IRGAM);
+				D1 PK = sd ? PE : PB, IPK = sd ? IPE : IPB, RC = sd ? RCE : RCB;
+				x[sd] = P * IPK;
+				rare[sd] = PK.v > P.v;
+				t[sd] = dsel(rare[sd], x[sd], K.PGAM * x[sd] + K.OGAM);
+				y[sd] = dside_root(t[sd], rare[sd], K);
+				den[sd] = dsel(rare[sd], K.GAM * P, RC * y[sd] * t[sd]);
+			}
+			D1 ir = 1.0 / (den[0] * den[1]);
+			for (int sd = 0; sd < 2; sd++) {
+				D1 PK = sd ? PE : PB, C = sd ? CE : CB;
+				D1 r = den[1 - sd] * ir;
+				F[sd] = dsel(rare[sd], K.DGAM * (C * y[sd] - C), (P - PK) * t[sd] * r);
+				FS[sd] = dsel(rare[sd], C * y[sd] * r, (K.QGAM * x[sd] + K.FGAM) * r * K.IRGAM);
 			}
 			D1 res = DU - F[0] - F[1];
 			P = P + res / (FS[0] + FS[1]);
@@ -269,19 +277,33 @@ __device__ __forceinline__ void rim_d1(D1& RI, D1& EI, D1& PI, D1& UI, D1& VI, D
 			if (!(fabs(res.v) > eps)) break;
 			if (max_iter > 0 && it >= max_iter) break;
 		}
+		D1 yy[2], den[2];
+		bool rr[2];
 		for (int sd = 0; sd < 2; sd++) {
 			D1 PK = sd ? PE : PB, IPK = sd ? IPE : IPB, IRK = sd ? IRE : IRB, C = sd ? CE : CB;
+			D1 RK = sd ? RE : RB;
+			double sg = sd ? 1.0 : -1.0;
+			rr[sd] = PK.v > P.v;
+			D1 t = dsel(rr[sd], P * IPK, (K.TGAM * P + K.HGAM * PK) * IRK);
+			yy[sd] = dside_root(t, rr[sd], K);
+			D1 Z = C * yy[sd];
+			D1 M = (-sg) * (RK * yy[sd]);
+			D1 Q = PK - P + M * ((-sg) * yy[sd]);
+			den[sd] = dsel(rr[sd], K.AGAM * (Z * Z * K.ISGAM), Q * M);
+		}
+		D1 is = 1.0 / (den[0] * den[1]);
+		for (int sd = 0; sd < 2; sd++) {
+			D1 PK = sd ? PE : PB, C = sd ? CE : CB;
 			D1 RK = sd ? RE : RB, UK = sd ? UE : UB;
 			double sg = sd ? 1.0 : -1.0;
-			bool rare = PK.v > P.v;
-			D1 t = dsel(rare, P * IPK, (K.TGAM * P + K.HGAM * PK) * IRK);
-			D1 y = dside_root(t, rare, K);
+			bool rare = rr[sd];
+			D1 y = yy[sd];
 			D1 Z = C * y;
 			D1 Er = Z * Z * K.ISGAM;
 			D1 D = UK + sg * y;
 			D1 M = (-sg) * (RK * y);
 			D1 Q = PK - P + M * ((-sg) * y);
-			D1 r = 1.0 / dsel(rare, K.AGAM * Er, Q * M);
+			D1 r = den[1 - sd] * is;
 			D1 Rs, Es, Us, head, tail;
 			if (rare) {
 				Es = Er;

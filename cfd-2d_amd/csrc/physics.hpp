@@ -103,9 +103,10 @@ __device__ __forceinline__ Prim ghost_state(int type, const double* par, double 
 // branch of each side on PK > P as in the reference.  max_iter > 0 caps the
 // loop and reports it in hit; iters (if given) receives the Newton
 // iterations taken (0 on the vacuum branch).  Restructured for the device:
-// each side's branch is one root and one reciprocal (side_newton,
-// side_star), so lanes that take different branches do not run both; 1/rho
-// and 1/p of a side come from one division; divisions by constants are
+// each side's branch is one root and one reciprocal (side_newton_a/b,
+// side_star_a/b), so lanes that take different branches do not run both;
+// the two sides' reciprocals come from one division (1/(a b) times b and
+// a), as do 1/rho and 1/p of both sides; divisions by constants are
 // products with their reciprocals.  These change the rounding of single
 // operations only (results agree with the reference to round-off, not bit
 // for bit); the FP64 division is the costliest operation of this solver.
@@ -193,17 +194,30 @@ struct RimSide {
 // src/global.cpp:278-303): rarefaction (PK > P) F = DGAM (Z - C),
 // F' = Z / (GAM P) with Z = C (P/PK)^OGAM; shock F = (P - PK) / Z,
 // F' = (QGAM P/PK + FGAM) / (RGAM Z PKK) with Z = R C sqrt(PKK),
-// PKK = PGAM P/PK + OGAM.  The two sides of the branch share one root and
-// one reciprocal (of GAM P or Z PKK).
-__device__ __forceinline__ void side_newton(const RimSide& s, double P, const RimConst& K, double& F, double& FS)
+// PKK = PGAM P/PK + OGAM.  The two sides of the branch share one root; the
+// reciprocal each needs (of GAM P or Z PKK, "den") is formed for both sides
+// of the problem from one division (side_newton_b takes it).
+struct SideNewton {
+	bool rare;
+	double x, t, y, den;
+};
+
+__device__ __forceinline__ SideNewton side_newton_a(const RimSide& s, double P, const RimConst& K)
 {
-	double x = P * s.IP;
-	bool rare = s.P > P;
-	double t = rare ? x : K.PGAM * x + K.OGAM;
-	double y = side_root(t, rare, K);
-	double r = 1.0 / (rare ? K.GAM * P : s.RC * y * t);
-	F = rare ? K.DGAM * (s.C * y - s.C) : (P - s.P) * t * r;
-	FS = rare ? s.C * y * r : (K.QGAM * x + K.FGAM) * r * K.IRGAM;
+	SideNewton v;
+	v.x = P * s.IP;
+	v.rare = s.P > P;
+	v.t = v.rare ? v.x : K.PGAM * v.x + K.OGAM;
+	v.y = side_root(v.t, v.rare, K);
+	v.den = v.rare ? K.GAM * P : s.RC * v.y * v.t;
+	return v;
+}
+
+__device__ __forceinline__ void side_newton_b(const RimSide& s, const SideNewton& v, double P, double r,
+                                              const RimConst& K, double& F, double& FS)
+{
+	F = v.rare ? K.DGAM * (s.C * v.y - s.C) : (P - s.P) * v.t * r;
+	FS = v.rare ? s.C * v.y * r : (K.QGAM * v.x + K.FGAM) * r * K.IRGAM;
 }
 
 // The star state (Rs, Es, Us) of one side at the converged P and its wave's
@@ -212,32 +226,45 @@ __device__ __forceinline__ void side_newton(const RimSide& s, double P, const Ri
 // U = UK + sg DGAM (Z - C), R = P / (AGAM E), head UK + sg C, tail U + sg Z.
 // Shock: w = sqrt((TGAM P + HGAM PK)/RK), D = UK + sg w, M = RK (UK - D),
 // Q = PK - P + M (UK - D), R = M^2/Q, U = D + Q/M, E = P Q / (M^2 AGAM),
-// head = tail = D.  One reciprocal per side: 1/(AGAM E) or 1/(Q M).
-__device__ __forceinline__ void side_star(const RimSide& s, double P, double sg, const RimConst& K, double& Rs,
-                                          double& Es, double& Us, double& head, double& tail)
+// head = tail = D.  One reciprocal per side, 1/(AGAM E) or 1/(Q M) ("den"),
+// the two sides' from one division (side_star_b takes it).
+struct SideStar {
+	bool rare;
+	double y, Z, Er, D, M, Q, den;
+};
+
+__device__ __forceinline__ SideStar side_star_a(const RimSide& s, double P, double sg, const RimConst& K)
 {
-	bool rare = s.P > P;
-	double t = rare ? P * s.IP : (K.TGAM * P + K.HGAM * s.P) * s.IR;
-	double y = side_root(t, rare, K);
-	double Z = s.C * y;
-	double Er = Z * Z * K.ISGAM;
-	double D = s.U + sg * y;
-	double M = -sg * s.R * y;
-	double Q = s.P - P + M * (-sg * y);
-	double r = 1.0 / (rare ? K.AGAM * Er : Q * M);
-	if (rare) {
-		Es = Er;
-		Us = s.U + sg * K.DGAM * (Z - s.C);
+	SideStar v;
+	v.rare = s.P > P;
+	double t = v.rare ? P * s.IP : (K.TGAM * P + K.HGAM * s.P) * s.IR;
+	v.y = side_root(t, v.rare, K);
+	v.Z = s.C * v.y;
+	v.Er = v.Z * v.Z * K.ISGAM;
+	v.D = s.U + sg * v.y;
+	v.M = -sg * s.R * v.y;
+	v.Q = s.P - P + v.M * (-sg * v.y);
+	v.den = v.rare ? K.AGAM * v.Er : v.Q * v.M;
+	return v;
+}
+
+__device__ __forceinline__ void side_star_b(const RimSide& s, const SideStar& v, double P, double sg, double r,
+                                            const RimConst& K, double& Rs, double& Es, double& Us, double& head,
+                                            double& tail)
+{
+	if (v.rare) {
+		Es = v.Er;
+		Us = s.U + sg * K.DGAM * (v.Z - s.C);
 		Rs = P * r;
 		head = s.U + sg * s.C;
-		tail = Us + sg * Z;
+		tail = Us + sg * v.Z;
 	} else {
-		double IM = Q * r;  // 1/M
-		Rs = M * M * M * r;
-		Us = D + Q * IM;
-		Es = P * Q * IM * IM * K.IAGAM;
-		head = D;
-		tail = D;
+		double IM = v.Q * r;  // 1/M
+		Rs = v.M * v.M * v.M * r;
+		Us = v.D + v.Q * IM;
+		Es = P * v.Q * IM * IM * K.IAGAM;
+		head = v.D;
+		tail = v.D;
 	}
 }
 
@@ -250,8 +277,10 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 
 	double US = 0.0, UF = 0.0;
 	double RF, RS, EF, ES, SBL, SFL, SSL, SEL;
-	// 1/rho and 1/p of a side from one division
-	const double iB = 1.0 / (RB * PB), iE = 1.0 / (RE * PE);
+	// 1/rho and 1/p of both sides from one division
+	const double rpB = RB * PB, rpE = RE * PE;
+	const double i4 = 1.0 / (rpB * rpE);
+	const double iB = rpE * i4, iE = rpB * i4;
 	RimSide B, E;
 	B.R = RB; B.P = PB; B.U = UB; B.IR = PB * iB; B.IP = RB * iB;
 	E.R = RE; E.P = PE; E.U = UE; E.IR = PE * iE; E.IP = RE * iE;
@@ -280,8 +309,10 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 		for (;;) {
 			if (P < eps) P = eps;
 			double F1, FS1, F2, FS2;
-			side_newton(B, P, K, F1, FS1);
-			side_newton(E, P, K, F2, FS2);
+			SideNewton nb = side_newton_a(B, P, K), ne = side_newton_a(E, P, K);
+			double ir = 1.0 / (nb.den * ne.den);
+			side_newton_b(B, nb, P, ne.den * ir, K, F1, FS1);
+			side_newton_b(E, ne, P, nb.den * ir, K, F2, FS2);
 			double res = DU - F1 - F2;
 			P = P + res / (FS1 + FS2);
 			++it;
@@ -292,8 +323,10 @@ __device__ __forceinline__ void rim(double& RI, double& EI, double& PI, double& 
 			}
 		}
 		if (iters) *iters = it;
-		side_star(B, P, -1.0, K, RF, EF, UF, SBL, SFL);
-		side_star(E, P, 1.0, K, RS, ES, US, SEL, SSL);
+		SideStar sb = side_star_a(B, P, -1.0, K), se = side_star_a(E, P, 1.0, K);
+		double is = 1.0 / (sb.den * se.den);
+		side_star_b(B, sb, P, -1.0, se.den * is, K, RF, EF, UF, SBL, SFL);
+		side_star_b(E, se, P, 1.0, sb.den * is, K, RS, ES, US, SEL, SSL);
 	}
 	if (SEL <= 0.0) {
 		RI = RE; EI = EE; UI = UE; VI = VE;
