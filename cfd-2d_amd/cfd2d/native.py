@@ -434,10 +434,16 @@ class Context:
                 pinned_array((nc, 4), np.float64), pinned_array((nc,), np.float64))
 
     def fuse_gradient(self, mode=True):
-        """Where the gradients are formed: True/1 in the edge pass, False/0 in
-        their own pass, None/-1 the default (edge pass unless a limiter is
-        on); the same results."""
-        m = -1 if mode is None else (1 if mode else 0)
+        """Where the gradients are formed: True/1 in the edge pass by each
+        edge's lanes, 2 or "tiled" in the tiled edge pass (once per cell and
+        tile, staged in LDS), False/0 in their own pass, None/-1 the default
+        (edge pass unless a limiter is on); the same results."""
+        if mode is None:
+            m = -1
+        elif mode == "tiled" or (not isinstance(mode, bool) and mode == 2):
+            m = 2
+        else:
+            m = 1 if mode else 0
         check(lib().cfd2d_fuse_gradient(self._h, m))
 
     def stream_state(self, mode=True):

@@ -69,6 +69,14 @@ struct Bufs {
 	double* bcpar;
 	double* mtab;  // 5 per material: M, Cv, gam, mu, kc
 	unsigned long long* err;  // [0] Newton cap hits, [1] non-finite cells
+	// tiled edge pass (k_edge_tile): per tile (int4) the first of its staged
+	// cells in tcell, their count, its first edge and edge count; tcell the
+	// staged cells of every tile, ascending; eslot per edge the LDS rows of
+	// c1 (low 16 bits) and c2 (high 16 bits, 0xffff on a boundary)
+	int* tiles;
+	int* tcell;
+	unsigned* eslot;
+	int ntiles;  // 0: the edges are not in c1 order, no tiled pass
 };
 
 struct Phys {
@@ -447,29 +455,15 @@ struct EdgeShape<3> {  // 512 threads
 };
 constexpr int EDGE_SHAPES = 4;
 
-template <bool MM, int FLUX, bool VISC, int GR, int SHAPE = 0>
-__global__ __attribute__((amdgpu_flat_work_group_size(1, EdgeShape<SHAPE>::NT),
-                          amdgpu_waves_per_eu(EdgeShape<SHAPE>::WPE))) void k_edge_flux(Bufs b, Phys ph)
+// One lane of an edge's pair from the primitives ps and gradients gs of the
+// cell cs it reconstructs (c2 for the second lane of an inner edge, c1
+// otherwise) onward: reconstruction, ghost, viscous terms, numerical flux and
+// the pair's combined store.  Shared by the edge passes, so they give the
+// same bits.
+template <bool MM, int FLUX, bool VISC>
+__device__ __forceinline__ void edge_lane(const Bufs& b, const Phys& ph, int e, int gp, int c1, bool inner, int cs,
+                                          const Prim& ps, const double* gs, double nx, double ny)
 {
-	int t = xcd_block() * EdgeShape<SHAPE>::NT + threadIdx.x;
-	int e = t >> 1, gp = t & 1;
-	if (e >= b.ne) return;  // both lanes of a pair leave together
-	int2 ce = edge_cells(b, e);
-	int c1 = ce.x, c2 = ce.y;
-	double nx = edge_geo(b, e)[0], ny = edge_geo(b, e)[1];
-	bool inner = c2 >= 0;
-	int cs = gp == 1 && inner ? c2 : c1;  // the cell this lane reconstructs
-	double gs[8];                          // its gradients
-	Prim ps;                               // and primitives
-	if (GR) {
-		int face[3];
-		Prim nb[3];
-		cell_gradient<MM, GR == 1 || GR == 3>(b, ph, cs, ps, face, nb, gs);
-		if (GR >= 3) minmod_limit(b, cs, face, ps, nb, gs);
-	} else {
-		ps = load_prim(b, cs);
-		ld8(b.G, cs, gs);
-	}
 	// Each lane extrapolates its own cell to its own Gauss point (kept) and to
 	// the partner's (sent); on an inner edge the pair swaps the sent states,
 	// so lane 0 holds (c1, c2) at Gauss point 1 and lane 1 at Gauss point 2 --
@@ -539,6 +533,82 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, EdgeShape<SHAPE>::NT),
 	double o[4];
 	for (int q = 0; q < 4; q++) o[q] = __shfl_xor(f[q], 1);
 	store_edge_flux(b.F, e, gp, f, o, edge_geo(b, e)[2] * 0.5);
+}
+
+template <bool MM, int FLUX, bool VISC, int GR, int SHAPE = 0>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, EdgeShape<SHAPE>::NT),
+                          amdgpu_waves_per_eu(EdgeShape<SHAPE>::WPE))) void k_edge_flux(Bufs b, Phys ph)
+{
+	int t = xcd_block() * EdgeShape<SHAPE>::NT + threadIdx.x;
+	int e = t >> 1, gp = t & 1;
+	if (e >= b.ne) return;  // both lanes of a pair leave together
+	int2 ce = edge_cells(b, e);
+	int c1 = ce.x, c2 = ce.y;
+	double nx = edge_geo(b, e)[0], ny = edge_geo(b, e)[1];
+	bool inner = c2 >= 0;
+	int cs = gp == 1 && inner ? c2 : c1;  // the cell this lane reconstructs
+	double gs[8];                          // its gradients
+	Prim ps;                               // and primitives
+	if (GR) {
+		int face[3];
+		Prim nb[3];
+		cell_gradient<MM, GR == 1 || GR == 3>(b, ph, cs, ps, face, nb, gs);
+		if (GR >= 3) minmod_limit(b, cs, face, ps, nb, gs);
+	} else {
+		ps = load_prim(b, cs);
+		ld8(b.G, cs, gs);
+	}
+	edge_lane<MM, FLUX, VISC>(b, ph, e, gp, c1, inner, cs, ps, gs, nx, ny);
+}
+
+// Tiled edge pass (fuse mode 2): one workgroup per tile of TILE_CELLS
+// consecutive cells.  The tile owns the edges whose c1 it holds -- a
+// contiguous edge range, since edges are numbered in c1 order -- and first
+// forms, once each, the gradients of every cell those edges read (its own
+// cells and the c2 cells beyond it) with the gradient pass's code, staging
+// primitives and gradients in LDS; then the lane pairs of its edges run
+// edge_lane from LDS.  A cell's gradient is formed by ~1.25 tiles instead of
+// by each of its three edges, and the edge lanes gather nothing but their
+// edge's own records.  The same bits as the other edge passes.
+constexpr int TILE_CELLS = 256;  // cells per tile (smaller when the staged set would not fit)
+constexpr int TILE_CAP = 384;    // staged cells per tile at most
+constexpr int TILE_ROW = 12;     // LDS doubles per staged cell: r, p, u, v, then the 8 gradients
+
+template <bool MM, int FLUX, bool VISC, int GR>
+__global__ __launch_bounds__(BLOCK) void k_edge_tile(Bufs b, Phys ph)
+{
+	__shared__ double stage[TILE_CAP * TILE_ROW];
+	int4 tl = reinterpret_cast<const int4*>(b.tiles)[xcd_block()];  // first staged cell, count, first edge, count
+	for (int k = threadIdx.x; k < tl.y; k += BLOCK) {
+		int c = b.tcell[tl.x + k];
+		Prim pi;
+		int face[3];
+		Prim nb[3];
+		double g[8];
+		cell_gradient<MM, GR == 1 || GR == 3>(b, ph, c, pi, face, nb, g);
+		if (GR >= 3) minmod_limit(b, c, face, pi, nb, g);
+		double* r = stage + TILE_ROW * k;
+		r[0] = pi.r;
+		r[1] = pi.p;
+		r[2] = pi.u;
+		r[3] = pi.v;
+		for (int q = 0; q < 8; q++) r[4 + q] = g[q];
+	}
+	__syncthreads();
+	// lane pairs (t, t + 1) stay together: BLOCK is even
+	for (int t = threadIdx.x; t < 2 * tl.w; t += BLOCK) {
+		int e = tl.z + (t >> 1), gp = t & 1;
+		int2 ce = edge_cells(b, e);
+		bool inner = ce.y >= 0;
+		bool second = gp == 1 && inner;
+		unsigned sl = b.eslot[e];
+		const double* r = stage + TILE_ROW * (int)(second ? sl >> 16 : sl & 0xffffu);
+		Prim ps{r[0], r[1], r[2], r[3]};
+		double gs[8];
+		for (int q = 0; q < 8; q++) gs[q] = r[4 + q];
+		edge_lane<MM, FLUX, VISC>(b, ph, e, gp, ce.x, inner, second ? ce.y : ce.x, ps, gs, edge_geo(b, e)[0],
+		                          edge_geo(b, e)[1]);
+	}
 }
 
 // residual of cell i: -F on c1's side, +F on c2's, ascending edge order
@@ -797,8 +867,9 @@ struct cfd2d_ctx {
 	// timing
 	bool timing = false;
 	unsigned timing_mask = ~0u;  // kernel ids (bit CFD2D_K_*) that get events
-	int fuse_mode = -1;          // 1: the edge pass forms the gradients, 0: a separate
-	                             // gradient pass, -1: 1 without a limiter, 0 with one
+	int fuse_mode = -1;          // 1: the edge pass forms the gradients, 2: the tiled edge
+	                             // pass does, 0: a separate gradient pass, -1: 1 without
+	                             // a limiter, 0 with one
 	bool prefer_graph = false;   // cfd2d_step replays a recorded step (chosen by cfd2d_autotune)
 	int edge_shape = 0;          // launch shape of the edge pass (EdgeShape; chosen by cfd2d_autotune)
 	int stream_mode = -1;        // update pass reads/writes the state non-temporally: 1 yes,
@@ -996,6 +1067,31 @@ void launch_edge(cfd2d_ctx* c)
 // afterwards) and a separate pass forms them with one (three limiter
 // evaluations per cell cost more than the pass they save on large meshes);
 // cfd2d_fuse_gradient and cfd2d_autotune change the choice
+template <bool MM, bool VISC, int GR>
+void launch_edge_tile(cfd2d_ctx* c)
+{
+	dim3 g(c->b.ntiles), blk(BLOCK);
+	rec_start(c, CFD2D_K_EDGE_FLUX);
+	switch (c->p.flux) {
+	case CFD2D_FLUX_LAX: hipLaunchKernelGGL((k_edge_tile<MM, CFD2D_FLUX_LAX, VISC, GR>), g, blk, 0, c->st, c->b, c->ph); break;
+	case CFD2D_FLUX_HLLC: hipLaunchKernelGGL((k_edge_tile<MM, CFD2D_FLUX_HLLC, VISC, GR>), g, blk, 0, c->st, c->b, c->ph); break;
+	case CFD2D_FLUX_ROE: hipLaunchKernelGGL((k_edge_tile<MM, CFD2D_FLUX_ROE, VISC, GR>), g, blk, 0, c->st, c->b, c->ph); break;
+	default: hipLaunchKernelGGL((k_edge_tile<MM, CFD2D_FLUX_GODUNOV, VISC, GR>), g, blk, 0, c->st, c->b, c->ph); break;
+	}
+	rec_stop(c);
+}
+
+template <bool MM, bool VISC>
+void launch_edge_tiled(cfd2d_ctx* c, int gr)
+{
+	switch (gr) {
+	case 1: launch_edge_tile<MM, VISC, 1>(c); break;
+	case 2: launch_edge_tile<MM, VISC, 2>(c); break;
+	case 3: launch_edge_tile<MM, VISC, 3>(c); break;
+	default: launch_edge_tile<MM, VISC, 4>(c); break;
+	}
+}
+
 template <bool MM>
 void launch_gradient_edge(cfd2d_ctx* c)
 {
@@ -1008,6 +1104,11 @@ void launch_gradient_edge(cfd2d_ctx* c)
 	}
 	bool lsq = c->p.gradient == CFD2D_GRAD_LSQ;
 	int gr = (lsq ? 1 : 2) + (lim ? 2 : 0);
+	if (c->fuse_mode == 2 && c->b.ntiles > 0) {
+		if (c->p.viscous) launch_edge_tiled<MM, true>(c, gr);
+		else launch_edge_tiled<MM, false>(c, gr);
+		return;
+	}
 	if (c->p.viscous) {
 		switch (gr) {
 		case 1: launch_edge_v<MM, true, 1>(c); break;
@@ -1114,6 +1215,45 @@ int check_launch(const char* what)
 }
 
 // least-squares weights w_k = A^{-1} d_k, A = sum d d^T (the oracle's lsq_weights)
+// Tiles of the tiled edge pass (k_edge_tile): tiles of T consecutive cells,
+// each owning the edges whose c1 it holds, staging c1 and (inner edges) c2
+// of those edges.  False when the edges are not in c1 order or a tile would
+// stage more than TILE_CAP cells.
+bool build_tiles(const std::vector<int>& c1, const std::vector<int>& c2, int nc, int T, std::vector<int>& tiles,
+                 std::vector<int>& tcell, std::vector<unsigned>& eslot)
+{
+	int ne = (int)c1.size();
+	for (int e = 1; e < ne; e++)
+		if (c1[e] < c1[e - 1]) return false;
+	int nt = (nc + T - 1) / T;
+	tiles.assign(4 * (size_t)nt, 0);
+	tcell.clear();
+	eslot.assign(ne, 0u);
+	std::vector<int> need;
+	int e = 0;
+	for (int t = 0; t < nt; t++) {
+		int e0 = e;
+		while (e < ne && c1[e] < (t + 1) * T) e++;
+		need.clear();
+		for (int k = e0; k < e; k++) {
+			need.push_back(c1[k]);
+			if (c2[k] >= 0) need.push_back(c2[k]);
+		}
+		std::sort(need.begin(), need.end());
+		need.erase(std::unique(need.begin(), need.end()), need.end());
+		if ((int)need.size() > TILE_CAP) return false;
+		tiles[4 * (size_t)t] = (int)tcell.size();
+		tiles[4 * (size_t)t + 1] = (int)need.size();
+		tiles[4 * (size_t)t + 2] = e0;
+		tiles[4 * (size_t)t + 3] = e - e0;
+		auto slot = [&](int cell) { return (unsigned)(std::lower_bound(need.begin(), need.end(), cell) - need.begin()); };
+		for (int k = e0; k < e; k++) eslot[k] = slot(c1[k]) | ((c2[k] >= 0 ? slot(c2[k]) : 0xffffu) << 16);
+		tcell.insert(tcell.end(), need.begin(), need.end());
+	}
+	if (tcell.empty()) tcell.push_back(0);
+	return e == ne;
+}
+
 void lsq_weights(const Mesh& m, std::vector<double>& w)
 {
 	int nc = m.nc;
@@ -1369,6 +1509,22 @@ int cfd2d_ctx_create(const cfd2d_mesh* mesh, const cfd2d_params* p, int n_bc, co
 		}
 	} else {
 		b.lsqw = nullptr;
+	}
+	{
+		// the tiled edge pass: the largest tile whose staged set fits
+		std::vector<int> tiles, tcell;
+		std::vector<unsigned> eslot;
+		b.ntiles = 0;
+		for (int T : {TILE_CELLS, 192, 128, 96, 64})
+			if (build_tiles(m.c1, ec2, nc, T, tiles, tcell, eslot)) {
+				if ((rc = upload(c, &b.tiles, tiles)) || (rc = upload(c, &b.tcell, tcell)) ||
+				    (rc = upload(c, &b.eslot, eslot))) {
+					destroy_ctx(c);
+					return rc;
+				}
+				b.ntiles = (int)(tiles.size() / 4);
+				break;
+			}
 	}
 	hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
 	if (e != hipSuccess) {
@@ -1667,7 +1823,9 @@ int cfd2d_stream_state(cfd2d_ctx* c, int mode)
 
 int cfd2d_fuse_gradient(cfd2d_ctx* c, int mode)
 {
-	if (!c || mode < -1 || mode > 1) return fail(CFD2D_ERR_ARG, "cfd2d_fuse_gradient: mode is -1, 0 or 1");
+	if (!c || mode < -1 || mode > 2) return fail(CFD2D_ERR_ARG, "cfd2d_fuse_gradient: mode is -1, 0, 1 or 2");
+	if (mode == 2 && c->b.ntiles == 0)
+		return fail(CFD2D_ERR_ARG, "cfd2d_fuse_gradient: no tiled edge pass for this mesh (edges not in c1 order)");
 	c->fuse_mode = mode;
 	drop_graph(c);  // the recorded step holds the previous choice
 	return CFD2D_OK;
@@ -1715,14 +1873,16 @@ int cfd2d_autotune(cfd2d_ctx* c, int trial_steps, int* fuse_mode, int* use_graph
 	int shape0 = c->edge_shape;
 	int best_fuse = 0, best_graph = 0, best_stream = 0, best_shape = 0;
 	// the code-path variants with the default launch shape, then the launch
-	// shapes of the edge pass with the fastest of them
-	for (int v = 0; v < 8 + EDGE_SHAPES - 1 && err == hipSuccess; v++) {
-		int fuse = v & 1, graph = (v >> 1) & 1, strm = (v >> 2) & 1, shape = 0;
-		if (v >= 8) {
+	// shapes of the per-edge pass with the fastest of them
+	int nfuse = c->b.ntiles > 0 ? 3 : 2, nbase = 4 * nfuse;
+	for (int v = 0; v < nbase + EDGE_SHAPES - 1 && err == hipSuccess; v++) {
+		int fuse = v % nfuse, graph = (v / nfuse) & 1, strm = (v / nfuse / 2) & 1, shape = 0;
+		if (v >= nbase) {
+			if (best_fuse == 2) break;  // the tiled pass has one launch shape
 			fuse = best_fuse;
 			graph = best_graph;
 			strm = best_stream;
-			shape = v - 7;
+			shape = v - nbase + 1;
 		}
 		c->fuse_mode = fuse;
 		c->prefer_graph = graph != 0;

@@ -172,9 +172,12 @@ int  cfd2d_wall_force(cfd2d_ctx* c, const int* edge_mask, double* fx, double* fy
 /* per-kernel HIP-event timing on the context stream */
 int  cfd2d_timing(cfd2d_ctx* c, int enable);
 /* Where the cell gradients are formed: mode 1 in the edge pass itself (each
- * edge's lane pair forms its two cells'), 0 in a separate gradient pass
- * whose records the edge pass reads, -1 (default) 1 without a limiter and 0
- * with one.  All modes give the same bits.  No reference counterpart. */
+ * edge's lane pair forms its two cells'), 2 in the tiled edge pass (a
+ * workgroup per tile of consecutive cells forms the gradients its edges
+ * read once each, staged in LDS; CFD2D_ERR_ARG when the mesh's edges are
+ * not in c1 order), 0 in a separate gradient pass whose records the edge
+ * pass reads, -1 (default) 1 without a limiter and 0 with one.  All modes
+ * give the same bits.  No reference counterpart. */
 int  cfd2d_fuse_gradient(cfd2d_ctx* c, int mode);
 /* Whether the update pass reads and writes the conserved state with
  * non-temporal loads and stores (kept out of the caches, leaving them to the
