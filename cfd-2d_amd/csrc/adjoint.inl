@@ -44,9 +44,13 @@ struct AdjBufs {
 	double* Acc;    // [nc][4] sum of a * W over the stages of the current step
 	double* Pb;     // [4][nc] adjoint of the primitives
 	double* Gb;     // [8][nc] adjoint of the gradients
-	                // (separate arrays: held as per-cell records, the gfx950
+	                // (component arrays: held as per-cell records, the gfx950
 	                // build of k_adj_grad returned wrong boundary-cell values
-	                // at -O2/-O3, correct at -O1 or -fno-unroll-loops)
+	                // at -O2/-O3, correct at -O1 or -fno-unroll-loops; both
+	                // layouts compiled for the host run clean under ASan and
+	                // UBSan with bounds checks and agree bitwise, so the source
+	                // holds no out-of-bounds or uninitialised access that would
+	                // explain it -- tests/test_hostemu.py)
 	double* Ea;     // [16][ne]
 	double* Ebc;    // [8][ne] boundary-parameter adjoints per edge, summed over stages
 	                // (two slots of 4: one per Gauss point of the edge pass)
