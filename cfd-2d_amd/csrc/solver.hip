@@ -752,57 +752,98 @@ __global__ __launch_bounds__(BLOCK) void k_update(Bufs b, Phys ph)
 	}
 }
 
-// FVM_TVD::remediateLimCells (src/methods/fvm_tvd.cpp:464-499): one
-// workgroup finds the 256-cell blocks that hold LIM-flagged cells, then one
-// lane walks those cells in ascending order (the reference's sequential
-// sweep: a later cell sees values remediated earlier), replacing each by the
-// area-weighted mean of the c2 cells of its edges in edge-list order (which
-// may be the cell itself) and advancing its 32-step counter.
+// FVM_TVD::remediateLimCells (src/methods/fvm_tvd.cpp:464-499): each
+// LIM-flagged cell, in ascending order, is replaced by the area-weighted mean
+// of the c2 cells of its edges in edge-list order (which may be the cell
+// itself) and its 32-step counter advances.  The sweep is sequential in the
+// reference -- a later cell sees values remediated earlier -- and stays so
+// here, but only where it must: one workgroup finds the 256-cell blocks that
+// hold LIM cells (the update pass flags them); per such block all lanes
+// gather their flagged cells' faces, c2 cells, areas and states at once into
+// LDS (a block's earlier blocks are already stored), then one lane runs the
+// sequential part over LDS alone, taking a c2 cell's new value from LDS when
+// it is a flagged cell of the same block that comes earlier, and then all
+// lanes store states, flags and primitives.  The same operations in the
+// same order as the sequential sweep.
 template <bool MM>
 __global__ __launch_bounds__(BLOCK) void k_remediate(Bufs b, Phys ph)
 {
+	__shared__ int pos[BLOCK];          // a lane's position among the block's flagged cells, -1 if none
+	__shared__ int scan[BLOCK];
+	__shared__ int nj[BLOCK][3];        // per position: c2 of each face (-1: boundary)
+	__shared__ double ns[BLOCK][3];     // its area
+	__shared__ double nu[BLOCK][3][4];  // its state when gathered
+	__shared__ double nw[BLOCK][4];     // the remediated state
 	size_t n = b.nc;
+	int tid = threadIdx.x;
 	if (*(volatile int*)b.anyflag == 0) return;  // no LIM cell anywhere this step
 	for (int base = 0; base < b.nblocks; base += BLOCK) {
-		int blk = base + (int)threadIdx.x;
+		int blk = base + tid;
 		int mine = blk < b.nblocks ? b.blockflag[blk] : 0;
 		if (!__syncthreads_or(mine)) continue;
-		if (threadIdx.x == 0) {
-			int lim = min(base + BLOCK, b.nblocks);
-			for (int bb = base; bb < lim; bb++) {
-				if (!b.blockflag[bb]) continue;
-				int end = min((bb + 1) * BLOCK, b.nc);
-				for (int i = bb * BLOCK; i < end; i++) {
-					unsigned fl = b.flags[i];
-					if (!(fl & 0x2u)) continue;
-					double sr[4] = {0.0, 0.0, 0.0, 0.0}, S = 0.0;
-					int4 fc = load_faces(b, i);
-					int fs[3] = {fc.x, fc.y, fc.z};
-					for (int k = 0; k < 3; k++) {
-						int e = fs[k] >> 1;
-						int j = edge_cells(b, e).y;
-						if (j >= 0) {
-							double s = b.S[j];
-							S += s;
-							for (int q = 0; q < 4; q++) sr[q] += b.U[q * n + j] * s;
-						}
+		int lim = min(base + BLOCK, b.nblocks);
+		for (int bb = base; bb < lim; bb++) {
+			if (!b.blockflag[bb]) continue;  // the same value in every lane
+			int c0 = bb * BLOCK, i = c0 + tid;
+			unsigned fl = i < b.nc ? b.flags[i] : 0u;
+			int f = (fl & 0x2u) ? 1 : 0;
+			// positions: inclusive prefix sum of the flags over the block
+			scan[tid] = f;
+			__syncthreads();
+			for (int d = 1; d < BLOCK; d <<= 1) {
+				int v = tid >= d ? scan[tid - d] : 0;
+				__syncthreads();
+				scan[tid] += v;
+				__syncthreads();
+			}
+			int cnt = scan[BLOCK - 1], p = scan[tid] - 1;
+			pos[tid] = f ? p : -1;
+			if (f) {
+				int4 fc = load_faces(b, i);
+				int fs[3] = {fc.x, fc.y, fc.z};
+				for (int k = 0; k < 3; k++) {
+					int j = edge_cells(b, fs[k] >> 1).y;
+					nj[p][k] = j;
+					if (j >= 0) {
+						ns[p][k] = b.S[j];
+						for (int q = 0; q < 4; q++) nu[p][k][q] = b.U[q * n + j];
 					}
-					double u[4];
-					for (int q = 0; q < 4; q++) {
-						u[q] = sr[q] / S;
-						b.U[q * n + i] = u[q];
-					}
-					fl += 0x010000u;
-					if (fl & 0x200000u) fl &= 0x001110u;
-					b.flags[i] = fl;
-					MatV m = mat_of<MM>(b, ph, i);
-					store_prim(b, i, cons_to_prim(u[0], u[1], u[2], u[3], m));
 				}
 			}
+			__syncthreads();
+			if (tid == 0) {
+				for (int r = 0; r < cnt; r++) {
+					double sr[4] = {0.0, 0.0, 0.0, 0.0}, S = 0.0;
+					for (int k = 0; k < 3; k++) {
+						int j = nj[r][k];
+						if (j < 0) continue;
+						int jl = j - c0;  // a flagged cell of this block remediated before this one?
+						int pj = jl >= 0 && jl < BLOCK ? pos[jl] : -1;
+						const double* u = pj >= 0 && pj < r ? nw[pj] : nu[r][k];
+						double s = ns[r][k];
+						S += s;
+						for (int q = 0; q < 4; q++) sr[q] += u[q] * s;
+					}
+					for (int q = 0; q < 4; q++) nw[r][q] = sr[q] / S;
+				}
+			}
+			__syncthreads();
+			if (f) {
+				double u[4];
+				for (int q = 0; q < 4; q++) {
+					u[q] = nw[p][q];
+					b.U[q * n + i] = u[q];
+				}
+				fl += 0x010000u;
+				if (fl & 0x200000u) fl &= 0x001110u;
+				b.flags[i] = fl;
+				MatV m = mat_of<MM>(b, ph, i);
+				store_prim(b, i, cons_to_prim(u[0], u[1], u[2], u[3], m));
+			}
+			__syncthreads();  // the next block gathers what this one stored
 		}
-		__syncthreads();
 	}
-	if (threadIdx.x == 0) *b.anyflag = 0;
+	if (tid == 0) *b.anyflag = 0;
 }
 
 // FVM_TVD::calcTimeStep (src/methods/fvm_tvd.cpp:216-240): local step

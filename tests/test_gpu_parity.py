@@ -310,6 +310,31 @@ def test_benchmark_size_conservation_and_variants():
     s.close()
 
 
+def test_dense_remediation_matches_oracle():
+    """Many LIM cells at once: the bounds case with p_max = 0.99 on an
+    8 000-cell tube flags the whole high-pressure half (~4 000 cells, up to
+    201 of a 256-cell block, next to each other: a cell then averages
+    neighbours remediated earlier in the same sweep), through the 32-step
+    release.  States and flags equal the oracle's sequential sweep after
+    every step."""
+    import dataclasses
+    from cfd2d import meshgen
+    case = dataclasses.replace(gc.case("bounds", p_max=0.99),
+                               mesh=meshgen.rect_mesh(nx=200, ny=20, lx=1.0, ly=0.2, seed=9))
+    s = native.GpuSolver(case)
+    o = oracle.OracleSolver(case)
+    most = 0
+    for st in range(40):
+        s.run(1)
+        o.run(1)
+        np.testing.assert_array_equal(s.flags(), o.flags())
+        assert np.all(_rel(s.state(), o.state(), axis=1) <= 1e-11), st
+        most = max(most, int(np.count_nonzero(s.flags() & 0x2)))
+    print("most LIM cells in one step: %d" % most)
+    assert most >= 3000
+    s.close()
+
+
 def test_sod_full_with_minmod_runs_and_bounds():
     """The full Sod state (which hangs the reference's unlimited path) runs
     with the minmod limiter; density stays within the initial bounds."""
