@@ -163,7 +163,8 @@ def batched_vs_serial(n_cases, steps, seed=7):
     return dict(cases=n_cases, cells=cells, steps=steps, batched=rate(t_batch), serial_eager=rate(t_serial),
                 serial_graph=rate(t_graph), ratio_vs_eager=t_serial / t_batch, ratio_vs_graph=t_graph / t_batch,
                 unit="cell-updates/s",
-                autotune=dict(batch=tuned, serial_fused=sum(p["fuse"] for p in picks),
+                autotune=dict(batch=tuned, serial_fused=sum(p["fuse"] == 1 for p in picks),
+                              serial_tiled=sum(p["fuse"] == 2 for p in picks),
                               serial_graph=sum(p["graph"] for p in picks),
                               serial_stream=sum(p["stream"] for p in picks), serial_cases=len(picks)))
 
@@ -291,6 +292,31 @@ def renumbering(n, steps=10):
     return out
 
 
+def remediation_cost(s, counts=(0, 100, 10000), seed=1):
+    """k_remediate (FVM_TVD::remediateLimCells) on the benchmark's context:
+    for each count, that many cells chosen at random are LIM-flagged, one
+    step is run and the remediation launch timed (HIP events).  Randomly
+    scattered cells are the sweep's worst case: nearly every one sits in a
+    256-cell block of its own.  Changes the context's state (run last)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    U0 = s.state()
+    out = {}
+    for k in counts:
+        s.set_state(U0)
+        f = np.zeros(s.nc, np.uint32)
+        f[rng.choice(s.nc, k, replace=False)] = 0x2
+        s.set_flags(f)
+        s.timing(True, kernels=["remediate"])
+        s.step(1)
+        s.sync()
+        ms, n = s.timing_read()["remediate"]
+        s.timing(False)
+        out[str(k)] = round(1e3 * ms / max(n, 1), 1)
+    s.set_flags(np.zeros(s.nc, np.uint32))
+    return dict(lim_cells_to_us=out, workload="LIM cells scattered at random over the benchmark's mesh, one step")
+
+
 def implicit_assembly(s, reps=20, traffic=None):
     """The implicit method's matrix fill (cfd2d_implicit_assemble) on the
     context of the main workload: device-only assembly timed per kernel with
@@ -410,6 +436,8 @@ def main():
         tim[DOMINANT] = tim_timed[DOMINANT]
     hits, nonfinite = s.status()
     impl = None if args.no_implicit else implicit_assembly(s, traffic=load_traffic("periodic_ns_%d" % args.n))
+    remed = remediation_cost(s)
+    log("remediation: %s" % remed)
     s.close()
     value = s.nc * args.steps / dt
     per = {}
@@ -448,6 +476,7 @@ def main():
         "kernels": per,
         "newton_cap_hits": hits, "nonfinite": nonfinite,
     }
+    out["remediation"] = remed
     if impl:
         out["implicit_assembly"] = impl
         log("implicit assembly: %s" % impl)

@@ -224,7 +224,7 @@ def test_autotune_restores_state_and_keeps_results(name, opts):
     b.run(7)
     U, F = a.state(), a.flags()
     choice = a.autotune(trial_steps=2)
-    assert choice["fuse"] in (0, 1) and choice["graph"] in (0, 1) and choice["stream"] in (0, 1)
+    assert choice["fuse"] in (0, 1, 2) and choice["graph"] in (0, 1) and choice["stream"] in (0, 1)
     assert choice["edge_shape"] in range(4)
     assert choice["ms_per_step"] > 0
     assert np.array_equal(a.state(), U) and np.array_equal(a.flags(), F)
