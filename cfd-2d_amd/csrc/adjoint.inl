@@ -191,6 +191,49 @@ __device__ __forceinline__ void lax_vjp(const Prim& L, const Prim& R, double nx,
 	Rb[0] = Rrb; Rb[1] = Rpb; Rb[2] = Rub; Rb[3] = Rvb;
 }
 
+// The adjoint of one side only: F(L, R; n) = -F(R, L; -n) holds exactly for
+// the LAX flux (every term changes sign with n, alpha does not), so the
+// adjoint of the cell's own reconstructed state O against the other side's
+// X is lax_vjp's left part evaluated with (L, R, n, fb) = (O, X, s n, s fb),
+// s = +1 for c1 and -1 for c2.  own_c2 keeps the forward pass's choice of
+// the side whose wave speed is alpha when the two tie (the right one, c2).
+// About half of lax_vjp's arithmetic: the other side's adjoint is formed by
+// the other cell.
+__device__ __forceinline__ void lax_vjp_own(const Prim& L, const Prim& R, double nx, double ny, double G,
+                                            double igm1, const double* fb, bool own_c2, double* Lb)
+{
+	double unl = L.u * nx + L.v * ny, unr = R.u * nx + R.v * ny;
+	double irl = 1.0 / L.r, irr = 1.0 / R.r;
+	double cl = sqrt(G * L.p * irl), cr = sqrt(G * R.p * irr);
+	double al = fabs(unl) + cl, ar = fabs(unr) + cr;
+	bool own = own_c2 ? !(ar > al) : al > ar;  // the forward pass's al > ar, seen from this side
+	double alpha = own ? al : ar;
+	double kl = 0.5 * (L.u * L.u + L.v * L.v), kr = 0.5 * (R.u * R.u + R.v * R.v);
+	double rel = L.p * igm1 + L.r * kl, rer = R.p * igm1 + R.r * kr;
+	double frl = L.r * unl;
+	double h0 = 0.5 * fb[0], h1 = 0.5 * fb[1], h2 = 0.5 * fb[2], h3 = 0.5 * fb[3];
+	double alb = -(h0 * (R.r - L.r) + h1 * (R.r * R.u - L.r * L.u) + h2 * (R.r * R.v - L.r * L.v) + h3 * (rer - rel));
+	double frlb = h0 + h1 * L.u + h2 * L.v;
+	double unlb = h3 * (rel + L.p) + frlb * L.r;
+	double rolb = frlb * unl + h0 * alpha, rulb = h1 * alpha, rvlb = h2 * alpha, relb = h3 * (unl + alpha);
+	double Lub = h1 * frl, Lvb = h2 * frl;
+	double Lpb = h1 * nx + h2 * ny + h3 * unl, Lrb = 0.0;
+	if (own) {
+		double ts = alb * 0.5 * cl;
+		unlb += alb * (unl < 0 ? -1.0 : 1.0);
+		Lpb += ts / L.p;
+		Lrb += -ts * irl;
+	}
+	Lrb += rolb + rulb * L.u + rvlb * L.v + relb * kl;
+	Lub += rulb * L.r + relb * L.r * L.u + unlb * nx;
+	Lvb += rvlb * L.r + relb * L.r * L.v + unlb * ny;
+	Lpb += relb * igm1;
+	Lb[0] = Lrb;
+	Lb[1] = Lpb;
+	Lb[2] = Lub;
+	Lb[3] = Lvb;
+}
+
 // one-direction forward-mode number for the HLLC Jacobian
 struct D1 {
 	double v, d;
@@ -590,24 +633,34 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 			L = own;
 			R = ghost_of<MM>(b, ph, e, i, L, pi);
 		}
-		double Lb[4], Rb[4], nb[2] = {0.0, 0.0};
-		if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb, GEO ? nb : nullptr);
-		else lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb, GEO ? nb : nullptr);
-		if (j < 0) {
-			int bc = -edge_cells(b, e).y - 1;
-			double dL[4], dC[4], dpar[4], dn[2];
-			ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], nx, ny, L, pi, mat_of<MM>(b, ph, i), Rb, dL, dC, dpar, dn);
-			for (int q = 0; q < 4; q++) {
-				Lb[q] += dL[q];
-				pb[q] += dC[q];
-				a.Ebc[(size_t)(4 * gp + q) * ne + e] += dpar[q];
+		double Ob[4], nb[2] = {0.0, 0.0};  // Ob: adjoint of this cell's reconstructed state
+		if (FLUX == CFD2D_FLUX_LAX && !GEO && j >= 0) {
+			// an inner face without geometry terms: this side's adjoint only
+			double sg = side ? -1.0 : 1.0, fs[4];
+			for (int q = 0; q < 4; q++) fs[q] = sg * fb[q];
+			lax_vjp_own(own, side ? L : R, sg * nx, sg * ny, ph.gam_flux, ph.igm1, fs, side != 0, Ob);
+		} else {
+			double Lb[4], Rb[4];
+			if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb, GEO ? nb : nullptr);
+			else lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb, GEO ? nb : nullptr);
+			if (j < 0) {
+				int bc = -edge_cells(b, e).y - 1;
+				double dL[4], dC[4], dpar[4], dn[2];
+				ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], nx, ny, L, pi, mat_of<MM>(b, ph, i), Rb, dL, dC, dpar,
+				          dn);
+				for (int q = 0; q < 4; q++) {
+					Lb[q] += dL[q];
+					pb[q] += dC[q];
+					a.Ebc[(size_t)(4 * gp + q) * ne + e] += dpar[q];
+				}
+				nb[0] += dn[0];
+				nb[1] += dn[1];
 			}
-			nb[0] += dn[0];
-			nb[1] += dn[1];
+			for (int q = 0; q < 4; q++) Ob[q] = side ? Rb[q] : Lb[q];
 		}
 		double ddx = 0.0, ddy = 0.0;  // adjoint of this cell's offset to this Gauss point
 		for (int q = 0; q < 4; q++) {
-			double o = side ? Rb[q] : Lb[q];
+			double o = Ob[q];
 			pb[q] += o;
 			gb[2 * q] += o * dx;
 			gb[2 * q + 1] += o * dy;

@@ -770,6 +770,7 @@ __global__ __launch_bounds__(BLOCK) void k_remediate(Bufs b, Phys ph)
 {
 	__shared__ int pos[BLOCK];          // a lane's position among the block's flagged cells, -1 if none
 	__shared__ int scan[BLOCK];
+	__shared__ int bflag[BLOCK];        // the block flags of the current group of blocks
 	__shared__ int nj[BLOCK][3];        // per position: c2 of each face (-1: boundary)
 	__shared__ double ns[BLOCK][3];     // its area
 	__shared__ double nu[BLOCK][3][4];  // its state when gathered
@@ -780,10 +781,12 @@ __global__ __launch_bounds__(BLOCK) void k_remediate(Bufs b, Phys ph)
 	for (int base = 0; base < b.nblocks; base += BLOCK) {
 		int blk = base + tid;
 		int mine = blk < b.nblocks ? b.blockflag[blk] : 0;
+		bflag[tid] = mine;
+		__syncthreads();
 		if (!__syncthreads_or(mine)) continue;
 		int lim = min(base + BLOCK, b.nblocks);
 		for (int bb = base; bb < lim; bb++) {
-			if (!b.blockflag[bb]) continue;  // the same value in every lane
+			if (!bflag[bb - base]) continue;  // the same value in every lane
 			int c0 = bb * BLOCK, i = c0 + tid;
 			unsigned fl = i < b.nc ? b.flags[i] : 0u;
 			int f = (fl & 0x2u) ? 1 : 0;

@@ -335,6 +335,27 @@ def test_dense_remediation_matches_oracle():
     s.close()
 
 
+def test_sod_tube_config_matches_oracle():
+    """The CPU-runnable benchmark configuration end to end: the 8 000-cell
+    jittered tube with the full Sod state, exact Godunov flux, Green-Gauss
+    gradients with the minmod limiter, Heun RK2, CFL 0.5, 1 000 steps, on the
+    GPU against the oracle.  States within 1e-9 relative L-inf per component
+    (the tolerance of the unlimited 1 000-step golden gate), flags equal,
+    no Newton cap hit."""
+    case = cases.shock_tube(200, 20, seed=1, mild=False, limiter="MINMOD", cfl=0.5, max_newton=500)
+    s = native.GpuSolver(case)
+    o = oracle.OracleSolver(case)
+    for n in (10, 90, 900):
+        s.run(n)
+        o.run(n)
+        err = _rel(s.state(), o.state(), axis=1)
+        assert np.all(err <= 1e-9), (n, err)
+        np.testing.assert_array_equal(s.flags(), o.flags())
+    assert s.status() == (0, 0) and o.newton_cap_hits() == 0
+    print("Sod tube after 1000 steps: max rel err per component", _rel(s.state(), o.state(), axis=1))
+    s.close()
+
+
 def test_sod_full_with_minmod_runs_and_bounds():
     """The full Sod state (which hangs the reference's unlimited path) runs
     with the minmod limiter; density stays within the initial bounds."""
