@@ -269,14 +269,17 @@ constexpr unsigned hipStreamNonBlocking = 1;
 constexpr unsigned hipHostMallocDefault = 0;
 
 namespace hostemu {
+struct Stream;
 struct Graph {
 	std::vector<std::function<void()>> ops;
+	std::vector<Stream*> joined;  // streams that joined the capture by waiting on its events
 };
 struct Stream {
 	Graph* capture = nullptr;
 };
 struct Event {
 	std::chrono::steady_clock::time_point t;
+	Graph* capture = nullptr;  // recorded into this capture
 };
 inline Stream g_null_stream;
 inline Stream* stream_of(Stream* s) { return s ? s : &g_null_stream; }
@@ -386,6 +389,8 @@ inline hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t* g)
 	hostemu::Stream* q = hostemu::stream_of(s);
 	if (!q->capture) return hipErrorInvalidValue;
 	*g = q->capture;
+	for (auto* j : q->capture->joined) j->capture = nullptr;
+	q->capture->joined.clear();
 	q->capture = nullptr;
 	return hipSuccess;
 }
@@ -419,9 +424,24 @@ inline hipError_t hipEventDestroy(hipEvent_t e)
 	delete e;
 	return hipSuccess;
 }
+inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
+constexpr unsigned hipEventDisableTiming = 2;
+// one in-order queue: an event is complete once recorded, except that a
+// stream waiting on an event recorded into a capture joins that capture
+// (its later work is recorded into the same graph, in enqueue order)
 inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t s = nullptr)
 {
+	e->capture = hostemu::stream_of(s)->capture;
 	hostemu::submit(s, [=] { e->t = std::chrono::steady_clock::now(); });
+	return hipSuccess;
+}
+inline hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned)
+{
+	hostemu::Stream* q = hostemu::stream_of(s);
+	if (e->capture && q->capture != e->capture) {
+		q->capture = e->capture;
+		e->capture->joined.push_back(q);
+	}
 	return hipSuccess;
 }
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
