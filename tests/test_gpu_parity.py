@@ -310,6 +310,33 @@ def test_benchmark_size_conservation_and_variants():
     s.close()
 
 
+def test_benchmark_size_matches_oracle():
+    """The benchmark's own case at full size (2,097,152 cells, periodic
+    Navier-Stokes, LSQ, exact Godunov, SSP-RK3, spectral dt) against the
+    oracle: the residual of the initial state within the 1e-12 gate of
+    test_residual_1e12 and the states after two steps within
+    1e-11 relative per component; the tiled edge pass gives the same bits."""
+    case = cases.periodic_pulse(1024, seed=2)
+    s = native.GpuSolver(case)
+    o = oracle.OracleSolver(case)
+    assert abs(s.tau()[0] - o.tau()[0]) <= 1e-13 * o.tau()[0]
+    R, Ro = s.residual(), o.residual()
+    den = np.maximum(np.abs(Ro), residual_scale(s.mesh.export()["cell_edges"], o.edge_fluxes())[None, :])
+    rerr = float(np.max(np.abs(R - Ro) / den))
+    assert rerr <= 1e-12, rerr
+    s.run(2)
+    o.run(2)
+    err = _rel(s.state(), o.state(), axis=1)
+    print("2.1 M cells: residual %.2e, after 2 steps max rel err per component %s" % (rerr, err))
+    assert np.all(err <= 1e-11), err
+    U2 = s.state()
+    s.set_state(case.initial_state())
+    s.fuse_gradient(2)
+    s.run(2)
+    assert np.array_equal(s.state(), U2)
+    s.close()
+
+
 def test_dense_remediation_matches_oracle():
     """Many LIM cells at once: the bounds case with p_max = 0.99 on an
     8 000-cell tube flags the whole high-pressure half (~4 000 cells, up to
