@@ -337,6 +337,35 @@ def test_benchmark_size_matches_oracle():
     s.close()
 
 
+@pytest.mark.parametrize("nx,ny", [(1, 1), (2, 1), (3, 2)])
+@pytest.mark.parametrize("opts", [dict(), dict(limiter="MINMOD", gradient="LSQ", rk_stages=3),
+                                  dict(viscous=True, flux="HLLC", cfl_mode="SPECTRAL", cfl=0.4)], ids=str)
+def test_tiny_meshes_match_oracle(nx, ny, opts):
+    """Meshes of 2 to 12 cells, where every cell touches the boundary (the
+    inlet, the outlet and both walls meet in one cell, or every edge of a
+    cell is a boundary edge) and every launch is a fraction of one
+    workgroup: 15 steps against the oracle for the three gradient
+    placements (separate pass, per-edge, tiled), which agree bitwise."""
+    import dataclasses
+    from cfd2d import meshgen
+    base = gc.case("inlet", **opts)
+    case = dataclasses.replace(base, mesh=meshgen.rect_mesh(nx=nx, ny=ny, lx=1.0, ly=0.2, seed=3))
+    o = oracle.OracleSolver(case)
+    o.run(15)
+    outs = []
+    for mode in (0, 1, 2):
+        s = native.GpuSolver(case)
+        s.fuse_gradient(mode)
+        s.run(15)
+        outs.append(s.state())
+        assert s.status() == (0, 0)
+        np.testing.assert_array_equal(s.flags(), o.flags())
+        s.close()
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    err = _rel(outs[0], o.state(), axis=1)
+    assert np.all(err <= 1e-11), err
+
+
 def test_dense_remediation_matches_oracle():
     """Many LIM cells at once: the bounds case with p_max = 0.99 on an
     8 000-cell tube flags the whole high-pressure half (~4 000 cells, up to

@@ -126,3 +126,23 @@ def test_fluxes_match_oracle(kind):
             assert err[~same].max() <= 1e-8
     else:
         assert err.max() <= 1e-12
+
+
+def test_empty_and_invalid_inputs():
+    """No rows is a valid call (nothing evaluated, no cap hit); an unknown
+    flux kind, gamma <= 1 and too many rows for the no-allocation entry
+    point are errors, reported as CFD2D_ERR_ARG without touching the device."""
+    out, its, hits = native.riemann(np.zeros((0, 8)), GAM)
+    assert out.shape == (0, 5) and its.shape == (0,) and hits == 0
+    for kind in ("GODUNOV", "LAX", "HLLC", "ROE"):
+        F, hits = native.flux(kind, np.zeros((0, 10)), GAM)
+        assert F.shape == (0, 4) and hits == 0
+        F, hits = native.flux(kind, np.zeros((0, 10)), GAM, noalloc=True)
+        assert F.shape == (0, 4) and hits == 0
+    row = random_rows(1, 1)
+    with pytest.raises(native.Cfd2dError):
+        native.flux(7, row, GAM)
+    with pytest.raises(native.Cfd2dError):
+        native.flux("LAX", row, 1.0)
+    with pytest.raises(native.Cfd2dError):
+        native.flux("LAX", np.repeat(row, native.NOALLOC_MAX_ROWS + 1, axis=0), GAM, noalloc=True)
