@@ -274,6 +274,27 @@ def test_viscous_periodic_matches_oracle():
     s.close()
 
 
+def test_periodic_ns_1000_steps_matches_oracle():
+    """The benchmark's scheme over the benchmark's horizon: periodic
+    Navier-Stokes (LSQ gradients, exact Godunov flux, SSP-RK3, spectral dt,
+    mu = 1e-4, Pr = 0.72) on 18 432 cells for 1 000 steps against the
+    oracle: conserved variables within 1e-9 relative L-inf per component
+    (the benchmark's parity target), checked after 10, 100 and 1 000 steps."""
+    case = cases.periodic_pulse(96, seed=2)
+    s = native.GpuSolver(case)
+    o = oracle.OracleSolver(case)
+    done = 0
+    for n, tol in ((10, 1e-11), (100, 1e-11), (1000, 1e-9)):
+        s.run(n - done)
+        o.run(n - done)
+        done = n
+        err = _rel(s.state(), o.state(), axis=1)
+        assert np.all(err <= tol), (n, err)
+    print("periodic NS 18 432 cells after 1000 steps: max rel err per component", err)
+    assert s.status() == (0, 0) and o.newton_cap_hits() == 0
+    s.close()
+
+
 def test_periodic_conservation():
     """Periodic torus: sum S*U is conserved to round-off."""
     case = cases.periodic_pulse(32, seed=6)
