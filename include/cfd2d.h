@@ -192,7 +192,8 @@ int  cfd2d_stream_state(cfd2d_ctx* c, int mode);
  * single-material contexts' unlimited and stored-gradient edge passes. */
 int  cfd2d_edge_shape(cfd2d_ctx* c, int shape);
 /* Autotuner (no reference counterpart): times trial_steps steps of each
- * variant -- gradients in the edge pass or in their own pass, eager launches
+ * variant -- gradients in their own pass, in the edge pass or in the tiled
+ * edge pass (when the mesh has tiles), eager launches
  * or a replayed recorded step, cached or streamed state in the update pass
  * -- on the context's current state, then the edge pass's launch shapes
  * (cfd2d_edge_shape) with the fastest of them, restores that state exactly,
@@ -219,9 +220,11 @@ int  cfd2d_ctx_stream(cfd2d_ctx* c, void** hip_stream);
  * skips either).  Forward states are kept every ckpt_every steps and each
  * segment but the last is recomputed once in the reverse sweep; ckpt_every
  * <= 0 picks the largest interval whose stage records (96 B per cell per RK
- * stage) fit in half of the free device memory.  Supports FLUX LAX and
- * HLLC, no limiter, inviscid, unsteady; the time step is held fixed.  The
- * context is left at the final state, as after cfd2d_step(c, nsteps). */
+ * stage) fit in half of the free device memory.  Supports FLUX LAX, HLLC
+ * and GODUNOV (the exact solver differentiated through its Newton
+ * iterations as evaluated), no limiter, inviscid, unsteady; the time step is
+ * held fixed.  The context is left at the final state, as after
+ * cfd2d_step(c, nsteps). */
 int  cfd2d_adjoint(cfd2d_ctx* c, int nsteps, int ckpt_every, const int* edge_mask, double* J, double* dJ_dU0,
                    double* dJ_dbc);
 /* cfd2d_adjoint plus dJ_dxy = dJ/d(node coordinates) as double[n_nodes][2]
