@@ -4,10 +4,12 @@
 // Objective J = sum over the masked edges of p(c1) * l * n_y after nsteps
 // steps (the pressure force of FVM_TVD_IMPLICIT::calcLiftForce,
 // src/methods/fvm_tvd_implicit.cpp:1088-1110, y component).  The reverse
-// sweep gives dJ/dU0 (every cell) and dJ/d(boundary parameters).  Scope:
-// LAX or HLLC flux, Green-Gauss or LSQ gradients, Heun RK2 or SSP-RK3, all
-// boundary types, periodic pairing; no limiter, no viscous terms; the time
-// step is held at the value in effect (it is not differentiated).
+// sweep gives dJ/dU0 (every cell), dJ/d(boundary parameters) and, on
+// request, the per-edge and per-cell geometry adjoints the host maps to node
+// coordinates.  Scope: LAX, HLLC or the exact Godunov flux, Green-Gauss or
+// LSQ gradients, Heun RK2 or SSP-RK3, all boundary types, periodic pairing;
+// no limiter, no viscous terms; the time step is held at the value in effect
+// (it is not differentiated).
 //
 // Per RK stage with input state U_s, U_{s+1} = a U0 + b (U_s + tau/S R(U_s)):
 //   given W = dJ/dU_{s+1}:  acc0 += a W,  dJ/dU_s = b W + (dR/dU_s)^T (b tau/S W)
@@ -18,12 +20,13 @@
 // per edge, one Gauss point each, into a per-edge buffer for both sides) then
 // k_adj_cell (each cell gathers its faces' entries in ascending edge order);
 // for LAX k_adj_face (each cell evaluates its three faces from its own side,
-// no edge buffer).  Then k_adj_grad: reverse of the gradient operator (own
+// no edge buffer; with geometry terms k_adj_edge's geometry-only mode beside
+// it).  Then k_adj_grad: reverse of the gradient operator (own
 // and neighbours' gradient adjoints, ghost terms), the primitive->conserved
 // Jacobian and the RK bookkeeping.  Boundary-parameter
 // adjoints accumulate per boundary edge and are summed per boundary object
-// once, in edge order.  No floating-point atomics: the result does not depend
-// on scheduling.
+// once, in an order fixed by the launch shape.  No floating-point atomics:
+// the result does not depend on scheduling.
 //
 // Storage: the forward kernels write each stage's P and G into per-stage
 // slots (the update pass writes the next stage's P and, while recording,
@@ -56,7 +59,8 @@ struct AdjBufs {
 	                // (two slots of 4: one per Gauss point of the edge pass)
 	double* bcbar;  // [nbc][4]
 	const int* cslot;   // [3][nc] slot of this cell in its neighbour's face list
-	const int* bedges;  // boundary edges in ascending order
+	const int* bedges;  // boundary edges grouped by boundary object, each run ascending
+	const int* bcoff;   // [nbc + 1] start of each object's run in bedges
 	int nbedges;
 	// geometry adjoints, summed over every stage (GEO kernels only; each slot
 	// is added to by one thread per kernel):
@@ -139,7 +143,8 @@ __device__ __forceinline__ void ghost_vjp(int type, const double* par, double nx
 // Written for few divisions: 1/r per side, and dc/dp = c/(2p),
 // dc/dr = -c/(2r) on the side that sets the wave speed.
 __device__ __forceinline__ void lax_vjp(const Prim& L, const Prim& R, double nx, double ny, double G, double igm1,
-                                        const double* fb, double* Lb, double* Rb, double* nb = nullptr)
+                                        const double* fb, double* Lb, double* Rb, double* nb = nullptr,
+                                        double* fo = nullptr)
 {
 	double unl = L.u * nx + L.v * ny, unr = R.u * nx + R.v * ny;
 	double irl = 1.0 / L.r, irr = 1.0 / R.r;
@@ -189,6 +194,12 @@ __device__ __forceinline__ void lax_vjp(const Prim& L, const Prim& R, double nx,
 	Rpb += rerb * igm1;
 	Lb[0] = Lrb; Lb[1] = Lpb; Lb[2] = Lub; Lb[3] = Lvb;
 	Rb[0] = Rrb; Rb[1] = Rpb; Rb[2] = Rub; Rb[3] = Rvb;
+	if (fo) {  // the flux itself (flux_lax's expressions on these intermediates)
+		fo[0] = 0.5 * (frr + frl - alpha * (R.r - L.r));
+		fo[1] = 0.5 * (frr * R.u + frl * L.u + (R.p + L.p) * nx - alpha * (R.r * R.u - L.r * L.u));
+		fo[2] = 0.5 * (frr * R.v + frl * L.v + (R.p + L.p) * ny - alpha * (R.r * R.v - L.r * L.v));
+		fo[3] = 0.5 * ((rer + R.p) * unr + (rel + L.p) * unl - alpha * (rer - rel));
+	}
 }
 
 // The adjoint of one side only: F(L, R; n) = -F(R, L; -n) holds exactly for
@@ -493,9 +504,14 @@ __device__ __forceinline__ void hllc_vjp(const Prim& L, const Prim& R, double nx
 // with Rb = scale * tau/S * W; each lane takes one Gauss point: the flux VJP,
 // the ghost VJP on a boundary, then its (r, p, u, v) adjoints on both sides.
 // Each lane adds its boundary-parameter adjoints into its own Ebc slot.
-template <bool MM, int FLUX, bool GEO>
+// GONLY (LAX with geometry terms, beside k_adj_face, which forms the (P, G)
+// and parameter adjoints): only the geometry terms, which need both sides'
+// adjoints of the face, and the stage's flux, which the gradient pass's
+// geometry terms read -- no Ea or Ebc.
+template <bool MM, int FLUX, bool GEO, bool GONLY = false>
 __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, double scale)
 {
+	static_assert(!GONLY || (GEO && FLUX == CFD2D_FLUX_LAX), "geometry-only mode: LAX with geometry terms");
 	int t = xcd_block() * BLOCK + threadIdx.x;
 	int e = t >> 1, gp = t & 1;
 	if (e >= b.ne) return;  // both lanes of a pair leave together
@@ -526,7 +542,17 @@ __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, 
 	double Lb[4], Rb[4], nb[2] = {0.0, 0.0};
 	if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb, GEO ? nb : nullptr);
 	else if (FLUX == CFD2D_FLUX_GODUNOV) godunov_vjp(L, R, nx, ny, ph.rk, ph.max_newton, fb, Lb, Rb, GEO ? nb : nullptr);
-	else lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb, GEO ? nb : nullptr);
+	else {
+		double f[4];
+		lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb, GEO ? nb : nullptr, GONLY ? f : nullptr);
+		if (GONLY) {
+			// the stage's flux as the edge pass stores it, for the gradient
+			// pass's edge-length and area terms
+			double o[4];
+			for (int q = 0; q < 4; q++) o[q] = __shfl_xor(f[q], 1);
+			store_edge_flux(b.F, e, gp, f, o, hl);
+		}
+	}
 	double Rsb[4];  // adjoint of c2's reconstructed state (inner edges)
 	if (c2 < 0) {
 		// the ghost's inputs: the reconstructed L, the cell centre, the parameters
@@ -537,17 +563,18 @@ __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, 
 			Lb[q] += dL[q];
 			Rsb[q] = 0.0;
 			Rb[q] = dC[q];
-			a.Ebc[(size_t)(4 * gp + q) * ne + e] += dpar[q];
+			if (!GONLY) a.Ebc[(size_t)(4 * gp + q) * ne + e] += dpar[q];
 		}
 		nb[0] += dn[0];
 		nb[1] += dn[1];
 	} else {
 		for (int q = 0; q < 4; q++) Rsb[q] = Rb[q];
 	}
-	for (int q = 0; q < 4; q++) {
-		a.Ea[(size_t)(4 * gp + q) * ne + e] = Lb[q];
-		a.Ea[(size_t)(8 + 4 * gp + q) * ne + e] = Rb[q];
-	}
+	if (!GONLY)
+		for (int q = 0; q < 4; q++) {
+			a.Ea[(size_t)(4 * gp + q) * ne + e] = Lb[q];
+			a.Ea[(size_t)(8 + 4 * gp + q) * ne + e] = Rb[q];
+		}
 	if (GEO) {
 		// this lane's Gauss point: the normal, and each side's offset (the
 		// reconstruction phi + g . d gives d-bar = sum_q phi-bar_q g_q)
@@ -580,8 +607,10 @@ __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, 
 // arithmetic of k_adj_edge, but no per-edge buffer and no second gather:
 // (P, G) adjoints go straight to Pb, Gb.  Two lanes per cell, one Gauss point
 // each, summed by a lane shuffle.  A cell owns its boundary faces and adds
-// their parameter adjoints into Ebc (one slot per Gauss point).
-template <bool MM, int FLUX, bool GEO>
+// their parameter adjoints into Ebc (one slot per Gauss point).  The
+// geometry terms, which need both sides' adjoints of a face, are formed by
+// edge (k_adj_edge's geometry-only mode).
+template <bool MM, int FLUX>
 __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, double scale)
 {
 	int t = xcd_block() * BLOCK + threadIdx.x;
@@ -633,48 +662,34 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 			L = own;
 			R = ghost_of<MM>(b, ph, e, i, L, pi);
 		}
-		double Ob[4], nb[2] = {0.0, 0.0};  // Ob: adjoint of this cell's reconstructed state
-		if (FLUX == CFD2D_FLUX_LAX && !GEO && j >= 0) {
-			// an inner face without geometry terms: this side's adjoint only
+		double Ob[4];  // adjoint of this cell's reconstructed state
+		if (FLUX == CFD2D_FLUX_LAX && j >= 0) {
+			// an inner face: this side's adjoint only
 			double sg = side ? -1.0 : 1.0, fs[4];
 			for (int q = 0; q < 4; q++) fs[q] = sg * fb[q];
 			lax_vjp_own(own, side ? L : R, sg * nx, sg * ny, ph.gam_flux, ph.igm1, fs, side != 0, Ob);
 		} else {
 			double Lb[4], Rb[4];
-			if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb, GEO ? nb : nullptr);
-			else lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb, GEO ? nb : nullptr);
+			if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb);
+			else lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb);
 			if (j < 0) {
 				int bc = -edge_cells(b, e).y - 1;
-				double dL[4], dC[4], dpar[4], dn[2];
+				double dL[4], dC[4], dpar[4];
 				ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], nx, ny, L, pi, mat_of<MM>(b, ph, i), Rb, dL, dC, dpar,
-				          dn);
+				          nullptr);
 				for (int q = 0; q < 4; q++) {
 					Lb[q] += dL[q];
 					pb[q] += dC[q];
 					a.Ebc[(size_t)(4 * gp + q) * ne + e] += dpar[q];
 				}
-				nb[0] += dn[0];
-				nb[1] += dn[1];
 			}
 			for (int q = 0; q < 4; q++) Ob[q] = side ? Rb[q] : Lb[q];
 		}
-		double ddx = 0.0, ddy = 0.0;  // adjoint of this cell's offset to this Gauss point
 		for (int q = 0; q < 4; q++) {
 			double o = Ob[q];
 			pb[q] += o;
 			gb[2 * q] += o * dx;
 			gb[2 * q + 1] += o * dy;
-			ddx += o * gi[2 * q];
-			ddy += o * gi[2 * q + 1];
-		}
-		if (GEO) {
-			// the face's normal is c1's to account for (both cells evaluate it)
-			if (side == 0) {
-				a.Ge[(size_t)(2 * gp) * ne + e] += nb[0];
-				a.Ge[(size_t)(2 * gp + 1) * ne + e] += nb[1];
-			}
-			a.Ge[(size_t)(4 + 4 * side + 2 * gp) * ne + e] += ddx;
-			a.Ge[(size_t)(5 + 4 * side + 2 * gp) * ne + e] += ddy;
 		}
 	}
 	for (int q = 0; q < 4; q++) pb[q] += __shfl_xor(pb[q], 1);
@@ -863,29 +878,27 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 	if (MODE != 2) store_records<4>(a.Acc, c0, nv, acc, lds_a);
 }
 
-// boundary-parameter adjoints: one workgroup sums the per-edge entries of
-// each boundary object in ascending edge order
-__global__ __launch_bounds__(BLOCK) void k_bc_reduce(Bufs b, AdjBufs a, int nbc)
+// boundary-parameter adjoints: one workgroup per boundary object sums the
+// per-edge entries of that object's run of edges, in an order fixed by the
+// launch shape (strided partial sums, then a tree over the workgroup)
+__global__ __launch_bounds__(BLOCK) void k_bc_reduce(Bufs b, AdjBufs a)
 {
-	__shared__ double part[BLOCK][4];
+	__shared__ double part[4][BLOCK];
 	size_t ne = b.ne;
-	for (int bc = 0; bc < nbc; bc++) {
-		double s[4] = {0, 0, 0, 0};
-		for (int k = threadIdx.x; k < a.nbedges; k += BLOCK) {
-			int e = a.bedges[k];
-			if (-edge_cells(b, e).y - 1 != bc) continue;
-			for (int q = 0; q < 4; q++) s[q] += a.Ebc[(size_t)q * ne + e] + a.Ebc[(size_t)(4 + q) * ne + e];
-		}
-		for (int q = 0; q < 4; q++) part[threadIdx.x][q] = s[q];
-		__syncthreads();
-		if (threadIdx.x == 0) {
-			double t[4] = {0, 0, 0, 0};
-			for (int k = 0; k < BLOCK; k++)
-				for (int q = 0; q < 4; q++) t[q] += part[k][q];
-			for (int q = 0; q < 4; q++) a.bcbar[4 * bc + q] = t[q];
-		}
+	int bc = blockIdx.x, t = threadIdx.x;
+	double s[4] = {0, 0, 0, 0};
+	for (int k = a.bcoff[bc] + t; k < a.bcoff[bc + 1]; k += BLOCK) {
+		int e = a.bedges[k];
+		for (int q = 0; q < 4; q++) s[q] += a.Ebc[(size_t)q * ne + e] + a.Ebc[(size_t)(4 + q) * ne + e];
+	}
+	for (int q = 0; q < 4; q++) part[q][t] = s[q];
+	__syncthreads();
+	for (int h = BLOCK / 2; h > 0; h >>= 1) {
+		if (t < h)
+			for (int q = 0; q < 4; q++) part[q][t] += part[q][t + h];
 		__syncthreads();
 	}
+	if (t < 4) a.bcbar[4 * bc + t] = part[t][0];
 }
 
 // [n][4] records -> [4][n] arrays
@@ -995,7 +1008,7 @@ int adj_topology(cfd2d_ctx* c)
 	HIPCHK(hipMemcpy(cface.data(), c->b.cface, cface.size() * sizeof(int), hipMemcpyDeviceToHost));
 	HIPCHK(hipMemcpy(cnb.data(), c->b.cnb, cnb.size() * sizeof(int), hipMemcpyDeviceToHost));
 	HIPCHK(hipMemcpy(eg.data(), c->b.eg, eg.size() * sizeof(double), hipMemcpyDeviceToHost));
-	std::vector<int> cslot(3 * (size_t)nc, 0), bed;
+	std::vector<int> cslot(3 * (size_t)nc, 0), bed, bcoff((size_t)c->nbc + 1, 0);
 	for (int i = 0; i < nc; i++)
 		for (int k = 0; k < 3; k++) {
 			int j = cnb[4 * (size_t)i + k];
@@ -1007,14 +1020,30 @@ int adj_topology(cfd2d_ctx* c)
 			if (found < 0) return fail(CFD2D_ERR_MESH, "adjoint: inconsistent face lists");
 			cslot[(size_t)k * nc + i] = found;
 		}
+	// boundary edges grouped by object (a counting sort, stable in the edge
+	// index); edges of an object index at or beyond nbc carry no parameter
+	std::vector<int> ebc(ne, -1);
 	for (int e = 0; e < ne; e++) {
 		int cells[2];
 		memcpy(cells, &eg[4 * (size_t)e + 3], sizeof(cells));
-		if (cells[1] < 0) bed.push_back(e);
+		int bc = -cells[1] - 1;
+		if (cells[1] < 0 && bc < c->nbc) {
+			ebc[e] = bc;
+			bcoff[bc + 1]++;
+		}
+	}
+	for (int k = 0; k < c->nbc; k++) bcoff[k + 1] += bcoff[k];
+	bed.resize(bcoff[c->nbc]);
+	{
+		std::vector<int> fill(bcoff.begin(), bcoff.end() - 1);
+		for (int e = 0; e < ne; e++)
+			if (ebc[e] >= 0) bed[fill[ebc[e]]++] = e;
 	}
 	int rc;
-	if ((rc = upload(c, &c->adj_cslot, cslot)) || (rc = upload(c, &c->adj_bedges, bed))) return rc;
-	c->adj_nbedges = (int)bed.size();
+	if ((rc = upload(c, &c->adj_cslot, cslot)) || (rc = upload(c, &c->adj_bedges, bed)) ||
+	    (rc = upload(c, &c->adj_bcoff, bcoff)))
+		return rc;
+	c->adj_nbedges = bcoff[c->nbc];
 	return 0;
 }
 
@@ -1070,7 +1099,9 @@ void launch_adj_grad(cfd2d_ctx* c, AdjWork* A, double wa, double wb, int mode)
 template <bool MM, bool GEO>
 void launch_adj_stage(cfd2d_ctx* c, AdjWork* A, double wa, double wb, int mode)
 {
-	if (GEO) launch_edge<MM>(c);
+	// the geometry terms read the stage's fluxes: for LAX the geometry-only
+	// edge reverse forms them, the other fluxes get them from the edge pass
+	if (GEO && c->p.flux != CFD2D_FLUX_LAX) launch_edge<MM>(c);
 	if (c->p.flux == CFD2D_FLUX_HLLC || c->p.flux == CFD2D_FLUX_GODUNOV) {
 		// VALU-heavy VJPs (forward-mode directions): once per edge, through the edge buffer
 		if (c->p.flux == CFD2D_FLUX_HLLC)
@@ -1081,9 +1112,13 @@ void launch_adj_stage(cfd2d_ctx* c, AdjWork* A, double wa, double wb, int mode)
 			                   c->st, c->b, c->ph, A->a, wb);
 		hipLaunchKernelGGL(k_adj_cell, dim3(grid_of(c->nc)), dim3(BLOCK), 0, c->st, c->b, A->a);
 	} else {
-		// cheap VJP: evaluated from both sides, no edge buffer
-		hipLaunchKernelGGL((k_adj_face<MM, CFD2D_FLUX_LAX, GEO>), dim3(grid_of(2 * c->nc)), dim3(BLOCK), 0, c->st,
+		// cheap VJP: evaluated from both sides, no edge buffer; the geometry
+		// terms (both sides' adjoints per edge) and the stage's fluxes by edge
+		hipLaunchKernelGGL((k_adj_face<MM, CFD2D_FLUX_LAX>), dim3(grid_of(2 * c->nc)), dim3(BLOCK), 0, c->st,
 		                   c->b, c->ph, A->a, wb);
+		if (GEO)
+			hipLaunchKernelGGL((k_adj_edge<MM, CFD2D_FLUX_LAX, true, true>), dim3(grid_of(2 * c->ne)), dim3(BLOCK), 0,
+			                   c->st, c->b, c->ph, A->a, wb);
 	}
 	if (c->p.gradient == CFD2D_GRAD_LSQ) launch_adj_grad<MM, true, GEO>(c, A, wa, wb, mode);
 	else launch_adj_grad<MM, false, GEO>(c, A, wa, wb, mode);
@@ -1198,6 +1233,7 @@ int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, do
 	AdjBufs& a = A->a;
 	a.cslot = c->adj_cslot;
 	a.bedges = c->adj_bedges;
+	a.bcoff = c->adj_bcoff;
 	a.nbedges = c->adj_nbedges;
 	if (K > 0) {
 		HIPCHK(hipMemcpyAsync(A->seed_cells, cells.data(), K * sizeof(int), hipMemcpyHostToDevice, c->st));
@@ -1294,7 +1330,7 @@ int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, do
 			c->b.G = G0;
 		}
 		if (c->nbc > 0 && a.nbedges > 0)
-			hipLaunchKernelGGL(k_bc_reduce, dim3(1), dim3(BLOCK), 0, c->st, c->b, a, c->nbc);
+			hipLaunchKernelGGL(k_bc_reduce, dim3(c->nbc), dim3(BLOCK), 0, c->st, c->b, a);
 		// per-cell records to the API's [4][nc], transposed on the device into
 		// the other W buffer (free now)
 		hipLaunchKernelGGL(k_records_to_arrays, dim3(grid_of(nc)), dim3(BLOCK), 0, c->st, (const double*)finW, finT,

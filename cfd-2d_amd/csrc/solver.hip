@@ -945,7 +945,8 @@ struct cfd2d_ctx {
 	std::vector<void*> allocs;
 	// adjoint topology, built on the first cfd2d_adjoint call
 	int* adj_cslot = nullptr;
-	int* adj_bedges = nullptr;
+	int* adj_bedges = nullptr;  // grouped by boundary object, ascending within one
+	int* adj_bcoff = nullptr;   // [nbc + 1] start of each object's run in adj_bedges
 	int adj_nbedges = -1;
 	// adjoint arena: checkpoints, stage records and work arrays, kept between
 	// calls and grown on demand
