@@ -86,6 +86,7 @@ SIGNATURES = {
     "cfd2d_adjoint_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "cfd2d_adjoint_geometry": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, _ip, C.POINTER(C.c_double),
                                          C.c_void_p, C.c_void_p, _dp]),
+    "cfd2d_geometry_vjp_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, _dp, _dp, C.c_void_p, _dp]),
     "cfd2d_implicit_assemble": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p]),
     "cfd2d_riemann": (C.c_int, [C.c_int, _dp, C.c_double, C.c_int, _dp, C.c_void_p, C.POINTER(C.c_longlong)]),
@@ -390,6 +391,18 @@ class Context:
                                            np.ascontiguousarray(edge_mask, np.int32), C.byref(J),
                                            gU.ctypes.data_as(C.c_void_p), gb.ctypes.data_as(C.c_void_p), gx))
         return J.value, gU.reshape(4, self.nc), gb.reshape(-1, 4)[:self.n_bc], gx.reshape(-1, 2)
+
+    def geometry_vjp_device(self, lsq, Ge, Gc, lny=None):
+        """cfd2d_geometry_vjp_device: the mesh reverse of cfd2d_mesh_geometry_vjp
+        on the GPU for Ge (13, n_edges), Gc (7, n_cells), lny (n_edges) or
+        None; returns dxy (n_nodes, 2)."""
+        Ge = np.ascontiguousarray(Ge, np.float64).ravel()
+        Gc = np.ascontiguousarray(Gc, np.float64).ravel()
+        ln = None if lny is None else np.ascontiguousarray(lny, np.float64)
+        dxy = np.zeros(2 * self.mesh.nn)
+        check(lib().cfd2d_geometry_vjp_device(self._h, self.mesh._h, int(bool(lsq)), Ge, Gc,
+                                              None if ln is None else ln.ctypes.data_as(C.c_void_p), dxy))
+        return dxy.reshape(-1, 2)
 
     def adjoint_graph(self, enable=True):
         """Record the adjoint's passes as graphs and replay them (default) or

@@ -1126,7 +1126,8 @@ void launch_adj_stage(cfd2d_ctx* c, AdjWork* A, double wa, double wb, int mode)
 
 // what a geometry-sensitivity call brings back from the reverse sweep
 struct GeoOut {
-	std::vector<double> Ge, Gc;  // [GE_N][ne], [GC_N][nc]
+	const double* Ge = nullptr;  // device [GE_N][ne], in the adjoint arena
+	const double* Gc = nullptr;  // device [GC_N][nc], in the adjoint arena
 	std::vector<double> lny;     // per edge: the objective's p(c1) where masked (adjoint of l n_y)
 };
 
@@ -1349,10 +1350,8 @@ int adjoint_run(cfd2d_ctx* c, int nsteps, int ck, const int* mask, double* J, do
 			std::fill(dJdbc, dJdbc + 4 * (size_t)c->nbc, 0.0);
 	}
 	if (geo) {
-		geo->Ge.resize(GE_N * (size_t)ne);
-		geo->Gc.resize(GC_N * (size_t)nc);
-		HIPCHK(hipMemcpyAsync(geo->Ge.data(), a.Ge, geo->Ge.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
-		HIPCHK(hipMemcpyAsync(geo->Gc.data(), a.Gc, geo->Gc.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+		geo->Ge = a.Ge;
+		geo->Gc = a.Gc;
 	}
 	HIPCHK(hipStreamSynchronize(c->st));
 	return check_launch("adjoint reverse sweep");
@@ -1372,25 +1371,6 @@ extern "C" int cfd2d_adjoint(cfd2d_ctx* c, int nsteps, int ckpt_every, const int
 	               : adjoint_run<false>(c, nsteps, ckpt_every, edge_mask, J, dJdU0, dJdbc);
 	c->timing = timing;
 	return rc;
-}
-
-extern "C" int cfd2d_adjoint_geometry(cfd2d_ctx* c, const cfd2d_mesh* m, int nsteps, int ckpt_every,
-                                      const int* edge_mask, double* J, double* dJdU0, double* dJdbc, double* dJdxy)
-{
-	if (!c || !m || nsteps < 1 || !edge_mask || !dJdxy) return fail(CFD2D_ERR_ARG, "cfd2d_adjoint_geometry: bad arguments");
-	if (m->m.nc != c->nc || m->m.ne != c->ne)
-		return fail(CFD2D_ERR_ARG, "cfd2d_adjoint_geometry: the mesh is not the context's (cell or edge count)");
-	if (c->p.flux == CFD2D_FLUX_ROE || c->p.limiter != CFD2D_LIMITER_NONE || c->p.viscous || c->p.steady)
-		return fail(CFD2D_ERR_CONFIG, "adjoint: needs FLUX GODUNOV, LAX or HLLC, no limiter, inviscid, unsteady");
-	bool timing = c->timing;
-	c->timing = false;
-	GeoOut geo;
-	int rc = c->mm ? adjoint_run<true>(c, nsteps, ckpt_every, edge_mask, J, dJdU0, dJdbc, &geo)
-	               : adjoint_run<false>(c, nsteps, ckpt_every, edge_mask, J, dJdU0, dJdbc, &geo);
-	c->timing = timing;
-	if (rc) return rc;
-	cfd2d::geometry_vjp(m->m, c->p.gradient == CFD2D_GRAD_LSQ, geo.Ge.data(), geo.Gc.data(), geo.lny.data(), dJdxy);
-	return CFD2D_OK;
 }
 
 extern "C" int cfd2d_adjoint_graph(cfd2d_ctx* c, int enable)

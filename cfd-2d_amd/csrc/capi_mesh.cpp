@@ -1,5 +1,6 @@
 // C ABI: errors and host meshes.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <new>
 #include <string>
@@ -19,6 +20,12 @@ int fail(int code, const std::string& msg)
 }
 
 void clear_error() { g_last_error.clear(); }
+
+unsigned long long next_mesh_id()
+{
+	static std::atomic<unsigned long long> next{1};
+	return next.fetch_add(1);
+}
 
 }  // namespace cfd2d
 
@@ -110,6 +117,7 @@ int cfd2d_mesh_periodic(cfd2d_mesh* m, int ga, int gb, double sx, double sy)
 	if (!m) return fail(CFD2D_ERR_ARG, "cfd2d_mesh_periodic: null mesh");
 	std::string err;
 	int rc = cfd2d::pair_periodic(m->m, ga, gb, sx, sy, err);
+	m->id = cfd2d::next_mesh_id();  // c2 and the shifts may have changed
 	return rc ? fail(CFD2D_ERR_PERIODIC, err) : CFD2D_OK;
 }
 

@@ -953,6 +953,11 @@ struct cfd2d_ctx {
 	void* adj_ws = nullptr;
 	size_t adj_ws_bytes = 0;
 	std::vector<double> adj_Jcase;  // objective per case of the last adjoint call
+	// the geometry adjoint's device copy of the mesh and work arrays, for the
+	// mesh with id geo_id (0: none yet)
+	void* geo_ws = nullptr;
+	size_t geo_ws_bytes = 0;
+	unsigned long long geo_id = 0;
 	// implicit assembly output (blocks, rhs, ctau, cols), grown on demand
 	void* impl_ws = nullptr;
 	size_t impl_ws_bytes = 0;
@@ -1366,6 +1371,7 @@ void destroy_ctx(cfd2d_ctx* c)
 	for (void* p : c->allocs) (void)hipFree(p);
 	if (c->adj_ws) (void)hipFree(c->adj_ws);
 	if (c->impl_ws) (void)hipFree(c->impl_ws);
+	if (c->geo_ws) (void)hipFree(c->geo_ws);
 	if (c->st) (void)hipStreamDestroy(c->st);
 	delete c;
 }
@@ -2035,5 +2041,6 @@ int cfd2d_ctx_stream(cfd2d_ctx* c, void** s)
 }  // extern "C"
 
 #include "adjoint.inl"
+#include "geometry.inl"
 #include "implicit.inl"
 #include "pointwise.inl"

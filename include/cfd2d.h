@@ -127,8 +127,8 @@ void cfd2d_mesh_destroy(cfd2d_mesh* m);
  * per face (ascending edge order) its Green-Gauss weight sign * n l (x, y),
  * or with lsq != 0 its least-squares weight (x, y); lny double[n_edges]:
  * adjoint of l * n_y -- returns dxy double[n_nodes][2], the adjoint of every
- * node coordinate (inputs may be NULL: zero).  cfd2d_adjoint_geometry's
- * host half; host only. */
+ * node coordinate (inputs may be NULL: zero).  Host only;
+ * cfd2d_geometry_vjp_device is the same on the GPU. */
 int  cfd2d_mesh_geometry_vjp(const cfd2d_mesh* m, int lsq, const double* Ge, const double* Gc, const double* lny,
                              double* dxy);
 
@@ -236,9 +236,20 @@ int  cfd2d_adjoint(cfd2d_ctx* c, int nsteps, int ckpt_every, const int* edge_mas
  * points, Heron areas, MeshReaderSalomeUnv.cpp:89-255, and the weights) to
  * the nodes.  The time step is held at its value (as in cfd2d_adjoint) and
  * so are the normals' orientation and periodic shifts.  Each stage also
- * forms its forward fluxes again.  Extension, no reference counterpart. */
+ * forms its forward fluxes again.  The geometry adjoints stay on the device:
+ * the reverse through the mesh builder runs there (cfd2d_geometry_vjp_device)
+ * on a device copy of m kept in the context while the same mesh is passed
+ * (a new handle or a periodic pairing counts as a new mesh).  Extension, no
+ * reference counterpart. */
 int  cfd2d_adjoint_geometry(cfd2d_ctx* c, const cfd2d_mesh* m, int nsteps, int ckpt_every, const int* edge_mask,
                             double* J, double* dJ_dU0, double* dJ_dbc, double* dJ_dxy);
+/* cfd2d_mesh_geometry_vjp on the context's GPU, for host arrays of the same
+ * layouts (Ge and Gc required, lny may be NULL: zero): the same arithmetic
+ * with every sum in the host function's order, so the same bits.  m must be
+ * the context's mesh (same cell and edge counts).  Extension, no reference
+ * counterpart. */
+int  cfd2d_geometry_vjp_device(cfd2d_ctx* c, const cfd2d_mesh* m, int lsq, const double* Ge, const double* Gc,
+                               const double* lny, double* dxy);
 /* Whether cfd2d_adjoint records its forward pass (checkpoints, recorded
  * segment, seed) and its reverse sweep (recomputed segments, every stage's
  * reverse kernels, the boundary sums) each as a HIP graph and replays it:

@@ -301,3 +301,47 @@ def test_batched_geometry_adjoint_equals_individual():
         assert abs(Jb - J) <= 1e-13 * abs(J)
         assert np.array_equal(gUb, gU)
         assert np.max(np.abs(gxb - gx)) <= 1e-13 * np.max(np.abs(gx))
+
+
+def _host_vjp(mesh, lsq, Ge, Gc, lny):
+    dxy = np.zeros(2 * mesh.nn)
+    vp = lambda a: a.ctypes.data_as(native.C.c_void_p)
+    native.check(native.lib().cfd2d_mesh_geometry_vjp(mesh._h, int(lsq), vp(Ge), vp(Gc), vp(lny), dxy))
+    return dxy.reshape(-1, 2)
+
+
+@pytest.mark.parametrize("kind,lsq", [("bump", False), ("bump", True), ("pulse", True), ("batch", False),
+                                      ("batch", True)])
+def test_geometry_vjp_device_equals_host(kind, lsq):
+    """cfd2d_geometry_vjp_device (the node-coordinate reverse that
+    cfd2d_adjoint_geometry runs on the GPU) gives the same bits as the host's
+    cfd2d_mesh_geometry_vjp (checked against central differences of the mesh
+    builder in test_geometry_adjoint.py) for random adjoints of every input:
+    a bump channel, a periodic mesh (shifted neighbours) and a 3-case batch
+    mesh; two calls on one context (the mesh's device copy reused), the
+    second with lny left out."""
+    from cfd2d import batch
+    if kind == "bump":
+        s = native.GpuSolver(bump("LAX", "LSQ" if lsq else "GREEN_GAUSS", 2, ny=9))
+        mesh, ctx = s.mesh, s
+    elif kind == "pulse":
+        s = native.GpuSolver(pulse("LAX", "LSQ", 2, n=10))
+        mesh, ctx = s.mesh, s
+    else:
+        cs = [cases.bump_channel(ny, height=0.05, seed=ny, flux="LAX", gradient="GREEN_GAUSS", limiter="NONE")
+              for ny in (4, 7, 5)]
+        s = batch.BatchSolver(cs)
+        mesh, ctx = s.mesh, s.ctx
+    rng = np.random.default_rng(11)
+    ex = mesh.export()
+    for call in range(2):
+        Ge = rng.standard_normal((13, mesh.ne))
+        Ge[8:12, ex["edge_int"][:, 3] < 0] = 0.0
+        Gc = rng.standard_normal((7, mesh.nc))
+        Gc[0, rng.random(mesh.nc) < 0.2] = 0.0  # cells without an area term
+        lny = np.where(rng.random(mesh.ne) < 0.2, rng.standard_normal(mesh.ne), 0.0) if call == 0 else None
+        host = _host_vjp(mesh, lsq, Ge, Gc, lny if lny is not None else np.zeros(mesh.ne))
+        dev = ctx.geometry_vjp_device(lsq, Ge, Gc, lny)
+        assert np.abs(host).max() > 1.0
+        assert np.array_equal(dev, host), (call, np.abs(dev - host).max())
+    s.close()
