@@ -67,6 +67,8 @@ struct AdjBufs {
 	double* Ge;  // [GE_N][ne] per edge: n-bar of the flux at Gauss points 1, 2
 	             // (x, y each), the Gauss-point offset adjoints of side s at
 	             // Gauss point g (x, y) at 4 + 4 s + 2 g, l-bar at GE_L
+	double* WF;  // [2][ne] per stage (GEO): W of c1 and of c2 (0 on a boundary)
+	             // dotted with the stage's flux F_e, for the area terms
 	double* Gc;  // [GC_N][nc] per cell: S-bar, then per face k the adjoint of
 	             // the Green-Gauss weight n l (x, y) or of the LSQ weight (x, y)
 };
@@ -506,8 +508,8 @@ __device__ __forceinline__ void hllc_vjp(const Prim& L, const Prim& R, double nx
 // Each lane adds its boundary-parameter adjoints into its own Ebc slot.
 // GONLY (LAX with geometry terms, beside k_adj_face, which forms the (P, G)
 // and parameter adjoints): only the geometry terms, which need both sides'
-// adjoints of the face, and the stage's flux, which the gradient pass's
-// geometry terms read -- no Ea or Ebc.
+// adjoints of the face -- no Ea or Ebc; the stage's flux comes from the
+// flux VJP's own intermediates (other fluxes: from the edge pass, in b.F).
 template <bool MM, int FLUX, bool GEO, bool GONLY = false>
 __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, double scale)
 {
@@ -540,18 +542,13 @@ __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, 
 		R = ghost_of<MM>(b, ph, e, c1, L, p1);
 	}
 	double Lb[4], Rb[4], nb[2] = {0.0, 0.0};
+	double Fe[4];  // GEO: the stage's flux through the edge
 	if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb, GEO ? nb : nullptr);
 	else if (FLUX == CFD2D_FLUX_GODUNOV) godunov_vjp(L, R, nx, ny, ph.rk, ph.max_newton, fb, Lb, Rb, GEO ? nb : nullptr);
 	else {
-		double f[4];
-		lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb, GEO ? nb : nullptr, GONLY ? f : nullptr);
-		if (GONLY) {
-			// the stage's flux as the edge pass stores it, for the gradient
-			// pass's edge-length and area terms
-			double o[4];
-			for (int q = 0; q < 4; q++) o[q] = __shfl_xor(f[q], 1);
-			store_edge_flux(b.F, e, gp, f, o, hl);
-		}
+		lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb, GEO ? nb : nullptr, GONLY ? Fe : nullptr);
+		if (GONLY)  // the stage's flux F_e = (f_1 + f_2) l/2, as the edge pass forms it
+			for (int q = 0; q < 4; q++) Fe[q] = (Fe[q] + __shfl_xor(Fe[q], 1)) * hl;
 	}
 	double Rsb[4];  // adjoint of c2's reconstructed state (inner edges)
 	if (c2 < 0) {
@@ -598,6 +595,20 @@ __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, 
 		a.Ge[(size_t)(5 + 2 * gp) * ne + e] += d1y;
 		a.Ge[(size_t)(8 + 2 * gp) * ne + e] += d2x;
 		a.Ge[(size_t)(9 + 2 * gp) * ne + e] += d2y;
+		// the stage's flux enters R_c1 with -, R_c2 with + (F_e = (f_1 + f_2) l/2):
+		// each lane gives its side's W . F_e for the area term, and the first
+		// the length adjoint F_e . (Rb_c2 - Rb_c1) / l = F_e . F_e-bar / (l l/2)
+		if (!GONLY) ld4(b.F, e, Fe);
+		double wv[4] = {0.0, 0.0, 0.0, 0.0};  // this lane's side's W, loaded again (a cache hit)
+		int cw = gp ? c2 : c1;
+		if (cw >= 0) ld4(a.Wn, cw, wv);
+		double wf = 0.0, lb = 0.0;
+		for (int q = 0; q < 4; q++) {
+			wf += wv[q] * Fe[q];
+			lb += Fe[q] * fb[q];
+		}
+		a.WF[(size_t)gp * ne + e] = wf;
+		if (gp == 0) a.Ge[(size_t)GE_L * ne + e] += lb / (hl * edge_geo(b, e)[2]);
 	}
 }
 
@@ -752,17 +763,18 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 		double isi = LSQ ? 0.0 : 1.0 / b.S[i];
 		double w[4];
 		ld4(a.Wn, i, w);
-		double Sb = 0.0, rbi[4];  // GEO: the cell's area adjoint; wb tau/S W (the stage's residual seed)
+		double Sb = 0.0;  // GEO: the cell's area adjoint
 		if (GEO) {
-			// the stage residual R_i (the forward fluxes of this stage, gathered
-			// as the update pass does): U' = ... + wb (tau/S) R, so
-			// d/dS (tau/S) = -(tau/S)/S gives S-bar -= wb (tau/S) (W . R) / S
-			double R[4];
-			gather_residual(b, i, R);
+			// U' = ... + wb (tau/S) R with the stage residual R_i = sum over the
+			// faces of -F_e (as c1) or +F_e (as c2), so d/dS (tau/S) = -(tau/S)/S
+			// gives S-bar -= wb (tau/S) (W . R) / S; W . F_e per side comes from
+			// the edge reverse
 			double si = wb * b.dtS[i], wr = 0.0;
-			for (int q = 0; q < 4; q++) {
-				wr += w[q] * R[q];
-				rbi[q] = si * w[q];
+			int4 fc = load_faces(b, i);
+			int fs[3] = {fc.x, fc.y, fc.z};
+			for (int k = 0; k < 3; k++) {
+				double t = a.WF[(size_t)(fs[k] & 1) * ne + (fs[k] >> 1)];
+				wr += (fs[k] & 1) ? t : -t;
 			}
 			Sb -= si * wr / b.S[i];
 			if (!LSQ) {  // Green-Gauss: G = (sum over faces) / S
@@ -837,21 +849,6 @@ __global__ __launch_bounds__(BLOCK) void k_adj_grad(Bufs b, Phys ph, AdjBufs a, 
 				}
 				a.Gc[(size_t)(1 + 2 * k) * n + i] += wbx;
 				a.Gc[(size_t)(2 + 2 * k) * n + i] += wby;
-				if (side == 0) {
-					// the edge length: F_e = (f_1 + f_2) l/2 enters R_c1 with -, R_c2 with +
-					double F[4];
-					ld4(b.F, e, F);
-					double rbj[4] = {0.0, 0.0, 0.0, 0.0};
-					if (j >= 0) {
-						double wj[4];
-						ld4(a.Wn, j, wj);
-						double sj = wb * b.dtS[j];
-						for (int q = 0; q < 4; q++) rbj[q] = sj * wj[q];
-					}
-					double lb = 0.0;
-					for (int q = 0; q < 4; q++) lb += F[q] * (rbj[q] - rbi[q]);
-					a.Ge[(size_t)GE_L * ne + e] += lb / edge_geo(b, e)[2];
-				}
 			}
 		}
 		if (GEO) a.Gc[i] += Sb;
@@ -979,6 +976,7 @@ void carve_work(AdjWork& W, Carve& cv, int nck, size_t nrec, size_t nc, size_t n
 	W.a.bcbar = cv.take<double>(4 * (size_t)(nbc > 0 ? nbc : 1));
 	W.a.Ge = geo ? cv.take<double>(GE_N * ne) : nullptr;
 	W.a.Gc = geo ? cv.take<double>(GC_N * nc) : nullptr;
+	W.a.WF = geo ? cv.take<double>(2 * ne) : nullptr;
 	W.seed_cells = cv.take<int>(K);
 	W.seed_w = cv.take<double>(K);
 	W.seed_p = cv.take<double>(K);
