@@ -1069,7 +1069,10 @@ void record_steps(cfd2d_ctx* c, AdjWork* A, int nsteps)
 			c->b.Po = A->Pst + (t + 1) * n4;
 			c->b.G = A->Gst + t * n8;
 			launch_gradient_edge<MM>(c);
-			c->ph.store_g = 1;  // the update pass stores the stage's gradient records
+			// the stage's gradient records: the separate gradient pass (fuse mode
+			// 0; no limiter here) has stored them already, otherwise the update
+			// pass forms and stores them
+			c->ph.store_g = c->fuse_mode != 0;
 			launch_update<MM>(c, S == 3 ? modes3[st] : modes2[st]);
 			c->ph.store_g = 0;
 		}
@@ -1091,9 +1094,9 @@ void launch_adj_grad(cfd2d_ctx* c, AdjWork* A, double wa, double wb, int mode)
 	else hipLaunchKernelGGL((k_adj_grad<MM, LSQ, 2, GEO>), g, blk, 0, c->st, c->b, c->ph, A->a, wa, wb);
 }
 
-// GEO: geometry adjoints too; the stage's forward fluxes are formed again
-// first (edge pass on the recorded P and G, into the context's flux buffer)
-// for the length and area terms
+// GEO: geometry adjoints too; with HLLC or Godunov the stage's forward
+// fluxes are formed again first (edge pass on the recorded P and G, into the
+// context's flux buffer) for the length and area terms
 template <bool MM, bool GEO>
 void launch_adj_stage(cfd2d_ctx* c, AdjWork* A, double wa, double wb, int mode)
 {

@@ -667,11 +667,12 @@ __device__ __forceinline__ void st_state(double* p, double v)
 // reads its input from b.U and the step-start state from b.U0 and writes
 // b.Un; launch_update points them so that the step-start state is never
 // copied (the reference's U_old copy, 318-321).  NT: see ld_state.  REC
-// (adjoint recording, where the stage's input primitives b.P and its output
-// b.Po are separate slots): the pass first forms the gradient records of
-// its input state, with the gradient pass's code, and stores them to b.G
-// through LDS -- the edge pass formed the same values but could only store
-// them one scattered record per cell.
+// (adjoint recording behind an edge pass that forms its own gradients, where
+// the stage's input primitives b.P and its output b.Po are separate slots):
+// the pass first forms the gradient records of its input state, with the
+// gradient pass's code, and stores them to b.G through LDS -- the edge pass
+// formed the same values, but stored from the lane of each cell's first face
+// they were single scattered records, slower in measurement.
 template <bool MM, int MODE, bool NT, bool REC>
 __global__ __launch_bounds__(BLOCK) void k_update(Bufs b, Phys ph)
 {

@@ -345,3 +345,21 @@ def test_geometry_vjp_device_equals_host(kind, lsq):
         assert np.abs(host).max() > 1.0
         assert np.array_equal(dev, host), (call, np.abs(dev - host).max())
     s.close()
+
+
+@pytest.mark.parametrize("flux,gradient,rk", [("LAX", "GREEN_GAUSS", 2), ("HLLC", "LSQ", 3)])
+def test_adjoint_recording_is_the_same_for_every_fuse_mode(flux, gradient, rk):
+    """The recorded gradient records come from the update pass (fuse modes 1
+    and 2, the per-edge and tiled edge passes) or from the separate gradient
+    pass (mode 0); all three give the same bits of J, dJ/dU0 and dJ/dpar."""
+    c = bump(flux, gradient, rk, ny=10)
+    res = []
+    for mode in (1, 0, 2):
+        g = native.GpuSolver(c)
+        g.fuse_gradient(mode)
+        res.append(g.adjoint(8, g.edge_mask("bump"), ckpt_every=3))
+        g.close()
+    for J, gU, gb in res[1:]:
+        assert J == res[0][0]
+        assert np.array_equal(gU, res[0][1])
+        assert np.array_equal(gb, res[0][2])
