@@ -246,8 +246,9 @@ int  cfd2d_adjoint_geometry(cfd2d_ctx* c, const cfd2d_mesh* m, int nsteps, int c
 /* cfd2d_mesh_geometry_vjp on the context's GPU, for host arrays of the same
  * layouts (Ge and Gc required, lny may be NULL: zero): the same arithmetic
  * with every sum in the host function's order, so the same bits.  m must be
- * the context's mesh (same cell and edge counts).  Extension, no reference
- * counterpart. */
+ * the context's mesh (same cell and edge counts).  Allocates device copies
+ * of Ge and Gc for the call (cfd2d_adjoint_geometry reads its own from the
+ * adjoint arena).  Extension, no reference counterpart. */
 int  cfd2d_geometry_vjp_device(cfd2d_ctx* c, const cfd2d_mesh* m, int lsq, const double* Ge, const double* Gc,
                                const double* lny, double* dxy);
 /* Whether cfd2d_adjoint records its forward pass (checkpoints, recorded
