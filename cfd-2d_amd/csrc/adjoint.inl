@@ -548,7 +548,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, 
 	else {
 		lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb, GEO ? nb : nullptr, GONLY ? Fe : nullptr);
 		if (GONLY)  // the stage's flux F_e = (f_1 + f_2) l/2, as the edge pass forms it
-			for (int q = 0; q < 4; q++) Fe[q] = (Fe[q] + __shfl_xor(Fe[q], 1)) * hl;
+			for (int q = 0; q < 4; q++) Fe[q] = (Fe[q] + pair_swap(Fe[q])) * hl;
 	}
 	double Rsb[4];  // adjoint of c2's reconstructed state (inner edges)
 	if (c2 < 0) {
@@ -703,8 +703,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 			gb[2 * q + 1] += o * dy;
 		}
 	}
-	for (int q = 0; q < 4; q++) pb[q] += __shfl_xor(pb[q], 1);
-	for (int q = 0; q < 8; q++) gb[q] += __shfl_xor(gb[q], 1);
+	for (int q = 0; q < 4; q++) pb[q] += pair_swap(pb[q]);
+	for (int q = 0; q < 8; q++) gb[q] += pair_swap(gb[q]);
 	if (gp == 0) {
 		sta4(a.Pb, b.nc, i, pb);
 		sta8(a.Gb, b.nc, i, gb);

@@ -88,7 +88,7 @@ __global__ __launch_bounds__(BLOCK) void k_impl_edge(Bufs b, Phys ph)
 	else flux_godunov(f, L, R, nx, ny, ph.rk, ph.max_newton, hit);
 	if (hit) atomicAdd(&b.err[0], 1ull);
 	double o[4];
-	for (int q = 0; q < 4; q++) o[q] = __shfl_xor(f[q], 1);
+	for (int q = 0; q < 4; q++) o[q] = pair_swap(f[q]);
 	store_edge_flux(b.F, e, gp, f, o, edge_geo(b, e)[2] * 0.5);
 }
 

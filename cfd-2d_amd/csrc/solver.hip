@@ -134,6 +134,18 @@ __device__ __forceinline__ void ld8(const double* base, size_t i, double* v)
 	ld4(base, 2 * i + 1, v + 4);
 }
 
+// The value x of the other lane of this lane's pair (lanes 2k and 2k + 1):
+// a DPP quad permutation [1, 0, 3, 2] of each 32-bit half, a VALU move,
+// where __shfl_xor(x, 1) goes through the LDS crossbar and waits for it.
+// The kernels keep both lanes of a pair active together.
+__device__ __forceinline__ double pair_swap(double x)
+{
+	unsigned long long v = __builtin_bit_cast(unsigned long long, x);
+	unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)v, 0xB1, 0xF, 0xF, false);
+	unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(v >> 32), 0xB1, 0xF, 0xF, false);
+	return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // An edge's record (32 bytes, two 16-byte loads): unit normal out of c1,
 // length, and its two cells (c2 = -(bc+1) on a boundary) in the last 8 bytes.
 __device__ __forceinline__ const double* edge_geo(const Bufs& b, size_t e)
@@ -474,10 +486,10 @@ __device__ __forceinline__ void edge_lane(const Bufs& b, const Phys& ph, int e, 
 	Prim keep = ps, send = ps, recv;
 	recon_g(keep, gs, dl[0], dl[1]);
 	recon_g(send, gs, dl[2], dl[3]);
-	recv.r = __shfl_xor(send.r, 1);
-	recv.p = __shfl_xor(send.p, 1);
-	recv.u = __shfl_xor(send.u, 1);
-	recv.v = __shfl_xor(send.v, 1);
+	recv.r = pair_swap(send.r);
+	recv.p = pair_swap(send.p);
+	recv.u = pair_swap(send.u);
+	recv.v = pair_swap(send.v);
 	Prim L, R;
 	if (inner) {
 		L = gp ? recv : keep;
@@ -499,7 +511,7 @@ __device__ __forceinline__ void edge_lane(const Bufs& b, const Phys& ph, int e, 
 		for (int q = 0; q < 4; q++) own[q] = gs[4 + q];  // ux uy vx vy
 		grad_T_g(gs, ps, mat_of<MM>(b, ph, cs), own[4], own[5]);
 		double oth[6];
-		for (int q = 0; q < 6; q++) oth[q] = __shfl_xor(own[q], 1);
+		for (int q = 0; q < 6; q++) oth[q] = pair_swap(own[q]);
 		double ux = own[0], uy = own[1], vx = own[2], vy = own[3], tx = own[4], ty = own[5];
 		if (inner) {
 			ux = 0.5 * (ux + oth[0]);
@@ -531,7 +543,7 @@ __device__ __forceinline__ void edge_lane(const Bufs& b, const Phys& ph, int e, 
 		f[3] -= ev;
 	}
 	double o[4];
-	for (int q = 0; q < 4; q++) o[q] = __shfl_xor(f[q], 1);
+	for (int q = 0; q < 4; q++) o[q] = pair_swap(f[q]);
 	store_edge_flux(b.F, e, gp, f, o, edge_geo(b, e)[2] * 0.5);
 }
 

@@ -30,6 +30,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <functional>
@@ -228,6 +229,28 @@ inline T __shfl_xor(T v, int mask, int = 64)
 	hostemu::yield();
 	return r;
 }
+
+// DPP quad permutation (dpp_ctrl < 0x100: lane 4k + j reads lane 4k +
+// perm[j], perm[j] = bits 2j..2j+1 of the control) of a 32-bit value; the
+// row and bank masks must be full, as in the kernels
+inline int hostemu_mov_dpp(int v, int ctrl, int row_mask, int bank_mask)
+{
+	if (ctrl < 0 || ctrl >= 0x100 || row_mask != 0xF || bank_mask != 0xF) {
+		fprintf(stderr, "hostemu: only full-mask DPP quad permutations are emulated\n");
+		abort();
+	}
+	auto& G = hostemu::g_group;
+	int t = hostemu::flat_tid();
+	int src = (t & ~3) | ((ctrl >> (2 * (t & 3))) & 3);
+	std::memcpy(G.lane[t], &v, sizeof(int));
+	hostemu::yield();
+	int r;
+	std::memcpy(&r, G.lane[src], sizeof(int));
+	hostemu::yield();
+	return r;
+}
+#define __builtin_amdgcn_mov_dpp(v, ctrl, row_mask, bank_mask, bound_ctrl) \
+	hostemu_mov_dpp((v), (ctrl), (row_mask), (bank_mask))
 
 template <typename T, typename U>
 inline T atomicAdd(T* p, U v)
