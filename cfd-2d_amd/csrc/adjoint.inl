@@ -628,15 +628,18 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 	int i = t >> 1, gp = t & 1;
 	if (i >= b.nc) return;  // both lanes of a pair leave together
 	size_t ne = b.ne;
-	Prim pi = load_prim(b, i);
+	// the records both lanes of the pair read are loaded half by each
+	double pv[4];
+	pair_load<4>(b.P, i, gp, pv);
+	Prim pi{pv[0], pv[1], pv[2], pv[3]};
 	double wi[4];
 	double si = scale * b.dtS[i];
-	ld4(a.Wn, i, wi);
+	pair_load<4>(a.Wn, i, gp, wi);
 	for (int q = 0; q < 4; q++) wi[q] *= si;
 	double pb[4] = {0, 0, 0, 0};
 	double gb[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 	double gi[8];  // this cell's gradient record, used on every face
-	ld8(b.G, i, gi);
+	pair_load<8>(b.G, i, gp, gi);
 	CellTop tp = load_top(b, i);
 	for (int k = 0; k < 3; k++) {
 		int f = tp.f[k];
@@ -647,11 +650,15 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 		// F_e-bar = (Rb_c2 - Rb_c1) l/2; this cell is c1 (side 0) or c2 (side 1)
 		double fb[4];
 		Prim pj = pi;
+		double gj[8];
 		if (j >= 0) {
-			pj = load_prim(b, j);
+			double pv[4];
+			pair_load<4>(b.P, j, gp, pv);
+			pj = Prim{pv[0], pv[1], pv[2], pv[3]};
+			pair_load<8>(b.G, j, gp, gj);
 			double sj = scale * b.dtS[j];
 			double wv[4];
-			ld4(a.Wn, j, wv);
+			pair_load<4>(a.Wn, j, gp, wv);
 			for (int q = 0; q < 4; q++) {
 				double wj = sj * wv[q];
 				fb[q] = (side ? wi[q] - wj : wj - wi[q]) * hl;
@@ -666,7 +673,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 		recon_g(own, gi, dx, dy);
 		if (j >= 0) {
 			Prim oth = pj;
-			recon(oth, b, j, b.edl[edl_index(e, 1 - side, gp)], b.edl[edl_index(e, 1 - side, gp) + 1]);
+			recon_g(oth, gj, b.edl[edl_index(e, 1 - side, gp)], b.edl[edl_index(e, 1 - side, gp) + 1]);
 			L = side ? oth : own;
 			R = side ? own : oth;
 		} else {

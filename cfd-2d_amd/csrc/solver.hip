@@ -146,6 +146,23 @@ __device__ __forceinline__ double pair_swap(double x)
 	return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+// A record that both lanes of a pair need (same index i in both): lane gp
+// loads half gp and the pair swaps the halves -- half the loads per lane.
+// N = 4 or 8 doubles.
+template <int N>
+__device__ __forceinline__ void pair_load(const double* base, size_t i, int gp, double* v)
+{
+	constexpr int H = N / 2;
+	double m[H], o[H];
+	const double* p = static_cast<const double*>(__builtin_assume_aligned(base + N * i + H * gp, 8 * H));
+	for (int q = 0; q < H; q++) m[q] = p[q];
+	for (int q = 0; q < H; q++) o[q] = pair_swap(m[q]);
+	for (int q = 0; q < H; q++) {
+		v[q] = gp ? o[q] : m[q];
+		v[H + q] = gp ? m[q] : o[q];
+	}
+}
+
 // An edge's record (32 bytes, two 16-byte loads): unit normal out of c1,
 // length, and its two cells (c2 = -(bc+1) on a boundary) in the last 8 bytes.
 __device__ __forceinline__ const double* edge_geo(const Bufs& b, size_t e)
