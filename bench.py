@@ -169,10 +169,12 @@ def batched_vs_serial(n_cases, steps, seed=7):
                               serial_stream=sum(p["stream"] for p in picks), serial_cases=len(picks)))
 
 
-def adjoint_vs_forward(steps, ckpt, ny=289):
+def adjoint_vs_forward(steps, ckpt, ny=289, flux="LAX"):
     """Adjoint workload: bump channel with ny=289 (501,126 triangles), Mach
-    0.8, bump height 0.05; LAX flux, Green-Gauss gradients, Heun RK2, no
-    limiter; J = y pressure force on the bump.  Wall time of cfd2d_adjoint
+    0.8, bump height 0.05; LAX flux (or the reference's exact Godunov flux,
+    whose reverse differentiates the Newton iterations as evaluated),
+    Green-Gauss gradients, Heun RK2, no limiter; J = y pressure force on the
+    bump.  Wall time of cfd2d_adjoint
     (forward pass, checkpoints, recomputation, reverse sweep, gradient copied
     to the host) over the wall time of the same number of plain steps, each
     the faster of two calls.  The first call with each checkpoint interval
@@ -181,7 +183,7 @@ def adjoint_vs_forward(steps, ckpt, ny=289):
     calls do."""
     from cfd2d import case as cases
     from cfd2d import native
-    c = cases.bump_channel(ny, height=0.05, mach=0.8, seed=3, flux="LAX", gradient="GREEN_GAUSS", limiter="NONE",
+    c = cases.bump_channel(ny, height=0.05, mach=0.8, seed=3, flux=flux, gradient="GREEN_GAUSS", limiter="NONE",
                            rk_stages=2)
     g = native.GpuSolver(c)
     mask = g.edge_mask("bump")
@@ -201,7 +203,7 @@ def adjoint_vs_forward(steps, ckpt, ny=289):
         return best, r
 
     t_fwd, _ = timed(lambda: g.run(steps), 2)
-    out = dict(cells=g.nc, edges=g.ne, steps=steps, workload="bump channel ny=%d, LAX/GG/RK2, Mach 0.8" % ny,
+    out = dict(cells=g.nc, edges=g.ne, steps=steps, workload="bump channel ny=%d, %s/GG/RK2, Mach 0.8" % (ny, flux),
                forward_s=round(t_fwd, 5))
     for name, ck in (("ckpt%d" % ckpt, ckpt), ("auto", 0)):
         t_first, _ = timed(lambda: g.adjoint(steps, mask, ckpt_every=ck), 1)
@@ -489,6 +491,8 @@ def main():
     if not args.no_adjoint:
         out["adjoint"] = adjoint_vs_forward(args.adj_steps, args.adj_ckpt)
         log("adjoint: %s" % out["adjoint"])
+        out["adjoint_godunov"] = adjoint_vs_forward(args.adj_steps, args.adj_ckpt, flux="GODUNOV")
+        log("adjoint, exact Godunov flux: %s" % out["adjoint_godunov"])
     if not args.no_badj:
         out["batched_adjoint"] = batched_adjoint(args.badj_cases, args.badj_steps)
         log("batched adjoint: %s" % out["batched_adjoint"])

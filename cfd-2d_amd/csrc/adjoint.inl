@@ -410,35 +410,64 @@ __device__ __forceinline__ void rim_d1(D1& RI, D1& EI, D1& PI, D1& UI, D1& VI, D
 	PI = K.AGAM * EI * RI;
 }
 
-// The exact Godunov flux's vector-Jacobian product by forward mode: one dual
-// evaluation of flux_godunov per input direction (the eight states, and the
-// normal when nb is given), each dotted with fb.
+// The exact Godunov flux's vector-Jacobian product.  The Riemann solver's
+// inputs that matter are six: density, pressure and normal velocity of each
+// side; the tangential velocities only pass through (VI is VB or VE), and
+// the rotation to and from the normal frame and the flux assembly are a few
+// products.  So the solver runs forward mode once per one of the six
+// directions (dual numbers through rim_d1, the forward solver operation for
+// operation); the first pass also seeds VB, whose only effect is VI's
+// tangent, 1 where the left tangential velocity was sampled.  The flux
+// assembly and the rotation are reversed by hand around it, and the normal's
+// adjoint (nb given) comes from those alone.
 __device__ __forceinline__ void godunov_vjp(const Prim& L, const Prim& R, double nx, double ny, const RimConst& K,
                                             int max_iter, const double* fb, double* Lb, double* Rb,
                                             double* nb = nullptr)
 {
-	const double lv[4] = {L.r, L.p, L.u, L.v}, rv[4] = {R.r, R.p, R.u, R.v};
-	int nd = nb ? 10 : 8;
-	for (int j = 0; j < nd; j++) {
-		D1 l[4], r[4];
-		for (int q = 0; q < 4; q++) {
-			l[q] = D1{lv[q], (j == q) ? 1.0 : 0.0};
-			r[q] = D1{rv[q], (j == 4 + q) ? 1.0 : 0.0};
-		}
-		D1 Nx{nx, j == 8 ? 1.0 : 0.0}, Ny{ny, j == 9 ? 1.0 : 0.0};
-		D1 unl = l[2] * Nx + l[3] * Ny, unr = r[2] * Nx + r[3] * Ny;
-		D1 utl = l[2] * Ny - l[3] * Nx, utr = r[2] * Ny - r[3] * Nx;
+	double unl = L.u * nx + L.v * ny, unr = R.u * nx + R.v * ny;
+	double utl = L.u * ny - L.v * nx, utr = R.u * ny - R.v * nx;
+	const double in[6] = {L.r, L.p, unl, R.r, R.p, unr};
+	double xb[6];  // adjoints of the six solver inputs
+	double RIb = 0.0, EIb = 0.0, PIb = 0.0, UNb = 0.0, UTb = 0.0, nxb = 0.0, nyb = 0.0, sel = 0.0;
+	for (int j = 0; j < 6; j++) {
+		D1 x[6];
+		for (int q = 0; q < 6; q++) x[q] = D1{in[q], q == j ? 1.0 : 0.0};
 		D1 RI, EI, PI, UN, UT;
-		rim_d1(RI, EI, PI, UN, UT, l[0], l[1], unl, utl, r[0], r[1], unr, utr, K, max_iter);
-		D1 UI = UN * Nx + UT * Ny, VI = UN * Ny - UT * Nx;
-		D1 f0 = RI * UN;
-		D1 f1 = f0 * UI + PI * Nx;
-		D1 f2 = f0 * VI + PI * Ny;
-		D1 f3 = (RI * (EI + 0.5 * (UI * UI + VI * VI)) + PI) * UN;
-		double s = fb[0] * f0.d + fb[1] * f1.d + fb[2] * f2.d + fb[3] * f3.d;
-		if (j < 4) Lb[j] = s;
-		else if (j < 8) Rb[j - 4] = s;
-		else nb[j - 8] = s;
+		rim_d1(RI, EI, PI, UN, UT, x[0], x[1], x[2], D1{utl, j == 0 ? 1.0 : 0.0}, x[3], x[4], x[5], D1{utr, 0.0}, K,
+		       max_iter);
+		if (j == 0) {
+			// the primal outputs: reverse of UI = UN nx + UT ny, VI = UN ny - UT nx,
+			// f0 = RI UN, f1 = f0 UI + PI nx, f2 = f0 VI + PI ny,
+			// f3 = (RI (EI + (UI^2 + VI^2)/2) + PI) UN
+			double ri = RI.v, ei = EI.v, pi = PI.v, un = UN.v, ut = UT.v;
+			double ui = un * nx + ut * ny, vi = un * ny - ut * nx;
+			double f0 = ri * un, h = ei + 0.5 * (ui * ui + vi * vi), A = ri * h + pi;
+			double f0b = fb[0] + fb[1] * ui + fb[2] * vi;
+			double uib = fb[1] * f0 + fb[3] * un * ri * ui;
+			double vib = fb[2] * f0 + fb[3] * un * ri * vi;
+			PIb = fb[1] * nx + fb[2] * ny + fb[3] * un;
+			RIb = f0b * un + fb[3] * un * h;
+			EIb = fb[3] * un * ri;
+			UNb = f0b * ri + fb[3] * A + uib * nx + vib * ny;
+			UTb = uib * ny - vib * nx;
+			nxb = fb[1] * pi + uib * un - vib * ut;
+			nyb = fb[2] * pi + uib * ut + vib * un;
+			sel = UT.d;
+		}
+		xb[j] = RIb * RI.d + EIb * EI.d + PIb * PI.d + UNb * UN.d;
+	}
+	double vbl = UTb * sel, vbr = UTb * (1.0 - sel);  // adjoints of the tangential velocities
+	Lb[0] = xb[0];
+	Lb[1] = xb[1];
+	Lb[2] = xb[2] * nx + vbl * ny;
+	Lb[3] = xb[2] * ny - vbl * nx;
+	Rb[0] = xb[3];
+	Rb[1] = xb[4];
+	Rb[2] = xb[5] * nx + vbr * ny;
+	Rb[3] = xb[5] * ny - vbr * nx;
+	if (nb) {  // u_n = u nx + v ny, u_t = u ny - v nx on each side
+		nb[0] = nxb + xb[2] * L.u - vbl * L.v + xb[5] * R.u - vbr * R.v;
+		nb[1] = nyb + xb[2] * L.v + vbl * L.u + xb[5] * R.v + vbr * R.u;
 	}
 }
 
