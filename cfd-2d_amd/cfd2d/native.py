@@ -91,6 +91,7 @@ SIGNATURES = {
                                           C.c_void_p, C.c_void_p]),
     "cfd2d_riemann": (C.c_int, [C.c_int, _dp, C.c_double, C.c_int, _dp, C.c_void_p, C.POINTER(C.c_longlong)]),
     "cfd2d_flux": (C.c_int, [C.c_int, C.c_int, _dp, C.c_double, C.c_int, _dp, C.POINTER(C.c_longlong)]),
+    "cfd2d_flux_vjp": (C.c_int, [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int, _dp]),
     "cfd2d_flux_noalloc": (C.c_int, [C.c_int, C.c_int, _dp, C.c_double, C.c_int, _dp, C.POINTER(C.c_longlong)]),
     "cfd2d_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
     "cfd2d_host_free": (C.c_int, [C.c_void_p]),
@@ -168,6 +169,21 @@ def flux(kind, rows, gam=1.4, max_newton=0, noalloc=False):
     f = lib().cfd2d_flux_noalloc if noalloc else lib().cfd2d_flux
     check(f(k, rows.shape[0], rows.ravel(), float(gam), int(max_newton), out.ravel(), C.byref(hits)))
     return out, hits.value
+
+
+def flux_vjp(kind, rows, fbar, gam=1.4, max_newton=0):
+    """The vector-Jacobian product of an adjoint flux (GODUNOV, LAX, HLLC) on
+    the device for rows (n, 10) as flux() and seeds fbar (n, 4); returns (n,
+    10) = adjoints of (rL, pL, uL, vL, rR, pR, uR, vR, nx, ny)."""
+    k = FLUX_KINDS[kind] if isinstance(kind, str) else int(kind)
+    rows = np.ascontiguousarray(rows, np.float64).reshape(-1, 10)
+    fbar = np.ascontiguousarray(fbar, np.float64).reshape(-1, 4)
+    if fbar.shape[0] != rows.shape[0]:
+        raise ValueError("flux_vjp: one seed row per state row")
+    out = np.zeros((rows.shape[0], 10))
+    check(lib().cfd2d_flux_vjp(k, rows.shape[0], rows.ravel(), fbar.ravel(), float(gam), int(max_newton),
+                               out.ravel()))
+    return out
 
 
 class Mesh:

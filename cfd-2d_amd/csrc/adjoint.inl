@@ -410,64 +410,470 @@ __device__ __forceinline__ void rim_d1(D1& RI, D1& EI, D1& PI, D1& UI, D1& VI, D
 	PI = K.AGAM * EI * RI;
 }
 
-// The exact Godunov flux's vector-Jacobian product.  The Riemann solver's
-// inputs that matter are six: density, pressure and normal velocity of each
-// side; the tangential velocities only pass through (VI is VB or VE), and
-// the rotation to and from the normal frame and the flux assembly are a few
-// products.  So the solver runs forward mode once per one of the six
-// directions (dual numbers through rim_d1, the forward solver operation for
-// operation); the first pass also seeds VB, whose only effect is VI's
-// tangent, 1 where the left tangential velocity was sampled.  The flux
-// assembly and the rotation are reversed by hand around it, and the normal's
-// adjoint (nb given) comes from those alone.
+// Reverse mode of rim (physics.hpp): the VJP of the Riemann solver's five
+// outputs with respect to its eight inputs, for the same evaluated map that
+// rim_d1 differentiates forward -- its values, branches, Newton iterations
+// and shared divisions, each operation reversed.  The iterates are not
+// stored: the reverse of Newton iteration k runs the loop again from its
+// start to k (n iterations cost n(n+1)/2 forward iterations).  With more than
+// RIM_REV_MAX iterations it returns false and leaves the work to the caller's
+// forward-mode path.
+constexpr int RIM_REV_MAX = 8;
+
+struct RimSetup {
+	double rpB, rpE, i4, iB, iE, IRB, IPB, IRE, IPE, CB, CE, RCB, RCE, EB, EE, DU;
+};
+
+__device__ __forceinline__ RimSetup rim_setup(double RB, double PB, double UB, double RE, double PE, double UE,
+                                              const RimConst& K)
+{
+	RimSetup f;
+	f.rpB = RB * PB;
+	f.rpE = RE * PE;
+	f.i4 = 1.0 / (f.rpB * f.rpE);
+	f.iB = f.rpE * f.i4;
+	f.iE = f.rpB * f.i4;
+	f.IRB = PB * f.iB;
+	f.IPB = RB * f.iB;
+	f.IRE = PE * f.iE;
+	f.IPE = RE * f.iE;
+	f.CB = sqrt(K.GAM * PB * f.IRB);
+	f.CE = sqrt(K.GAM * PE * f.IRE);
+	f.RCB = RB * f.CB;
+	f.RCE = RE * f.CE;
+	f.EB = f.CB * f.CB * K.ISGAM;
+	f.EE = f.CE * f.CE * K.ISGAM;
+	f.DU = UB - UE;
+	return f;
+}
+
+// one Newton iteration of rim at P: the clamp, both sides' terms, the update
+struct RimNewton {
+	double Pc, x[2], t[2], y[2], den[2], ir, F[2], FS[2], res, S;
+	bool clamped, rare[2];
+};
+
+__device__ __forceinline__ RimNewton rim_newton(const RimSetup& f, double P, double PB, double PE, const RimConst& K)
+{
+	RimNewton s;
+	s.clamped = P < 1.0e-5;
+	s.Pc = s.clamped ? 1.0e-5 : P;
+	for (int sd = 0; sd < 2; sd++) {
+		double PK = sd ? PE : PB, IPK = sd ? f.IPE : f.IPB, RC = sd ? f.RCE : f.RCB;
+		s.x[sd] = s.Pc * IPK;
+		s.rare[sd] = PK > s.Pc;
+		s.t[sd] = s.rare[sd] ? s.x[sd] : K.PGAM * s.x[sd] + K.OGAM;
+		s.y[sd] = side_root(s.t[sd], s.rare[sd], K);
+		s.den[sd] = s.rare[sd] ? K.GAM * s.Pc : RC * s.y[sd] * s.t[sd];
+	}
+	s.ir = 1.0 / (s.den[0] * s.den[1]);
+	for (int sd = 0; sd < 2; sd++) {
+		double PK = sd ? PE : PB, C = sd ? f.CE : f.CB;
+		double r = s.den[1 - sd] * s.ir;
+		s.F[sd] = s.rare[sd] ? K.DGAM * (C * s.y[sd] - C) : (s.Pc - PK) * s.t[sd] * r;
+		s.FS[sd] = s.rare[sd] ? C * s.y[sd] * r : (K.QGAM * s.x[sd] + K.FGAM) * r * K.IRGAM;
+	}
+	s.res = f.DU - s.F[0] - s.F[1];
+	s.S = s.FS[0] + s.FS[1];
+	return s;
+}
+
+// adjoints of the solver inputs and of the setup values
+struct RimAdj {
+	double RB, PB, UB, VB, RE, PE, UE, VE, IRB, IPB, IRE, IPE, CB, CE, RCB, RCE, EB, EE, DU;
+};
+
+// reverse of one Newton iteration: Pnb is the adjoint of the iterate it
+// produced; returns the adjoint of the iterate it started from
+__device__ __forceinline__ double rim_newton_rev(const RimSetup& f, const RimNewton& s, double PB, double PE,
+                                                 const RimConst& K, double Pnb, RimAdj& a)
+{
+	double Pcb = Pnb;
+	double resb = Pnb / s.S;
+	double Sb = -Pnb * s.res / (s.S * s.S);
+	a.DU += resb;
+	double yb[2], tb[2], xb[2] = {0.0, 0.0}, denb[2] = {0.0, 0.0}, irb = 0.0;
+	for (int sd = 0; sd < 2; sd++) {
+		double PK = sd ? PE : PB, C = sd ? f.CE : f.CB;
+		double r = s.den[1 - sd] * s.ir;
+		double Fb = -resb, FSb = Sb, rb = 0.0, Cb = 0.0;
+		yb[sd] = 0.0;
+		tb[sd] = 0.0;
+		if (s.rare[sd]) {
+			Cb += Fb * K.DGAM * (s.y[sd] - 1.0);
+			yb[sd] += Fb * K.DGAM * C;
+			Cb += FSb * s.y[sd] * r;
+			yb[sd] += FSb * C * r;
+			rb += FSb * C * s.y[sd];
+		} else {
+			double dp = s.Pc - PK;
+			Pcb += Fb * s.t[sd] * r;
+			(sd ? a.PE : a.PB) -= Fb * s.t[sd] * r;
+			tb[sd] += Fb * dp * r;
+			rb += Fb * dp * s.t[sd];
+			xb[sd] += FSb * K.QGAM * r * K.IRGAM;
+			rb += FSb * (K.QGAM * s.x[sd] + K.FGAM) * K.IRGAM;
+		}
+		(sd ? a.CE : a.CB) += Cb;
+		denb[1 - sd] += rb * s.ir;
+		irb += rb * s.den[1 - sd];
+	}
+	double q = s.den[0] * s.den[1], qb = -irb / (q * q);
+	denb[0] += qb * s.den[1];
+	denb[1] += qb * s.den[0];
+	for (int sd = 0; sd < 2; sd++) {
+		double RC = sd ? f.RCE : f.RCB, IPK = sd ? f.IPE : f.IPB;
+		if (s.rare[sd]) {
+			Pcb += denb[sd] * K.GAM;
+		} else {
+			(sd ? a.RCE : a.RCB) += denb[sd] * s.y[sd] * s.t[sd];
+			yb[sd] += denb[sd] * RC * s.t[sd];
+			tb[sd] += denb[sd] * RC * s.y[sd];
+		}
+		tb[sd] += yb[sd] * ((s.rare[sd] ? K.OGAM : 0.5) * s.y[sd] / s.t[sd]);
+		xb[sd] += s.rare[sd] ? tb[sd] : tb[sd] * K.PGAM;
+		Pcb += xb[sd] * IPK;
+		(sd ? a.IPE : a.IPB) += xb[sd] * s.Pc;
+	}
+	return s.clamped ? 0.0 : Pcb;
+}
+
+__device__ __forceinline__ bool rim_vjp(double RB, double PB, double UB, double VB, double RE, double PE, double UE,
+                                        double VE, const RimConst& K, int max_iter, double RIb, double EIb, double PIb,
+                                        double UIb, double VIb, double* xb)
+{
+	const double eps = 1.0e-5;
+	RimAdj a{};
+	RimSetup f = rim_setup(RB, PB, UB, RE, PE, UE, K);
+	bool vac = f.DU * K.AGAM < -2.0 * (f.CB + f.CE);
+	double P0 = 0.0, P = 0.0;
+	int n = 0;
+	// star states: y, Z, M, Q, den per side, their shared reciprocal, outputs
+	double yy[2] = {0.0, 0.0}, ts[2] = {0.0, 0.0}, Zs[2] = {0.0, 0.0}, Ms[2] = {0.0, 0.0}, Qs[2] = {0.0, 0.0};
+	double dens[2] = {1.0, 1.0}, is = 0.0;
+	bool rr[2] = {false, false};
+	double Rs[2] = {0.0, 0.0}, Es[2] = {0.0, 0.0}, Us[2] = {0.0, 0.0}, head[2], tail[2];
+	if (vac) {
+		head[0] = UB - f.CB;
+		tail[0] = UB + 2.0 * f.CB / K.AGAM;
+		tail[1] = UE - 2.0 * f.CE / K.AGAM;
+		head[1] = UE + f.CE;
+	} else {
+		P0 = (PB * f.RCE + PE * f.RCB + f.DU * f.RCB * f.RCE) / (f.RCB + f.RCE);
+		P = P0;
+		for (;;) {
+			RimNewton s = rim_newton(f, P, PB, PE, K);
+			P = s.Pc + s.res / s.S;
+			++n;
+			if (!(fabs(s.res) > eps)) break;
+			if (max_iter > 0 && n >= max_iter) break;
+		}
+		if (n > RIM_REV_MAX) return false;
+		for (int sd = 0; sd < 2; sd++) {
+			double PK = sd ? PE : PB, IPK = sd ? f.IPE : f.IPB, IRK = sd ? f.IRE : f.IRB, C = sd ? f.CE : f.CB;
+			double RK = sd ? RE : RB, sg = sd ? 1.0 : -1.0;
+			rr[sd] = PK > P;
+			ts[sd] = rr[sd] ? P * IPK : (K.TGAM * P + K.HGAM * PK) * IRK;
+			yy[sd] = side_root(ts[sd], rr[sd], K);
+			Zs[sd] = C * yy[sd];
+			Ms[sd] = (-sg) * (RK * yy[sd]);
+			Qs[sd] = PK - P + Ms[sd] * ((-sg) * yy[sd]);
+			dens[sd] = rr[sd] ? K.AGAM * (Zs[sd] * Zs[sd] * K.ISGAM) : Qs[sd] * Ms[sd];
+		}
+		is = 1.0 / (dens[0] * dens[1]);
+		for (int sd = 0; sd < 2; sd++) {
+			double C = sd ? f.CE : f.CB, UK = sd ? UE : UB, sg = sd ? 1.0 : -1.0;
+			double r = dens[1 - sd] * is;
+			if (rr[sd]) {
+				Es[sd] = Zs[sd] * Zs[sd] * K.ISGAM;
+				Us[sd] = UK + (sg * K.DGAM) * (Zs[sd] - C);
+				Rs[sd] = P * r;
+				head[sd] = UK + sg * C;
+				tail[sd] = Us[sd] + sg * Zs[sd];
+			} else {
+				double IM = Qs[sd] * r, D = UK + sg * yy[sd];
+				Rs[sd] = Ms[sd] * Ms[sd] * Ms[sd] * r;
+				Us[sd] = D + Qs[sd] * IM;
+				Es[sd] = P * Qs[sd] * IM * IM * K.IAGAM;
+				head[sd] = D;
+				tail[sd] = D;
+			}
+		}
+	}
+	// sampling (SBL, SFL = head[0], tail[0]; SEL, SSL = head[1], tail[1])
+	double RI, EI;
+	int br;
+	if (head[1] <= 0.0) {
+		br = 0;
+		RI = RE;
+		EI = f.EE;
+	} else if (head[0] >= 0.0) {
+		br = 1;
+		RI = RB;
+		EI = f.EB;
+	} else if (tail[1] >= 0.0 && tail[0] <= 0.0) {
+		br = Us[1] >= 0.0 ? 2 : 3;
+		RI = br == 2 ? Rs[0] : Rs[1];
+		EI = br == 2 ? Es[0] : Es[1];
+	} else {
+		br = tail[0] > 0.0 ? 4 : 5;
+		RI = EI = 0.0;  // the fan's values are formed in its reverse below
+	}
+	// PI = AGAM EI RI; the fans' RI, EI depend on each other, handled there
+	double RIt = RIb, EIt = EIb;
+	if (br < 4) {
+		RIt += PIb * K.AGAM * EI;
+		EIt += PIb * K.AGAM * RI;
+	}
+	double Rsb[2] = {0.0, 0.0}, Esb[2] = {0.0, 0.0}, Usb[2] = {0.0, 0.0};
+	switch (br) {
+	case 0:
+		a.RE += RIt;
+		a.EE += EIt;
+		a.UE += UIb;
+		a.VE += VIb;
+		break;
+	case 1:
+		a.RB += RIt;
+		a.EB += EIt;
+		a.UB += UIb;
+		a.VB += VIb;
+		break;
+	case 2:
+		Rsb[0] += RIt;
+		Esb[0] += EIt;
+		Usb[0] += UIb;
+		a.VB += VIb;
+		break;
+	case 3:
+		Rsb[1] += RIt;
+		Esb[1] += EIt;
+		Usb[1] += UIb;
+		a.VE += VIb;
+		break;
+	default: {
+		// UI = (UK +- DGGAM sqrt(EK)) IDGAM1, EI = UI^2 ISGAM, RI = RK (EI/EK)^(1/AGAM),
+		// PI = AGAM EI RI (left fan: K = B and +; right: K = E and -)
+		bool left = br == 4;
+		double UK = left ? UB : UE, EK = left ? f.EB : f.EE, RK = left ? RB : RE, pm = left ? 1.0 : -1.0;
+		double sq = sqrt(EK);
+		double ui = (UK + pm * (K.DGGAM * sq)) * K.IDGAM1;
+		double ei = ui * ui * K.ISGAM;
+		double w = ei / EK, pw = pow_iagam(w, K);
+		double ri = RK * pw;
+		double rib = RIb + PIb * K.AGAM * ei, eib = EIb + PIb * K.AGAM * ri;
+		double pwb = rib * RK;
+		double RKb = rib * pw;
+		double wb = pwb * (K.IAGAM * pw / w);
+		eib += wb / EK;
+		double EKb = -wb * ei / (EK * EK);
+		double uib = UIb + eib * 2.0 * ui * K.ISGAM;
+		double UKb = uib * K.IDGAM1;
+		EKb += pm * uib * K.IDGAM1 * K.DGGAM / (2.0 * sq);
+		if (left) {
+			a.RB += RKb;
+			a.EB += EKb;
+			a.UB += UKb;
+			a.VB += VIb;
+		} else {
+			a.RE += RKb;
+			a.EE += EKb;
+			a.UE += UKb;
+			a.VE += VIb;
+		}
+	}
+	}
+	if (!vac) {
+		// the star states
+		double Pb = 0.0, isb = 0.0, denb[2] = {0.0, 0.0};
+		double yb[2] = {0.0, 0.0}, Zb[2] = {0.0, 0.0}, Mb[2] = {0.0, 0.0}, Qb[2] = {0.0, 0.0};
+		for (int sd = 0; sd < 2; sd++) {
+			double C = sd ? f.CE : f.CB, sg = sd ? 1.0 : -1.0;
+			double r = dens[1 - sd] * is, rb = 0.0;
+			if (rr[sd]) {
+				Zb[sd] += Esb[sd] * 2.0 * Zs[sd] * K.ISGAM;
+				(sd ? a.UE : a.UB) += Usb[sd];
+				Zb[sd] += Usb[sd] * (sg * K.DGAM);
+				(sd ? a.CE : a.CB) -= Usb[sd] * (sg * K.DGAM);
+				Pb += Rsb[sd] * r;
+				rb += Rsb[sd] * P;
+			} else {
+				double IM = Qs[sd] * r, M = Ms[sd], Q = Qs[sd];
+				Mb[sd] += Rsb[sd] * 3.0 * M * M * r;
+				rb += Rsb[sd] * M * M * M;
+				double IMb = Usb[sd] * Q;
+				Qb[sd] += Usb[sd] * IM;
+				(sd ? a.UE : a.UB) += Usb[sd];  // D = UK + sg y
+				yb[sd] += Usb[sd] * sg;
+				double e2 = Esb[sd] * K.IAGAM;
+				Pb += e2 * Q * IM * IM;
+				Qb[sd] += e2 * P * IM * IM;
+				IMb += e2 * 2.0 * P * Q * IM;
+				Qb[sd] += IMb * r;
+				rb += IMb * Q;
+			}
+			denb[1 - sd] += rb * is;
+			isb += rb * dens[1 - sd];
+		}
+		double q = dens[0] * dens[1], qb = -isb / (q * q);
+		denb[0] += qb * dens[1];
+		denb[1] += qb * dens[0];
+		for (int sd = 0; sd < 2; sd++) {
+			double C = sd ? f.CE : f.CB, RK = sd ? RE : RB, IPK = sd ? f.IPE : f.IPB, IRK = sd ? f.IRE : f.IRB;
+			double PK = sd ? PE : PB, sg = sd ? 1.0 : -1.0;
+			if (rr[sd]) {
+				Zb[sd] += denb[sd] * K.AGAM * K.ISGAM * 2.0 * Zs[sd];
+			} else {
+				Qb[sd] += denb[sd] * Ms[sd];
+				Mb[sd] += denb[sd] * Qs[sd];
+			}
+			(sd ? a.PE : a.PB) += Qb[sd];
+			Pb -= Qb[sd];
+			Mb[sd] += Qb[sd] * ((-sg) * yy[sd]);
+			yb[sd] += Qb[sd] * Ms[sd] * (-sg);
+			(sd ? a.RE : a.RB) += Mb[sd] * (-sg) * yy[sd];
+			yb[sd] += Mb[sd] * (-sg) * RK;
+			(sd ? a.CE : a.CB) += Zb[sd] * yy[sd];
+			yb[sd] += Zb[sd] * C;
+			double tb = yb[sd] * ((rr[sd] ? K.OGAM : 0.5) * yy[sd] / ts[sd]);
+			if (rr[sd]) {
+				Pb += tb * IPK;
+				(sd ? a.IPE : a.IPB) += tb * P;
+			} else {
+				double u = K.TGAM * P + K.HGAM * PK;
+				Pb += tb * IRK * K.TGAM;
+				(sd ? a.PE : a.PB) += tb * IRK * K.HGAM;
+				(sd ? a.IRE : a.IRB) += tb * u;
+			}
+		}
+		// the Newton iterations, last to first, each from its recomputed start
+		for (int k = n - 1; k >= 0; k--) {
+			double Pk = P0;
+			for (int m = 0; m < k; m++) {
+				RimNewton s = rim_newton(f, Pk, PB, PE, K);
+				Pk = s.Pc + s.res / s.S;
+			}
+			RimNewton s = rim_newton(f, Pk, PB, PE, K);
+			Pb = rim_newton_rev(f, s, PB, PE, K, Pb, a);
+		}
+		// the acoustic initial guess
+		double num = PB * f.RCE + PE * f.RCB + f.DU * f.RCB * f.RCE, dd = f.RCB + f.RCE;
+		double numb = Pb / dd, ddb = -Pb * num / (dd * dd);
+		a.PB += numb * f.RCE;
+		a.RCE += numb * PB;
+		a.PE += numb * f.RCB;
+		a.RCB += numb * PE;
+		a.DU += numb * f.RCB * f.RCE;
+		a.RCB += numb * f.DU * f.RCE;
+		a.RCE += numb * f.DU * f.RCB;
+		a.RCB += ddb;
+		a.RCE += ddb;
+	}
+	// the setup
+	a.UB += a.DU;
+	a.UE -= a.DU;
+	a.CB += a.EB * 2.0 * f.CB * K.ISGAM;
+	a.CE += a.EE * 2.0 * f.CE * K.ISGAM;
+	a.RB += a.RCB * f.CB;
+	a.CB += a.RCB * RB;
+	a.RE += a.RCE * f.CE;
+	a.CE += a.RCE * RE;
+	double abB = a.CB / (2.0 * f.CB), abE = a.CE / (2.0 * f.CE);
+	a.PB += abB * K.GAM * f.IRB;
+	a.IRB += abB * K.GAM * PB;
+	a.PE += abE * K.GAM * f.IRE;
+	a.IRE += abE * K.GAM * PE;
+	a.PB += a.IRB * f.iB;
+	a.RB += a.IPB * f.iB;
+	double iBb = a.IRB * PB + a.IPB * RB;
+	a.PE += a.IRE * f.iE;
+	a.RE += a.IPE * f.iE;
+	double iEb = a.IRE * PE + a.IPE * RE;
+	double rpEb = iBb * f.i4, rpBb = iEb * f.i4;
+	double i4b = iBb * f.rpE + iEb * f.rpB;
+	double q4 = f.rpB * f.rpE, q4b = -i4b / (q4 * q4);
+	rpBb += q4b * f.rpE;
+	rpEb += q4b * f.rpB;
+	a.RB += rpBb * PB;
+	a.PB += rpBb * RB;
+	a.RE += rpEb * PE;
+	a.PE += rpEb * RE;
+	xb[0] = a.RB;
+	xb[1] = a.PB;
+	xb[2] = a.UB;
+	xb[3] = a.VB;
+	xb[4] = a.RE;
+	xb[5] = a.PE;
+	xb[6] = a.UE;
+	xb[7] = a.VE;
+	return true;
+}
+
+// The exact Godunov flux's vector-Jacobian product.  The solver's primal
+// outputs first (rim), then the flux assembly and the rotation to the normal
+// frame reversed by hand, then the solver itself by rim_vjp.  When the
+// Newton loop ran too long for rim_vjp, forward mode instead: the solver's
+// outputs depend on six inputs (density, pressure and normal velocity of
+// each side; the tangential velocities only pass through, VI is VB or VE),
+// so one dual pass (rim_d1) per one of those six, the first also seeding VB,
+// whose tangent in VI tells which side was sampled.  The normal's adjoint (nb
+// given) comes from the rotation and the assembly alone.
 __device__ __forceinline__ void godunov_vjp(const Prim& L, const Prim& R, double nx, double ny, const RimConst& K,
                                             int max_iter, const double* fb, double* Lb, double* Rb,
                                             double* nb = nullptr)
 {
 	double unl = L.u * nx + L.v * ny, unr = R.u * nx + R.v * ny;
 	double utl = L.u * ny - L.v * nx, utr = R.u * ny - R.v * nx;
-	const double in[6] = {L.r, L.p, unl, R.r, R.p, unr};
-	double xb[6];  // adjoints of the six solver inputs
-	double RIb = 0.0, EIb = 0.0, PIb = 0.0, UNb = 0.0, UTb = 0.0, nxb = 0.0, nyb = 0.0, sel = 0.0;
-	for (int j = 0; j < 6; j++) {
-		D1 x[6];
-		for (int q = 0; q < 6; q++) x[q] = D1{in[q], q == j ? 1.0 : 0.0};
-		D1 RI, EI, PI, UN, UT;
-		rim_d1(RI, EI, PI, UN, UT, x[0], x[1], x[2], D1{utl, j == 0 ? 1.0 : 0.0}, x[3], x[4], x[5], D1{utr, 0.0}, K,
-		       max_iter);
-		if (j == 0) {
-			// the primal outputs: reverse of UI = UN nx + UT ny, VI = UN ny - UT nx,
-			// f0 = RI UN, f1 = f0 UI + PI nx, f2 = f0 VI + PI ny,
-			// f3 = (RI (EI + (UI^2 + VI^2)/2) + PI) UN
-			double ri = RI.v, ei = EI.v, pi = PI.v, un = UN.v, ut = UT.v;
-			double ui = un * nx + ut * ny, vi = un * ny - ut * nx;
-			double f0 = ri * un, h = ei + 0.5 * (ui * ui + vi * vi), A = ri * h + pi;
-			double f0b = fb[0] + fb[1] * ui + fb[2] * vi;
-			double uib = fb[1] * f0 + fb[3] * un * ri * ui;
-			double vib = fb[2] * f0 + fb[3] * un * ri * vi;
-			PIb = fb[1] * nx + fb[2] * ny + fb[3] * un;
-			RIb = f0b * un + fb[3] * un * h;
-			EIb = fb[3] * un * ri;
-			UNb = f0b * ri + fb[3] * A + uib * nx + vib * ny;
-			UTb = uib * ny - vib * nx;
-			nxb = fb[1] * pi + uib * un - vib * ut;
-			nyb = fb[2] * pi + uib * ut + vib * un;
-			sel = UT.d;
+	double RI, EI, PI, UN, UT;
+	int hit = 0;
+	rim(RI, EI, PI, UN, UT, L.r, L.p, unl, utl, R.r, R.p, unr, utr, K, max_iter, hit);
+	// reverse of UI = UN nx + UT ny, VI = UN ny - UT nx, f0 = RI UN,
+	// f1 = f0 UI + PI nx, f2 = f0 VI + PI ny, f3 = (RI (EI + (UI^2 + VI^2)/2) + PI) UN
+	double ui = UN * nx + UT * ny, vi = UN * ny - UT * nx;
+	double f0 = RI * UN, h = EI + 0.5 * (ui * ui + vi * vi), A = RI * h + PI;
+	double f0b = fb[0] + fb[1] * ui + fb[2] * vi;
+	double uib = fb[1] * f0 + fb[3] * UN * RI * ui;
+	double vib = fb[2] * f0 + fb[3] * UN * RI * vi;
+	double PIb = fb[1] * nx + fb[2] * ny + fb[3] * UN;
+	double RIb = f0b * UN + fb[3] * UN * h;
+	double EIb = fb[3] * UN * RI;
+	double UNb = f0b * RI + fb[3] * A + uib * nx + vib * ny;
+	double UTb = uib * ny - vib * nx;
+	double nxb = fb[1] * PI + uib * UN - vib * UT;
+	double nyb = fb[2] * PI + uib * UT + vib * UN;
+	double xb[8];  // adjoints of (RB, PB, UB, VB, RE, PE, UE, VE)
+	if (!rim_vjp(L.r, L.p, unl, utl, R.r, R.p, unr, utr, K, max_iter, RIb, EIb, PIb, UNb, UTb, xb)) {
+		const double in[6] = {L.r, L.p, unl, R.r, R.p, unr};
+		double x6[6], sel = 0.0;
+		for (int j = 0; j < 6; j++) {
+			D1 x[6];
+			for (int q = 0; q < 6; q++) x[q] = D1{in[q], q == j ? 1.0 : 0.0};
+			D1 dRI, dEI, dPI, dUN, dUT;
+			rim_d1(dRI, dEI, dPI, dUN, dUT, x[0], x[1], x[2], D1{utl, j == 0 ? 1.0 : 0.0}, x[3], x[4], x[5],
+			       D1{utr, 0.0}, K, max_iter);
+			if (j == 0) sel = dUT.d;
+			x6[j] = RIb * dRI.d + EIb * dEI.d + PIb * dPI.d + UNb * dUN.d;
 		}
-		xb[j] = RIb * RI.d + EIb * EI.d + PIb * PI.d + UNb * UN.d;
+		xb[0] = x6[0];
+		xb[1] = x6[1];
+		xb[2] = x6[2];
+		xb[3] = UTb * sel;
+		xb[4] = x6[3];
+		xb[5] = x6[4];
+		xb[6] = x6[5];
+		xb[7] = UTb * (1.0 - sel);
 	}
-	double vbl = UTb * sel, vbr = UTb * (1.0 - sel);  // adjoints of the tangential velocities
 	Lb[0] = xb[0];
 	Lb[1] = xb[1];
-	Lb[2] = xb[2] * nx + vbl * ny;
-	Lb[3] = xb[2] * ny - vbl * nx;
-	Rb[0] = xb[3];
-	Rb[1] = xb[4];
-	Rb[2] = xb[5] * nx + vbr * ny;
-	Rb[3] = xb[5] * ny - vbr * nx;
+	Lb[2] = xb[2] * nx + xb[3] * ny;
+	Lb[3] = xb[2] * ny - xb[3] * nx;
+	Rb[0] = xb[4];
+	Rb[1] = xb[5];
+	Rb[2] = xb[6] * nx + xb[7] * ny;
+	Rb[3] = xb[6] * ny - xb[7] * nx;
 	if (nb) {  // u_n = u nx + v ny, u_t = u ny - v nx on each side
-		nb[0] = nxb + xb[2] * L.u - vbl * L.v + xb[5] * R.u - vbr * R.v;
-		nb[1] = nyb + xb[2] * L.v + vbl * L.u + xb[5] * R.v + vbr * R.u;
+		nb[0] = nxb + xb[2] * L.u - xb[3] * L.v + xb[6] * R.u - xb[7] * R.v;
+		nb[1] = nyb + xb[2] * L.v + xb[3] * L.u + xb[6] * R.v + xb[7] * R.u;
 	}
 }
 

@@ -181,3 +181,58 @@ extern "C" int cfd2d_flux(int kind, int n, const double* rows, double gam, int m
 {
 	return pw_alloc_run(kind, n, rows, gam, max_newton, out, nullptr, cap_hits);
 }
+
+namespace {
+
+// one row per thread: the vector-Jacobian product of a flux the adjoint
+// differentiates, (Lb, Rb, nb) = fb^T dF/d(L, R, n)
+__global__ __launch_bounds__(BLOCK) void k_pw_vjp(int kind, int n, const double* rows, const double* fbar,
+                                                  double* out, RimConst K, int max_newton)
+{
+	int i = blockIdx.x * BLOCK + threadIdx.x;
+	if (i >= n) return;
+	const double* r = rows + 10 * (size_t)i;
+	Prim L{r[0], r[1], r[2], r[3]}, R{r[4], r[5], r[6], r[7]};
+	double fb[4], Lb[4], Rb[4], nb[2];
+	for (int q = 0; q < 4; q++) fb[q] = fbar[4 * (size_t)i + q];
+	if (kind == CFD2D_FLUX_LAX) lax_vjp(L, R, r[8], r[9], K.GAM, 1.0 / (K.GAM - 1.0), fb, Lb, Rb, nb);
+	else if (kind == CFD2D_FLUX_HLLC) hllc_vjp(L, R, r[8], r[9], K.GAM, fb, Lb, Rb, nb);
+	else godunov_vjp(L, R, r[8], r[9], K, max_newton, fb, Lb, Rb, nb);
+	double* o = out + 10 * (size_t)i;
+	for (int q = 0; q < 4; q++) {
+		o[q] = Lb[q];
+		o[4 + q] = Rb[q];
+	}
+	o[8] = nb[0];
+	o[9] = nb[1];
+}
+
+}  // namespace
+
+extern "C" int cfd2d_flux_vjp(int kind, int n, const double* rows, const double* fbar, double gam, int max_newton,
+                              double* out)
+{
+	if (n < 0 || (n > 0 && (!rows || !fbar || !out))) return fail(CFD2D_ERR_ARG, "cfd2d_flux_vjp: bad arguments");
+	if (kind != CFD2D_FLUX_GODUNOV && kind != CFD2D_FLUX_LAX && kind != CFD2D_FLUX_HLLC)
+		return fail(CFD2D_ERR_ARG, "cfd2d_flux_vjp: the adjoint's fluxes are GODUNOV, LAX and HLLC");
+	if (!(gam > 1.0)) return fail(CFD2D_ERR_ARG, "cfd2d_flux_vjp: gamma must exceed 1");
+	if (n == 0) return CFD2D_OK;
+	double* st = nullptr;
+	size_t nd = 24 * (size_t)n;  // rows [n][10], fbar [n][4], out [n][10]
+	HIPCHK(hipMalloc(&st, nd * sizeof(double)));
+	int rc = CFD2D_OK;
+	hipError_t e = hipMemcpy(st, rows, 10 * (size_t)n * sizeof(double), hipMemcpyHostToDevice);
+	if (e == hipSuccess) e = hipMemcpy(st + 10 * (size_t)n, fbar, 4 * (size_t)n * sizeof(double), hipMemcpyHostToDevice);
+	if (e == hipSuccess) {
+		int mn = max_newton > 0 ? max_newton : CFD2D_NEWTON_CAP;
+		hipLaunchKernelGGL(k_pw_vjp, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, 0, kind, n, (const double*)st,
+		                   (const double*)(st + 10 * (size_t)n), st + 14 * (size_t)n, rim_const(gam), mn);
+		e = hipGetLastError();
+	}
+	if (e == hipSuccess) e = hipMemcpy(out, st + 14 * (size_t)n, 10 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost);
+	if (e != hipSuccess) rc = hip_fail(e, "cfd2d_flux_vjp");
+	hipError_t e2 = hipFree(st);
+	if (rc) return rc;
+	if (e2 != hipSuccess) return hip_fail(e2, "hipFree");
+	return CFD2D_OK;
+}

@@ -319,6 +319,16 @@ int  cfd2d_flux(int kind, int n, const double* rows, double gam, int max_newton,
 enum { CFD2D_NOALLOC_MAX_ROWS = 65536 };
 int  cfd2d_flux_noalloc(int kind, int n, const double* rows, double gam, int max_newton, double* out,
                         long long* cap_hits);
+/* The vector-Jacobian product of a flux the adjoint differentiates (kind
+ * CFD2D_FLUX_GODUNOV, LAX or HLLC) on n state pairs: rows double[n][10] as
+ * cfd2d_flux, fbar double[n][4] the seeds; out double[n][10] = fbar^T dF/dx
+ * for x = (rL, pL, uL, vL, rR, pR, uR, vR, nx, ny), the normal's components
+ * taken as independent.  The same device functions as cfd2d_adjoint's
+ * reverse sweep (the exact Godunov flux: the Newton iterations as evaluated,
+ * reversed).  Host arrays, staged through device memory allocated for the
+ * call.  No counterpart in the reference. */
+int  cfd2d_flux_vjp(int kind, int n, const double* rows, const double* fbar, double gam, int max_newton,
+                    double* out);
 
 /* Page-locked host memory (hipHostMalloc): output buffers of
  * cfd2d_implicit_assemble placed here are copied at the link's full rate.

@@ -14,6 +14,8 @@ agreement is to round-off, not bitwise:
       stop test within rounding)  1e-8, and at most 0.5 % of the rows
   fluxes  1e-12 relative to the row's flux scale (Godunov: rows that
       stopped one iteration apart 1e-8)
+The flux VJPs (cfd2d_flux_vjp, the reverse sweep's functions) are checked
+against the tangent model by the dot-product identity.
 """
 import math
 
@@ -146,3 +148,72 @@ def test_empty_and_invalid_inputs():
         native.flux("LAX", row, 1.0)
     with pytest.raises(native.Cfd2dError):
         native.flux("LAX", np.repeat(row, native.NOALLOC_MAX_ROWS + 1, axis=0), GAM, noalloc=True)
+
+
+def vacuum_rows():
+    """State pairs that separate fast enough to open a vacuum (u_R - u_L = 16
+    against 2 (c_L + c_R) / (gamma - 1) = 11.8), shifted in frame so that the
+    sample at x/t = 0 falls in the left state, the left fan, the vacuum, the
+    right fan or the right state; turned to a slanted normal."""
+    s = np.linspace(-14.0, 14.0, 57)
+    th = 0.7
+    nx, ny = np.cos(th), np.sin(th)
+    rows = np.empty((len(s), 10))
+    for side, (r, p, un) in enumerate([(1.0, 1.0, -8.0 + s), (0.8, 0.6, 8.0 + s)]):
+        ut = 0.3 * (side + 1) * np.ones_like(s)
+        rows[:, 4 * side: 4 * side + 4] = np.stack([np.full_like(s, r), np.full_like(s, p), un * nx + ut * ny,
+                                                    un * ny - ut * nx], axis=1)
+    rows[:, 8], rows[:, 9] = nx, ny
+    return rows
+
+
+def _tangent_flux(kind, rows, v):
+    """J(rows) v of the tangent model's flux (oracle/tangent.py: the
+    reference's rim_orig and the LAX and HLLC fluxes on row-vectorised dual
+    numbers, the normal's components independent)."""
+    import tangent
+    m = tangent.Model.__new__(tangent.Model)
+    m.gam_flux, m.flux = GAM, kind
+    D = [tangent.Dual(rows[:, k].copy(), v[:, k].copy()) for k in range(10)]
+    with np.errstate(invalid="ignore", divide="ignore"):  # both sides of each where() are evaluated
+        f = m.numflux(D[0:4], D[4:8], D[8], D[9])
+    return np.stack([fq.d for fq in f], axis=1)
+
+
+@pytest.mark.parametrize("kind", ["GODUNOV", "LAX", "HLLC"])
+def test_flux_vjp_dot_product_with_tangent_model(kind):
+    """cfd2d_flux_vjp against the tangent model by the dot-product identity
+    <fbar, J v> = <J^T fbar, v>, for random seeds and directions, on 3 000
+    random state pairs, the Riemann corpus (Toro's tests, near-vacuum, strong
+    shocks) and pairs that open a vacuum: star, fan, vacuum and outer-state
+    samples all taken.  The exact
+    Godunov flux's VJP is the reverse-mode solver; rows whose Newton loop
+    stopped at another iteration than the tangent model's (the residual on
+    the stop test within rounding, at most 0.5 %) differentiate another map
+    and are left out.  Tolerance 1e-9 of the sum of the products' magnitudes
+    (the tangent model divides where the device multiplies by reciprocals)."""
+    rows = np.concatenate([random_rows(3000, 3), corpus_rows(), vacuum_rows()])
+    rng = np.random.default_rng(17)
+    fb = rng.standard_normal((len(rows), 4))
+    v = rng.standard_normal((len(rows), 10)) * np.abs(rows)
+    vjp = native.flux_vjp(kind, rows, fb, GAM, max_newton=500)
+    jv = _tangent_flux(kind, rows, v)
+    a = np.sum(vjp * v, axis=1)
+    b = np.sum(fb * jv, axis=1)
+    scale = np.sum(np.abs(vjp * v), axis=1) + np.sum(np.abs(fb * jv), axis=1) + 1e-300
+    err = np.abs(a - b) / scale
+    assert np.all(np.isfinite(vjp))
+    keep = np.ones(len(rows), bool)
+    if kind == "GODUNOV":
+        un = lambda s: rows[:, 4 * s + 2] * rows[:, 8] + rows[:, 4 * s + 3] * rows[:, 9]
+        ut = lambda s: rows[:, 4 * s + 2] * rows[:, 9] - rows[:, 4 * s + 3] * rows[:, 8]
+        rr = np.stack([rows[:, 0], rows[:, 1], un(0), ut(0), rows[:, 4], rows[:, 5], un(1), ut(1)], axis=1)
+        _, its, _ = native.riemann(rr, GAM, 500)
+        _, its_o = oracle.rim(np.concatenate([rr, np.full((len(rr), 1), GAM)], axis=1))
+        keep = its == its_o
+        assert (~keep).sum() <= 0.005 * len(rows)
+        print("iterations per row: max %d, rows on the vacuum branch %d" % (its.max(), (its == 0).sum()))
+        assert (its == 0).sum() >= 40  # the vacuum rows
+        assert (its > 8).any()  # rows past the reverse solver's recomputation limit: the forward-mode path
+    print("%s: rows %d, kept %d, max rel err %.3e" % (kind, len(rows), keep.sum(), err[keep].max()))
+    assert err[keep].max() <= 1e-9
