@@ -877,63 +877,138 @@ __device__ __forceinline__ void godunov_vjp(const Prim& L, const Prim& R, double
 	}
 }
 
-// HLLC (the same formulas as flux_hllc) on one-direction duals
-__device__ __forceinline__ void hllc_d1(D1* f, const D1* L, const D1* R, D1 nx, D1 ny, double G)
+// adjoint accumulators of one side's state (r, p, u, v), its normal
+// velocity, energy r e + r |u|^2/2 and sound speed
+struct SideAdj {
+	double r = 0.0, p = 0.0, u = 0.0, v = 0.0, un = 0.0, E = 0.0, c = 0.0;
+};
+
+// reverse of the Euler flux (r un, r u un + p nx, r v un + p ny, (E + p) un)
+// of one side for seeds g; nxb, nyb take the normal's direct terms
+__device__ __forceinline__ void euler_flux_rev(const Prim& W, double un, double E, double nx, double ny,
+                                               const double* g, SideAdj& a, double& nxb, double& nyb)
 {
-	D1 unl = L[2] * nx + L[3] * ny, unr = R[2] * nx + R[3] * ny;
-	D1 cl = dsqrt(G * L[1] / L[0]), cr = dsqrt(G * R[1] / R[0]);
-	D1 El = L[1] / (G - 1.0) + 0.5 * L[0] * (L[2] * L[2] + L[3] * L[3]);
-	D1 Er = R[1] / (G - 1.0) + 0.5 * R[0] * (R[2] * R[2] + R[3] * R[3]);
-	D1 a1 = unl - cl, a2 = unr - cr, b1 = unl + cl, b2 = unr + cr;
-	D1 sl = a1.v < a2.v ? a1 : a2;
-	D1 sr = b1.v > b2.v ? b1 : b2;
-	if (sl.v >= 0.0) {
-		f[0] = L[0] * unl;
-		f[1] = L[0] * L[2] * unl + L[1] * nx;
-		f[2] = L[0] * L[3] * unl + L[1] * ny;
-		f[3] = (El + L[1]) * unl;
-		return;
-	}
-	if (sr.v <= 0.0) {
-		f[0] = R[0] * unr;
-		f[1] = R[0] * R[2] * unr + R[1] * nx;
-		f[2] = R[0] * R[3] * unr + R[1] * ny;
-		f[3] = (Er + R[1]) * unr;
-		return;
-	}
-	D1 ml = L[0] * (sl - unl), mr = R[0] * (sr - unr);
-	D1 ss = (R[1] - L[1] + L[0] * unl * (sl - unl) - R[0] * unr * (sr - unr)) / (ml - mr);
-	bool lft = ss.v >= 0.0;
-	const D1* K = lft ? L : R;
-	D1 sk = lft ? sl : sr, unk = lft ? unl : unr, Ek = lft ? El : Er;
-	D1 k = (lft ? ml : mr) / (sk - ss);
-	D1 us0 = k, us1 = k * (K[2] + (ss - unk) * nx), us2 = k * (K[3] + (ss - unk) * ny);
-	D1 us3 = k * (Ek / K[0] + (ss - unk) * (ss + K[1] / (K[0] * (sk - unk))));
-	D1 fk0 = K[0] * unk, fk1 = K[0] * K[2] * unk + K[1] * nx, fk2 = K[0] * K[3] * unk + K[1] * ny;
-	D1 fk3 = (Ek + K[1]) * unk;
-	f[0] = fk0 + sk * (us0 - K[0]);
-	f[1] = fk1 + sk * (us1 - K[0] * K[2]);
-	f[2] = fk2 + sk * (us2 - K[0] * K[3]);
-	f[3] = fk3 + sk * (us3 - Ek);
+	a.r += g[0] * un + g[1] * W.u * un + g[2] * W.v * un;
+	a.un += g[0] * W.r + g[1] * W.r * W.u + g[2] * W.r * W.v + g[3] * (E + W.p);
+	a.u += g[1] * W.r * un;
+	a.v += g[2] * W.r * un;
+	a.p += g[1] * nx + g[2] * ny + g[3] * un;
+	a.E += g[3] * un;
+	nxb += g[1] * W.p;
+	nyb += g[2] * W.p;
 }
 
-// nb (if given): the adjoint of the normal, two more forward-mode directions
+// the state's own terms: E = p/(G - 1) + r |u|^2 / 2, c = sqrt(G p / r),
+// un = u nx + v ny
+__device__ __forceinline__ void side_rev(const Prim& W, double c, double nx, double ny, double G, SideAdj& a,
+                                         double* Wb, double& nxb, double& nyb)
+{
+	double r = a.r, p = a.p, u = a.u, v = a.v;
+	p += a.E / (G - 1.0);
+	r += a.E * 0.5 * (W.u * W.u + W.v * W.v);
+	u += a.E * W.r * W.u;
+	v += a.E * W.r * W.v;
+	double ab = a.c / (2.0 * c);  // c = sqrt(a), a = G p / r
+	p += ab * G / W.r;
+	r -= ab * (G * W.p) / (W.r * W.r);
+	u += a.un * nx;
+	v += a.un * ny;
+	nxb += a.un * W.u;
+	nyb += a.un * W.v;
+	Wb[0] = r;
+	Wb[1] = p;
+	Wb[2] = u;
+	Wb[3] = v;
+}
+
+// The HLLC flux's vector-Jacobian product (flux_hllc reversed by hand: the
+// outer states' Euler fluxes, or the contact speed, star state and wave
+// speeds of the star region); nb (if given) takes the normal's adjoint
 __device__ __forceinline__ void hllc_vjp(const Prim& L, const Prim& R, double nx, double ny, double G, const double* fb,
                                          double* Lb, double* Rb, double* nb = nullptr)
 {
-	const double lv[4] = {L.r, L.p, L.u, L.v}, rv[4] = {R.r, R.p, R.u, R.v};
-	int nd = nb ? 10 : 8;
-	for (int j = 0; j < nd; j++) {
-		D1 l[4], r[4], f[4];
-		for (int q = 0; q < 4; q++) {
-			l[q] = D1{lv[q], (j == q) ? 1.0 : 0.0};
-			r[q] = D1{rv[q], (j == 4 + q) ? 1.0 : 0.0};
-		}
-		hllc_d1(f, l, r, D1{nx, j == 8 ? 1.0 : 0.0}, D1{ny, j == 9 ? 1.0 : 0.0}, G);
-		double s = fb[0] * f[0].d + fb[1] * f[1].d + fb[2] * f[2].d + fb[3] * f[3].d;
-		if (j < 4) Lb[j] = s;
-		else if (j < 8) Rb[j - 4] = s;
-		else nb[j - 8] = s;
+	double unl = L.u * nx + L.v * ny, unr = R.u * nx + R.v * ny;
+	double cl = sqrt(G * L.p / L.r), cr = sqrt(G * R.p / R.r);
+	double El = L.p / (G - 1.0) + 0.5 * L.r * (L.u * L.u + L.v * L.v);
+	double Er = R.p / (G - 1.0) + 0.5 * R.r * (R.u * R.u + R.v * R.v);
+	double a1 = unl - cl, a2 = unr - cr, b1 = unl + cl, b2 = unr + cr;
+	bool sl1 = a1 < a2, sr1 = b1 > b2;
+	double sl = sl1 ? a1 : a2, sr = sr1 ? b1 : b2;
+	SideAdj al, ar;
+	double nxb = 0.0, nyb = 0.0;
+	if (sl >= 0.0) {
+		euler_flux_rev(L, unl, El, nx, ny, fb, al, nxb, nyb);
+	} else if (sr <= 0.0) {
+		euler_flux_rev(R, unr, Er, nx, ny, fb, ar, nxb, nyb);
+	} else {
+		double ml = L.r * (sl - unl), mr = R.r * (sr - unr);
+		double num = R.p - L.p + L.r * unl * (sl - unl) - R.r * unr * (sr - unr), dens = ml - mr;
+		double ss = num / dens;
+		bool lft = ss >= 0.0;
+		const Prim& K = lft ? L : R;
+		SideAdj& ak = lft ? al : ar;
+		double sk = lft ? sl : sr, unk = lft ? unl : unr, Ek = lft ? El : Er, mk = lft ? ml : mr;
+		double g = sk - ss, k = mk / g;
+		double d = ss - unk, e = sk - unk, he = K.r * e, h = K.p / he;
+		double B3 = ss + h, inner = Ek / K.r + d * B3;
+		double us1 = k * (K.u + d * nx), us2 = k * (K.v + d * ny), us3 = k * inner;
+		// f = F(K) + sk (us - U(K))
+		double skb = fb[0] * (k - K.r) + fb[1] * (us1 - K.r * K.u) + fb[2] * (us2 - K.r * K.v) + fb[3] * (us3 - Ek);
+		double us0b = fb[0] * sk, us1b = fb[1] * sk, us2b = fb[2] * sk, us3b = fb[3] * sk;
+		ak.r -= fb[0] * sk + fb[1] * sk * K.u + fb[2] * sk * K.v;
+		ak.u -= fb[1] * sk * K.r;
+		ak.v -= fb[2] * sk * K.r;
+		ak.E -= fb[3] * sk;
+		euler_flux_rev(K, unk, Ek, nx, ny, fb, ak, nxb, nyb);
+		// us3 = k (Ek/Kr + d (ss + Kp / (Kr e)))
+		double kb = us3b * inner, innerb = us3b * k;
+		ak.E += innerb / K.r;
+		ak.r -= innerb * Ek / (K.r * K.r);
+		double db = innerb * B3, B3b = innerb * d;
+		double ssb = B3b, hb = B3b;
+		ak.p += hb / he;
+		double heb = -hb * K.p / (he * he);
+		ak.r += heb * e;
+		double eb = heb * K.r;
+		skb += eb;
+		ak.un -= eb;
+		// us1, us2 = k (Ku + d nx), k (Kv + d ny); us0 = k
+		kb += us1b * (K.u + d * nx) + us2b * (K.v + d * ny) + us0b;
+		ak.u += us1b * k;
+		ak.v += us2b * k;
+		db += us1b * k * nx + us2b * k * ny;
+		nxb += us1b * k * d;
+		nyb += us2b * k * d;
+		ssb += db;
+		ak.un -= db;
+		// k = mk / (sk - ss)
+		double mkb = kb / g, gb = -kb * mk / (g * g);
+		skb += gb;
+		ssb -= gb;
+		// ss = (Rp - Lp + Lr unl (sl - unl) - Rr unr (sr - unr)) / (ml - mr)
+		double numb = ssb / dens, densb = -ssb * num / (dens * dens);
+		double mlb = densb + (lft ? mkb : 0.0), mrb = -densb + (lft ? 0.0 : mkb);
+		ar.p += numb;
+		al.p -= numb;
+		al.r += numb * unl * (sl - unl) + mlb * (sl - unl);
+		al.un += numb * L.r * (sl - unl) - numb * L.r * unl - mlb * L.r;
+		double slb = numb * L.r * unl + mlb * L.r + (lft ? skb : 0.0);
+		ar.r += -numb * unr * (sr - unr) + mrb * (sr - unr);
+		ar.un += -numb * R.r * (sr - unr) + numb * R.r * unr - mrb * R.r;
+		double srb = -numb * R.r * unr + mrb * R.r + (lft ? 0.0 : skb);
+		// sl = min(unl - cl, unr - cr), sr = max(unl + cl, unr + cr)
+		SideAdj& as = sl1 ? al : ar;
+		as.un += slb;
+		as.c -= slb;
+		SideAdj& at = sr1 ? al : ar;
+		at.un += srb;
+		at.c += srb;
+	}
+	side_rev(L, cl, nx, ny, G, al, Lb, nxb, nyb);
+	side_rev(R, cr, nx, ny, G, ar, Rb, nxb, nyb);
+	if (nb) {
+		nb[0] = nxb;
+		nb[1] = nyb;
 	}
 }
 
