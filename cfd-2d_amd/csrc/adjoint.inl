@@ -16,7 +16,8 @@
 // The vector-Jacobian product of R runs over the primitives P and gradients
 // G the forward pass recorded for that stage.  First the reverse of the flux,
 // of the ghost on boundaries and of the reconstruction into (P, G) adjoints
-// per cell: for HLLC (a VALU-heavy VJP) once per edge, k_adj_edge (two lanes
+// per cell: for HLLC and the exact Godunov flux (VJPs heavy in VALU
+// work) once per edge, k_adj_edge (two lanes
 // per edge, one Gauss point each, into a per-edge buffer for both sides) then
 // k_adj_cell (each cell gathers its faces' entries in ascending edge order);
 // for LAX k_adj_face (each cell evaluates its three faces from its own side,
@@ -247,7 +248,8 @@ __device__ __forceinline__ void lax_vjp_own(const Prim& L, const Prim& R, double
 	Lb[3] = Lvb;
 }
 
-// one-direction forward-mode number for the HLLC Jacobian
+// one-direction forward-mode (dual) number: the exact Godunov flux's VJP
+// past the reverse solver's iteration limit
 struct D1 {
 	double v, d;
 };
@@ -1621,7 +1623,7 @@ void launch_adj_stage(cfd2d_ctx* c, AdjWork* A, double wa, double wb, int mode)
 	// edge reverse forms them, the other fluxes get them from the edge pass
 	if (GEO && c->p.flux != CFD2D_FLUX_LAX) launch_edge<MM>(c);
 	if (c->p.flux == CFD2D_FLUX_HLLC || c->p.flux == CFD2D_FLUX_GODUNOV) {
-		// VALU-heavy VJPs (forward-mode directions): once per edge, through the edge buffer
+		// VJPs heavy in VALU work: once per edge, through the edge buffer
 		if (c->p.flux == CFD2D_FLUX_HLLC)
 			hipLaunchKernelGGL((k_adj_edge<MM, CFD2D_FLUX_HLLC, GEO>), dim3(grid_of(2 * c->ne)), dim3(BLOCK), 0,
 			                   c->st, c->b, c->ph, A->a, wb);
