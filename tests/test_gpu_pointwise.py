@@ -132,8 +132,9 @@ def test_fluxes_match_oracle(kind):
 
 def test_empty_and_invalid_inputs():
     """No rows is a valid call (nothing evaluated, no cap hit); an unknown
-    flux kind, gamma <= 1 and too many rows for the no-allocation entry
-    point are errors, reported as CFD2D_ERR_ARG without touching the device."""
+    flux kind (for the VJP: a flux the adjoint does not differentiate),
+    gamma <= 1 and too many rows for the no-allocation entry point are
+    errors, reported as CFD2D_ERR_ARG without touching the device."""
     out, its, hits = native.riemann(np.zeros((0, 8)), GAM)
     assert out.shape == (0, 5) and its.shape == (0,) and hits == 0
     for kind in ("GODUNOV", "LAX", "HLLC", "ROE"):
@@ -148,6 +149,13 @@ def test_empty_and_invalid_inputs():
         native.flux("LAX", row, 1.0)
     with pytest.raises(native.Cfd2dError):
         native.flux("LAX", np.repeat(row, native.NOALLOC_MAX_ROWS + 1, axis=0), GAM, noalloc=True)
+    # the flux VJP: the adjoint's fluxes only (no Roe), gamma > 1
+    for kind in ("GODUNOV", "LAX", "HLLC"):
+        assert native.flux_vjp(kind, np.zeros((0, 10)), np.zeros((0, 4)), GAM).shape == (0, 10)
+    with pytest.raises(native.Cfd2dError):
+        native.flux_vjp("ROE", row, np.ones((1, 4)), GAM)
+    with pytest.raises(native.Cfd2dError):
+        native.flux_vjp("GODUNOV", row, np.ones((1, 4)), 1.0)
 
 
 def vacuum_rows():
