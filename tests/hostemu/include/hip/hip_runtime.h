@@ -249,6 +249,7 @@ inline int hostemu_mov_dpp(int v, int ctrl, int row_mask, int bank_mask)
 	hostemu::yield();
 	return r;
 }
+#define __builtin_amdgcn_s_setprio(p) ((void)(p))
 #define __builtin_amdgcn_mov_dpp(v, ctrl, row_mask, bank_mask, bound_ctrl) \
 	hostemu_mov_dpp((v), (ctrl), (row_mask), (bank_mask))
 
