@@ -172,7 +172,8 @@ def batched_vs_serial(n_cases, steps, seed=7):
 def adjoint_vs_forward(steps, ckpt, ny=289, flux="LAX"):
     """Adjoint workload: bump channel with ny=289 (501,126 triangles), Mach
     0.8, bump height 0.05; LAX flux (or the reference's exact Godunov flux,
-    whose reverse differentiates the Newton iterations as evaluated),
+    whose reverse differentiates the Newton iterations as evaluated, or
+    HLLC),
     Green-Gauss gradients, Heun RK2, no limiter; J = y pressure force on the
     bump.  Wall time of cfd2d_adjoint
     (forward pass, checkpoints, recomputation, reverse sweep, gradient copied
@@ -493,6 +494,8 @@ def main():
         log("adjoint: %s" % out["adjoint"])
         out["adjoint_godunov"] = adjoint_vs_forward(args.adj_steps, args.adj_ckpt, flux="GODUNOV")
         log("adjoint, exact Godunov flux: %s" % out["adjoint_godunov"])
+        out["adjoint_hllc"] = adjoint_vs_forward(args.adj_steps, args.adj_ckpt, flux="HLLC")
+        log("adjoint, HLLC flux: %s" % out["adjoint_hllc"])
     if not args.no_badj:
         out["batched_adjoint"] = batched_adjoint(args.badj_cases, args.badj_steps)
         log("batched adjoint: %s" % out["batched_adjoint"])
