@@ -8,8 +8,10 @@
 //
 // Execution model: every launch runs at once, one workgroup after another;
 // the workgroup's threads are fibers that meet at __syncthreads and at each
-// lane exchange (__shfl_xor), so kernels run with the same synchronisation
-// they have on the GPU.  A thread that returns takes no part in later
+// lane exchange (__shfl_xor, and the DPP quad permutations of
+// __builtin_amdgcn_mov_dpp), so kernels run with the same synchronisation
+// they have on the GPU; wave priority (__builtin_amdgcn_s_setprio) is a
+// no-op.  A thread that returns takes no part in later
 // barriers, as an exited wave leaves s_barrier.
 // Streams are all the same in-order queue; graph capture records the
 // launches and copies and a graph launch replays them; events read the host
