@@ -822,7 +822,7 @@ __device__ __forceinline__ bool rim_vjp(double RB, double PB, double UB, double 
 // given) comes from the rotation and the assembly alone.
 __device__ __forceinline__ void godunov_vjp(const Prim& L, const Prim& R, double nx, double ny, const RimConst& K,
                                             int max_iter, const double* fb, double* Lb, double* Rb,
-                                            double* nb = nullptr)
+                                            double* nb = nullptr, double* fo = nullptr)
 {
 	double unl = L.u * nx + L.v * ny, unr = R.u * nx + R.v * ny;
 	double utl = L.u * ny - L.v * nx, utr = R.u * ny - R.v * nx;
@@ -833,6 +833,12 @@ __device__ __forceinline__ void godunov_vjp(const Prim& L, const Prim& R, double
 	// f1 = f0 UI + PI nx, f2 = f0 VI + PI ny, f3 = (RI (EI + (UI^2 + VI^2)/2) + PI) UN
 	double ui = UN * nx + UT * ny, vi = UN * ny - UT * nx;
 	double f0 = RI * UN, h = EI + 0.5 * (ui * ui + vi * vi), A = RI * h + PI;
+	if (fo) {  // the flux itself, as flux_godunov forms it
+		fo[0] = f0;
+		fo[1] = f0 * ui + PI * nx;
+		fo[2] = f0 * vi + PI * ny;
+		fo[3] = A * UN;
+	}
 	double f0b = fb[0] + fb[1] * ui + fb[2] * vi;
 	double uib = fb[1] * f0 + fb[3] * UN * RI * ui;
 	double vib = fb[2] * f0 + fb[3] * UN * RI * vi;
@@ -927,7 +933,7 @@ __device__ __forceinline__ void side_rev(const Prim& W, double c, double nx, dou
 // outer states' Euler fluxes, or the contact speed, star state and wave
 // speeds of the star region); nb (if given) takes the normal's adjoint
 __device__ __forceinline__ void hllc_vjp(const Prim& L, const Prim& R, double nx, double ny, double G, const double* fb,
-                                         double* Lb, double* Rb, double* nb = nullptr)
+                                         double* Lb, double* Rb, double* nb = nullptr, double* fo = nullptr)
 {
 	double unl = L.u * nx + L.v * ny, unr = R.u * nx + R.v * ny;
 	double cl = sqrt(G * L.p / L.r), cr = sqrt(G * R.p / R.r);
@@ -938,10 +944,16 @@ __device__ __forceinline__ void hllc_vjp(const Prim& L, const Prim& R, double nx
 	double sl = sl1 ? a1 : a2, sr = sr1 ? b1 : b2;
 	SideAdj al, ar;
 	double nxb = 0.0, nyb = 0.0;
-	if (sl >= 0.0) {
-		euler_flux_rev(L, unl, El, nx, ny, fb, al, nxb, nyb);
-	} else if (sr <= 0.0) {
-		euler_flux_rev(R, unr, Er, nx, ny, fb, ar, nxb, nyb);
+	if (sl >= 0.0 || sr <= 0.0) {
+		const Prim& W = sl >= 0.0 ? L : R;
+		double un = sl >= 0.0 ? unl : unr, E = sl >= 0.0 ? El : Er;
+		euler_flux_rev(W, un, E, nx, ny, fb, sl >= 0.0 ? al : ar, nxb, nyb);
+		if (fo) {  // the flux itself, as flux_hllc forms it
+			fo[0] = W.r * un;
+			fo[1] = W.r * W.u * un + W.p * nx;
+			fo[2] = W.r * W.v * un + W.p * ny;
+			fo[3] = (E + W.p) * un;
+		}
 	} else {
 		double ml = L.r * (sl - unl), mr = R.r * (sr - unr);
 		double num = R.p - L.p + L.r * unl * (sl - unl) - R.r * unr * (sr - unr), dens = ml - mr;
@@ -954,6 +966,12 @@ __device__ __forceinline__ void hllc_vjp(const Prim& L, const Prim& R, double nx
 		double d = ss - unk, e = sk - unk, he = K.r * e, h = K.p / he;
 		double B3 = ss + h, inner = Ek / K.r + d * B3;
 		double us1 = k * (K.u + d * nx), us2 = k * (K.v + d * ny), us3 = k * inner;
+		if (fo) {
+			fo[0] = K.r * unk + sk * (k - K.r);
+			fo[1] = K.r * K.u * unk + K.p * nx + sk * (us1 - K.r * K.u);
+			fo[2] = K.r * K.v * unk + K.p * ny + sk * (us2 - K.r * K.v);
+			fo[3] = (Ek + K.p) * unk + sk * (us3 - Ek);
+		}
 		// f = F(K) + sk (us - U(K))
 		double skb = fb[0] * (k - K.r) + fb[1] * (us1 - K.r * K.u) + fb[2] * (us2 - K.r * K.v) + fb[3] * (us3 - Ek);
 		double us0b = fb[0] * sk, us1b = fb[1] * sk, us2b = fb[2] * sk, us3b = fb[3] * sk;
@@ -1018,10 +1036,10 @@ __device__ __forceinline__ void hllc_vjp(const Prim& L, const Prim& R, double nx
 // with Rb = scale * tau/S * W; each lane takes one Gauss point: the flux VJP,
 // the ghost VJP on a boundary, then its (r, p, u, v) adjoints on both sides.
 // Each lane adds its boundary-parameter adjoints into its own Ebc slot.
-// GONLY (LAX with geometry terms, beside k_adj_face, which forms the (P, G)
-// and parameter adjoints): only the geometry terms, which need both sides'
-// adjoints of the face -- no Ea or Ebc; the stage's flux comes from the
-// flux VJP's own intermediates (other fluxes: from the edge pass, in b.F).
+// GEO: the geometry terms too, with the stage's flux from the flux VJP's own
+// evaluation.  GONLY (LAX with geometry terms, beside k_adj_face, which forms
+// the (P, G) and parameter adjoints): only the geometry terms, which need
+// both sides' adjoints of the face -- no Ea or Ebc.
 template <bool MM, int FLUX, bool GEO, bool GONLY = false>
 __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, double scale)
 {
@@ -1054,14 +1072,13 @@ __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, 
 		R = ghost_of<MM>(b, ph, e, c1, L, p1);
 	}
 	double Lb[4], Rb[4], nb[2] = {0.0, 0.0};
-	double Fe[4];  // GEO: the stage's flux through the edge
-	if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb, GEO ? nb : nullptr);
-	else if (FLUX == CFD2D_FLUX_GODUNOV) godunov_vjp(L, R, nx, ny, ph.rk, ph.max_newton, fb, Lb, Rb, GEO ? nb : nullptr);
-	else {
-		lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb, GEO ? nb : nullptr, GONLY ? Fe : nullptr);
-		if (GONLY)  // the stage's flux F_e = (f_1 + f_2) l/2, as the edge pass forms it
-			for (int q = 0; q < 4; q++) Fe[q] = (Fe[q] + pair_swap(Fe[q])) * hl;
-	}
+	double Fe[4];  // GEO: the stage's flux through the edge, from the VJP's own evaluation
+	if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb, GEO ? nb : nullptr, GEO ? Fe : nullptr);
+	else if (FLUX == CFD2D_FLUX_GODUNOV)
+		godunov_vjp(L, R, nx, ny, ph.rk, ph.max_newton, fb, Lb, Rb, GEO ? nb : nullptr, GEO ? Fe : nullptr);
+	else lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb, GEO ? nb : nullptr, GEO ? Fe : nullptr);
+	if (GEO)  // F_e = (f_1 + f_2) l/2, as the edge pass forms it
+		for (int q = 0; q < 4; q++) Fe[q] = (Fe[q] + pair_swap(Fe[q])) * hl;
 	double Rsb[4];  // adjoint of c2's reconstructed state (inner edges)
 	if (c2 < 0) {
 		// the ghost's inputs: the reconstructed L, the cell centre, the parameters
@@ -1110,7 +1127,6 @@ __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, 
 		// the stage's flux enters R_c1 with -, R_c2 with + (F_e = (f_1 + f_2) l/2):
 		// each lane gives its side's W . F_e for the area term, and the first
 		// the length adjoint F_e . (Rb_c2 - Rb_c1) / l = F_e . F_e-bar / (l l/2)
-		if (!GONLY) ld4(b.F, e, Fe);
 		double wv[4] = {0.0, 0.0, 0.0, 0.0};  // this lane's side's W, loaded again (a cache hit)
 		int cw = gp ? c2 : c1;
 		if (cw >= 0) ld4(a.Wn, cw, wv);
@@ -1613,15 +1629,11 @@ void launch_adj_grad(cfd2d_ctx* c, AdjWork* A, double wa, double wb, int mode)
 	else hipLaunchKernelGGL((k_adj_grad<MM, LSQ, 2, GEO>), g, blk, 0, c->st, c->b, c->ph, A->a, wa, wb);
 }
 
-// GEO: geometry adjoints too; with HLLC or Godunov the stage's forward
-// fluxes are formed again first (edge pass on the recorded P and G, into the
-// context's flux buffer) for the length and area terms
+// GEO: geometry adjoints too (the length and area terms take the stage's
+// fluxes from the edge reverse's own flux evaluation)
 template <bool MM, bool GEO>
 void launch_adj_stage(cfd2d_ctx* c, AdjWork* A, double wa, double wb, int mode)
 {
-	// the geometry terms read the stage's fluxes: for LAX the geometry-only
-	// edge reverse forms them, the other fluxes get them from the edge pass
-	if (GEO && c->p.flux != CFD2D_FLUX_LAX) launch_edge<MM>(c);
 	if (c->p.flux == CFD2D_FLUX_HLLC || c->p.flux == CFD2D_FLUX_GODUNOV) {
 		// VJPs heavy in VALU work: once per edge, through the edge buffer
 		if (c->p.flux == CFD2D_FLUX_HLLC)
