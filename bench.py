@@ -8,7 +8,7 @@ exact Godunov flux, SSP-RK3, CFL 0.4 with the spectral time step fixed at
 init.  A step = one full time step (three RK stages) over all cells; the
 metric is cell-updates/s with the state resident in HBM.
 
-Prints one JSON line.  --gpus above 1 exits: this build targets one GPU.
+Prints one JSON line.  --gpus above 1 exits: the library runs on one GPU.
 """
 import argparse
 import json
@@ -121,7 +121,7 @@ def reference_cpu(steps=100, opt="O2"):
 
 
 def batched_vs_serial(n_cases, steps, seed=7):
-    """Config-3 style batch: n_cases bump channels (2k-60k cells, log-uniform,
+    """Batched bump channels: n_cases channels (2k-60k cells, log-uniform,
     inflow Mach 0.2-3.0, angle -5..5 deg; Euler, exact Godunov, minmod, RK2)
     advanced together in one index space, against the same cases advanced one
     at a time (eager launches, and replayed HIP graphs)."""
@@ -406,7 +406,7 @@ def main():
     ap.add_argument("--no-implicit", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        print("bench.py: the library runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     from cfd2d import case as cases

@@ -573,7 +573,7 @@ Method* initMethod(char* fileName)
 	if (!strcmp(methodName, "FVM_TVD") || !strcmp(methodName, "FVM_TVD_GPU")) {
 		m = new FVM_TVD();
 	} else {
-		log("ERROR: method '%s' is not provided by this build (only FVM_TVD).\n", methodName);
+		log("ERROR: method '%s' is not provided by this library (only FVM_TVD).\n", methodName);
 		exit(1);
 	}
 	try {

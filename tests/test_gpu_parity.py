@@ -72,7 +72,7 @@ def residual_scale(cell_edges, edgeflux):
 
 
 def test_residual_1e12(golden):
-    """The north-star residual gate: per-cell residual within 1e-12 relative,
+    """The one-step residual gate: per-cell residual within 1e-12 relative,
     |R_gpu - R_ref| / max(|R_ref|, sum_f |F_f|_inf l_f/2) <= 1e-12."""
     name, case, g = golden
     s = native.GpuSolver(case)
