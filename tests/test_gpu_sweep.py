@@ -1,0 +1,109 @@
+"""Seeded sweep over the scheme's option space, GPU against the oracle.
+
+The other parity tests fix one combination each.  Here a seeded generator
+draws 60 cases from the product of flux (Godunov, LAX, HLLC, Roe), gradient
+(Green-Gauss, least squares), limiter (none, minmod), RK order (2, 3), time
+step rule (reference, spectral), CFL, mesh family (mild tube, full Sod tube,
+bump channel with either outlet type and sub- to supersonic inflow, viscous
+periodic square), where the gradients are formed (separate pass, edge pass,
+tiled edge pass) and whether the update pass streams the state.  Each case
+runs 15 steps on both sides:
+- the one-step residual within 1e-11 and the state within 1e-10 relative
+  per component, flags equal;
+- where the scheme breaks down (unlimited schemes across the full Sod jump),
+  the same cells' residuals are non-finite on both sides, and the GPU's
+  error report must match the oracle: non-finite states on both sides, or
+  the same number of Newton iterations that hit the cap.
+"""
+import numpy as np
+import pytest
+
+from cfd2d import case as cases
+from cfd2d import native
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 15
+ERR_ARG, ERR_NEWTON, ERR_NONFINITE = 1, 3, 4  # CFD2D_ERR_ARG, CFD2D_ERR_NEWTON, CFD2D_ERR_NONFINITE
+
+
+def _rel(a, b):
+    den = np.max(np.abs(b), axis=1)
+    den = np.where(den == 0, 1.0, den)
+    return np.max(np.abs(a - b), axis=1) / den
+
+
+def _draw(rng):
+    kind = str(rng.choice(["tube_mild", "tube_sod", "bump", "periodic"]))
+    opts = dict(flux=str(rng.choice(["GODUNOV", "LAX", "HLLC", "ROE"])),
+                gradient=str(rng.choice(["GREEN_GAUSS", "LSQ"])),
+                limiter=str(rng.choice(["NONE", "MINMOD"])),
+                rk_stages=int(rng.choice([2, 3])),
+                cfl_mode=str(rng.choice(["REFERENCE", "SPECTRAL"])),
+                cfl=float(rng.choice([0.3, 0.45])))
+    seed = int(rng.integers(1, 1000))
+    if kind in ("tube_mild", "tube_sod"):
+        c = cases.shock_tube(int(rng.integers(10, 40)), int(rng.integers(3, 8)), seed=seed, mild=kind == "tube_mild",
+                             **opts)
+    elif kind == "bump":
+        c = cases.bump_channel(int(rng.integers(6, 14)), height=float(rng.uniform(0.02, 0.1)),
+                               mach=float(rng.uniform(0.3, 2.5)), angle_deg=float(rng.uniform(-5, 5)), seed=seed,
+                               outlet=str(rng.choice(["BOUND_OUTLET", "BOUND_PRESSURE_OUTLET"])), **opts)
+    else:
+        c = cases.periodic_pulse(int(rng.integers(8, 20)), seed=seed, **opts)
+    fuse = [None, 0, 1, 2][int(rng.integers(0, 4))]
+    stream = bool(rng.integers(0, 2))
+    return kind, opts, c, fuse, stream
+
+
+def test_seeded_option_sweep_matches_oracle():
+    rng = np.random.default_rng(1)
+    counts = dict(matched=0, nonfinite=0, newton_cap=0)
+    worst = 0.0
+    for it in range(60):
+        kind, opts, c, fuse, stream = _draw(rng)
+        label = "case %d: %s %s fuse=%s stream=%s cells=%d" % (it, kind, opts, fuse, stream, c.mesh.n_cells)
+        print(label, flush=True)
+        s = native.GpuSolver(c)
+        try:
+            try:
+                s.ctx.fuse_gradient(fuse)
+            except native.Cfd2dError as ex:  # no tiled edge pass for this mesh
+                assert fuse == 2 and ex.code == ERR_ARG, label
+                s.ctx.fuse_gradient(None)
+            s.ctx.stream_state(stream)
+            o = oracle.OracleSolver(c)
+            # an unlimited reconstruction across the full Sod jump reaches negative
+            # states at some Gauss points: those residuals are non-finite on both sides
+            Rs, Ro = s.residual(), o.residual()
+            fin = np.isfinite(Ro)
+            assert np.array_equal(np.isfinite(Rs), fin), label
+            assert _rel(np.where(fin, Rs, 0.0), np.where(fin, Ro, 0.0)).max() <= 1e-11, label
+            try:
+                s.run(STEPS)
+                failed = None
+            except native.Cfd2dError as ex:
+                failed = ex
+            o.run(STEPS)
+            finite_o = np.isfinite(o.state()).all()
+            if failed is None:
+                assert finite_o, label
+                err = _rel(s.state(), o.state()).max()
+                assert err <= 1e-10, (label, err)
+                assert np.array_equal(s.flags(), o.flags()), label
+                assert s.status() == (0, o.newton_cap_hits()), label
+                worst = max(worst, err)
+                counts["matched"] += 1
+            elif failed.code == ERR_NEWTON:
+                assert s.status()[0] == o.newton_cap_hits() > 0, label
+                if finite_o:
+                    assert _rel(s.state(), o.state()).max() <= 1e-10, label
+                counts["newton_cap"] += 1
+            else:
+                assert failed.code == ERR_NONFINITE and not finite_o, (label, failed)
+                counts["nonfinite"] += 1
+        finally:
+            s.close()
+    print("sweep:", counts, "worst state error %.3e" % worst)
+    assert counts["matched"] >= 45
