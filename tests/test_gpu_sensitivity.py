@@ -363,3 +363,64 @@ def test_adjoint_recording_is_the_same_for_every_fuse_mode(flux, gradient, rk):
         assert J == res[0][0]
         assert np.array_equal(gU, res[0][1])
         assert np.array_equal(gb, res[0][2])
+
+
+def _draw_adjoint_case(rng, tube):
+    flux = str(rng.choice(["LAX", "HLLC", "GODUNOV"]))
+    gradient = str(rng.choice(["GREEN_GAUSS", "LSQ"]))
+    rk = int(rng.choice([2, 3]))
+    if not tube:
+        c = cases.bump_channel(int(rng.integers(4, 8)), height=float(rng.uniform(0.02, 0.08)),
+                               mach=float(rng.uniform(0.3, 2.0)), angle_deg=float(rng.uniform(-5, 5)),
+                               seed=int(rng.integers(1, 1000)),
+                               outlet=str(rng.choice(["BOUND_OUTLET", "BOUND_PRESSURE_OUTLET"])), flux=flux,
+                               gradient=gradient, limiter="NONE", rk_stages=rk)
+        kind = "bump"
+    else:
+        c = cases.shock_tube(int(rng.integers(8, 16)), int(rng.integers(3, 6)), seed=int(rng.integers(1, 1000)),
+                             mild=True, flux=flux, gradient=gradient, limiter="NONE", rk_stages=rk)
+        kind = "tube"
+    return kind, c, int(rng.choice([0, 1, 3, 5]))
+
+
+def test_seeded_adjoint_sweep_matches_tangent():
+    """Seeded draws beyond the fixed list: LAX, HLLC or Godunov; Green-Gauss
+    or LSQ; RK2 or RK3; bump channels with either outlet type and inflow
+    Mach 0.3-2.0 at an angle, or (every third case) the mild shock tube (slip
+    and no-slip walls, outlets); checkpoint intervals 0 (every stage
+    recorded), 1, 3 or 5; the start state perturbed by 1e-3.  J within 1e-12
+    and <dJ/dU0, v> + <dJ/dpar, w> within 1e-9 of the tangent model, as in
+    the fixed list."""
+    rng = np.random.default_rng(5)
+    for it in range(15):
+        kind, c, ck = _draw_adjoint_case(rng, tube=it % 3 == 2)
+        cfg = c.config
+        label = "case %d: %s %s %s RK%d ckpt %d cells %d" % (it, kind, cfg.flux, cfg.gradient, cfg.rk_stages, ck,
+                                                          c.mesh.n_cells)
+        print(label, flush=True)
+        s = oracle.OracleSolver(c)
+        tau, _ = s.tau()
+        eg = s.geometry["edge_group"]
+        m = c.mesh
+        names = ["bump", "wall_bottom"] if kind == "bump" else ["edge_bottom"]  # J of order 1, no cancellation
+        mask = np.isin(eg, [m.group_index(nm) for nm in names]).astype(np.int32)
+        n = 10
+        # a uniform start puts the two sides' wave speeds of every face at a tie
+        # (LAX alpha, the HLLC bounds), where the map has no unique derivative and
+        # the device's contracted reconstruction and the model can differ in the
+        # last bit that breaks it: start from a seeded perturbation instead
+        U0 = c.initial_state()
+        U0 = U0 * (1.0 + 1e-3 * rng.standard_normal(U0.shape))
+        U0[1:3] += 1e-3 * U0[0] * rng.standard_normal(U0[1:3].shape)
+        g = native.GpuSolver(c)
+        g.set_tau(tau)
+        g.set_state(U0)
+        J, gU, gb = g.adjoint(n, mask, ckpt_every=ck)
+        g.close()
+        v = rng.standard_normal(U0.shape) * np.abs(U0) * 1e-2
+        tm = tangent.Model(s)
+        w = rng.standard_normal(tm.bcpar0.shape) * np.abs(tm.bcpar0) * 1e-2
+        Jm, dJ = tangent.tangent(s, U0, v, w, n, tau, mask)
+        assert abs(J - Jm) <= 1e-12 * abs(Jm), label
+        ad = float(np.sum(gU * v)) + float(np.sum(gb * w))
+        assert abs(ad - dJ) <= 1e-9 * abs(dJ), (label, ad, dJ)
