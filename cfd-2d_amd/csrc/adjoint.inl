@@ -1152,6 +1152,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_edge(Bufs b, Phys ph, AdjBufs a, 
 template <bool MM, int FLUX>
 __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, double scale)
 {
+	static_assert(FLUX == CFD2D_FLUX_LAX, "the one-sided face VJP holds for the LAX flux only");
 	int t = xcd_block() * BLOCK + threadIdx.x;
 	int i = t >> 1, gp = t & 1;
 	if (i >= b.nc) return;  // both lanes of a pair leave together
@@ -1209,27 +1210,24 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face(Bufs b, Phys ph, AdjBufs a, 
 			R = ghost_of<MM>(b, ph, e, i, L, pi);
 		}
 		double Ob[4];  // adjoint of this cell's reconstructed state
-		if (FLUX == CFD2D_FLUX_LAX && j >= 0) {
+		if (j >= 0) {
 			// an inner face: this side's adjoint only
 			double sg = side ? -1.0 : 1.0, fs[4];
 			for (int q = 0; q < 4; q++) fs[q] = sg * fb[q];
 			lax_vjp_own(own, side ? L : R, sg * nx, sg * ny, ph.gam_flux, ph.igm1, fs, side != 0, Ob);
 		} else {
+			// a boundary face (always the cell's own side 0): both sides, the
+			// ghost's adjoint taken back to the cell through the ghost map
 			double Lb[4], Rb[4];
-			if (FLUX == CFD2D_FLUX_HLLC) hllc_vjp(L, R, nx, ny, ph.gam_flux, fb, Lb, Rb);
-			else lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb);
-			if (j < 0) {
-				int bc = -edge_cells(b, e).y - 1;
-				double dL[4], dC[4], dpar[4];
-				ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], nx, ny, L, pi, mat_of<MM>(b, ph, i), Rb, dL, dC, dpar,
-				          nullptr);
-				for (int q = 0; q < 4; q++) {
-					Lb[q] += dL[q];
-					pb[q] += dC[q];
-					a.Ebc[(size_t)(4 * gp + q) * ne + e] += dpar[q];
-				}
+			lax_vjp(L, R, nx, ny, ph.gam_flux, ph.igm1, fb, Lb, Rb);
+			int bc = -edge_cells(b, e).y - 1;
+			double dL[4], dC[4], dpar[4];
+			ghost_vjp(b.bctype[bc], &b.bcpar[4 * bc], nx, ny, L, pi, mat_of<MM>(b, ph, i), Rb, dL, dC, dpar, nullptr);
+			for (int q = 0; q < 4; q++) {
+				Ob[q] = Lb[q] + dL[q];
+				pb[q] += dC[q];
+				a.Ebc[(size_t)(4 * gp + q) * ne + e] += dpar[q];
 			}
-			for (int q = 0; q < 4; q++) Ob[q] = side ? Rb[q] : Lb[q];
 		}
 		for (int q = 0; q < 4; q++) {
 			double o = Ob[q];
