@@ -219,7 +219,8 @@ def _bump_nodes(c):
 
 
 @pytest.mark.parametrize("flux,gradient,rk", [("LAX", "GREEN_GAUSS", 2), ("HLLC", "LSQ", 3), ("LAX", "LSQ", 2),
-                                              ("HLLC", "GREEN_GAUSS", 3), ("GODUNOV", "GREEN_GAUSS", 2)])
+                                              ("HLLC", "GREEN_GAUSS", 3), ("GODUNOV", "GREEN_GAUSS", 2),
+                                              ("GODUNOV", "LSQ", 3)])
 def test_geometry_adjoint_matches_tangent(flux, gradient, rk):
     """cfd2d_adjoint_geometry: <dJ/dxy, dxy> equals the tangent model's dJ[dxy]
     with the geometry rebuilt from the nodes (every node moved) to 1e-9, and
