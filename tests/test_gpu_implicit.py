@@ -161,3 +161,34 @@ def test_implicit_run_with_device_assembly_matches_oracle_assembly(flux, steady)
         den = np.abs(h.U).max(axis=1)
         assert np.all(np.abs(g.U - h.U).max(axis=1) <= 1e-10 * den)
     g.close()
+
+
+def test_seeded_assembly_sweep_matches_oracle():
+    """Seeded draws beyond the fixed cases: the mild tube or a bump channel
+    (outlet, inflow Mach 0.3-2.5 at an angle), Godunov or LAX, steady or
+    unsteady, CFL 0.3-1.0 and tau 1e-3-5e-3, each from a seeded 1e-2
+    perturbation of its state (set on both sides); the fill within the
+    tolerances above."""
+    rng = np.random.default_rng(9)
+    for it in range(20):
+        flux = str(rng.choice(["GODUNOV", "LAX"]))
+        if it % 2:
+            case = cases.shock_tube(int(rng.integers(8, 20)), int(rng.integers(3, 6)), seed=int(rng.integers(1, 999)),
+                                    mild=True, flux=flux, limiter="NONE", max_newton=200)
+        else:
+            case = cases.bump_channel(int(rng.integers(5, 12)), height=float(rng.uniform(0.02, 0.1)),
+                                      mach=float(rng.uniform(0.3, 2.5)), angle_deg=float(rng.uniform(-5, 5)),
+                                      seed=int(rng.integers(1, 999)), outlet="BOUND_OUTLET", flux=flux,
+                                      limiter="NONE", max_newton=200)
+        steady = bool(rng.integers(0, 2))
+        cfl, tau = float(rng.uniform(0.3, 1.0)), float(rng.uniform(1e-3, 5e-3))
+        print("case %d: %s cells %d steady %s cfl %.3f tau %.2e" % (it, flux, case.mesh.n_cells, steady, cfl, tau),
+              flush=True)
+        s = native.GpuSolver(case)
+        o = oracle.OracleSolver(case)
+        U = s.state()
+        U = U * (1.0 + 1e-2 * rng.standard_normal(U.shape))
+        s.set_state(U)
+        o.set_state(U)
+        compare(s, o, steady, cfl=cfl, tau=tau)
+        s.close()
